@@ -1,0 +1,207 @@
+"""dmrgpp_amd: MI355X-native superblock Kron matvec for DMRG++ (Python host mirror).
+
+The product is the C-ABI library ``libdmrgpp_amd.so`` built from ``dmrgpp_amd/csrc`` (HIP for
+gfx950). This module loads it with ctypes and mirrors the reference plugin interface
+``BatchedGemm<T>`` (dmrg/GPUPlugin/BatchedGemm.h:11-56): construct from the patch sizes and the
+A/B block pointers, then ``apply_Htarget(vin, vout)`` computes vout = H vin.
+
+There is no CPU fallback: if the library is missing or no GPU is visible the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .problem import KronProblem, PluginArgs  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdmrgpp_amd.so")
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+PREC_BF16X3 = 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+
+class DmkError(RuntimeError):
+    pass
+
+
+class DmkInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64), ("npatches", ctypes.c_int64), ("noperator", ctypes.c_int64),
+        ("nterms", ctypes.c_int64), ("nterms_identity", ctypes.c_int64),
+        ("stage1_tasks", ctypes.c_int64), ("stage2_tasks", ctypes.c_int64),
+        ("stage1_tiles", ctypes.c_int64), ("stage2_tiles", ctypes.c_int64),
+        ("max_patch_left", ctypes.c_int64), ("max_patch_right", ctypes.c_int64),
+        ("apack_elems", ctypes.c_int64), ("bpack_elems", ctypes.c_int64),
+        ("bx_elems", ctypes.c_int64),
+        ("flops_algorithmic", ctypes.c_double), ("flops_plugin", ctypes.c_double),
+        ("flops_executed", ctypes.c_double), ("flops_stage1", ctypes.c_double),
+        ("flops_stage2", ctypes.c_double), ("bytes_compulsory", ctypes.c_double),
+        ("bytes_bx", ctypes.c_double), ("precision", ctypes.c_int32), ("pad", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
+_lib = None
+
+# Every symbol include/dmrgpp_amd.h declares, with its ctypes signature.
+_VP = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_I = ctypes.c_int
+_D = ctypes.c_double
+SIGNATURES = {
+    "dmk_create_d": (_I, [_SZ, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(_VP)]),
+    "dmk_destroy": (None, [_VP]),
+    "dmk_apply_d": (_I, [_VP, _VP, _VP]),
+    "dmk_apply_host_d": (_I, [_VP, _VP, _VP]),
+    "dmk_apply_multi_d": (_I, [_VP, _I, _VP, _SZ, _VP, _SZ, _I]),
+    "dmk_set_precision": (_I, [_VP, _I]),
+    "dmk_get_info": (_I, [_VP, ctypes.POINTER(DmkInfo)]),
+    "dmk_sync": (_I, [_VP]),
+    "dmk_profile_apply_d": (_I, [_VP, _VP, _VP, _I, ctypes.POINTER(_D), ctypes.POINTER(_D),
+                                 ctypes.POINTER(_D)]),
+    "dmk_device_alloc": (_I, [_SZ, ctypes.POINTER(_VP)]),
+    "dmk_device_free": (_I, [_VP]),
+    "dmk_memcpy_h2d": (_I, [_VP, _VP, _SZ]),
+    "dmk_memcpy_d2h": (_I, [_VP, _VP, _SZ]),
+    "dmk_device_count": (_I, [ctypes.POINTER(_I)]),
+    "dmk_last_error": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Loads libdmrgpp_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DmkError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "or `make -C dmrgpp_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int, what: str):
+    if status != 0:
+        msg = lib().dmk_last_error()
+        raise DmkError(f"{what} failed ({status}): {msg.decode() if msg else ''}")
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _check(lib().dmk_device_count(ctypes.byref(c)), "dmk_device_count")
+    return c.value
+
+
+class DeviceArray:
+    """A device allocation of ``size`` elements of ``dtype`` owned through the C ABI."""
+
+    def __init__(self, size: int, dtype=np.float64):
+        self.size = int(size)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().dmk_device_alloc(self.nbytes, ctypes.byref(p)), "dmk_device_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size <= self.size
+        _check(lib().dmk_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes), "dmk_memcpy_h2d")
+
+    def download(self, count: int | None = None) -> np.ndarray:
+        count = self.size if count is None else int(count)
+        out = np.empty(count, dtype=self.dtype)
+        _check(lib().dmk_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "dmk_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(lib().dmk_device_free(self.ptr), "dmk_device_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BatchedGemm:
+    """Mirror of ``BatchedGemm<double>`` (dmrg/GPUPlugin/BatchedGemm.h:11-56) over the C ABI."""
+
+    def __init__(self, prob: KronProblem, precision: str | int = "fp64"):
+        if prob.dtype != np.float64:
+            raise DmkError("complex problems are not supported by the GPU path yet")
+        args = prob.plugin_args()
+        h = ctypes.c_void_p()
+        _check(lib().dmk_create_d(args.noperator, args.npatches, ctypes.addressof(args.L),
+                                  ctypes.addressof(args.R), ctypes.addressof(args.A),
+                                  ctypes.addressof(args.ldA), ctypes.addressof(args.B),
+                                  ctypes.addressof(args.ldB), ctypes.byref(h)), "dmk_create_d")
+        self.handle = h.value
+        self.n = prob.n
+        self.set_precision(precision)
+
+    def set_precision(self, precision: str | int):
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        _check(lib().dmk_set_precision(self.handle, p), "dmk_set_precision")
+        self.precision = p
+
+    def apply_Htarget(self, vin: DeviceArray, vout: DeviceArray):
+        """vout = H vin on device arrays (BatchedGemm.h:30-31)."""
+        _check(lib().dmk_apply_d(self.handle, vin.ptr, vout.ptr), "dmk_apply_d")
+
+    def apply_multi(self, vin: DeviceArray, vout: DeviceArray, nvec: int, beta: int = 0):
+        _check(lib().dmk_apply_multi_d(self.handle, nvec, vin.ptr, self.n, vout.ptr, self.n, beta),
+               "dmk_apply_multi_d")
+
+    def apply_host(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == self.n
+        y = np.empty_like(x)
+        _check(lib().dmk_apply_host_d(self.handle, x.ctypes.data, y.ctypes.data),
+               "dmk_apply_host_d")
+        return y
+
+    def sync(self):
+        _check(lib().dmk_sync(self.handle), "dmk_sync")
+
+    def info(self) -> dict:
+        inf = DmkInfo()
+        _check(lib().dmk_get_info(self.handle, ctypes.byref(inf)), "dmk_get_info")
+        return inf.as_dict()
+
+    def profile(self, vin: DeviceArray, vout: DeviceArray, iters: int) -> tuple:
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().dmk_profile_apply_d(self.handle, vin.ptr, vout.ptr, iters, ctypes.byref(a),
+                                         ctypes.byref(b), ctypes.byref(c)), "dmk_profile_apply_d")
+        return a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().dmk_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,237 @@
+// extern "C" layer of dmrgpp_amd (declarations and reference citations in include/dmrgpp_amd.h).
+#include "dmrgpp_amd.h"
+#include "kron_plan.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+struct dmk_handle {
+	std::unique_ptr<dmk::Plan> plan;
+};
+
+namespace dmk {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+const char* last_error() { return g_last_error.c_str(); }
+
+} // namespace dmk
+
+namespace {
+
+template <typename F> int guarded(F&& f)
+{
+	try {
+		return f();
+	} catch (const std::invalid_argument& e) {
+		dmk::set_last_error(e.what());
+		return DMK_ERR_ARGUMENT;
+	} catch (const std::runtime_error& e) {
+		dmk::set_last_error(e.what());
+		const std::string w = e.what();
+		if (w.find("hip") != std::string::npos)
+			return DMK_ERR_DEVICE;
+		if (w.find("unsupported") != std::string::npos)
+			return DMK_ERR_UNSUPPORTED;
+		return DMK_ERR_ARGUMENT;
+	} catch (const std::exception& e) {
+		dmk::set_last_error(e.what());
+		return DMK_ERR_DEVICE;
+	}
+}
+
+dmk::Plan& plan_of(dmk_handle* h)
+{
+	if (!h || !h->plan)
+		throw std::invalid_argument("null dmk handle");
+	return *h->plan;
+}
+
+} // namespace
+
+extern "C" {
+
+int dmk_create_d(size_t noperator, size_t npatches, const size_t* left_patch_size,
+                 const size_t* right_patch_size, const double* const* Amatrix,
+                 const size_t* ld_Amatrix, const double* const* Bmatrix, const size_t* ld_Bmatrix,
+                 dmk_handle** handle)
+{
+	return guarded([&] {
+		if (!handle)
+			throw std::invalid_argument("dmk_create_d: null handle pointer");
+		*handle = nullptr;
+		auto h  = std::make_unique<dmk_handle>();
+		h->plan = std::make_unique<dmk::Plan>(int64_t(noperator), int64_t(npatches),
+		                                      left_patch_size, right_patch_size, Amatrix,
+		                                      ld_Amatrix, Bmatrix, ld_Bmatrix);
+		*handle = h.release();
+		return DMK_OK;
+	});
+}
+
+void dmk_destroy(dmk_handle* handle) { delete handle; }
+
+int dmk_apply_d(dmk_handle* handle, const double* vin, double* vout)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		p.apply(vin, p.n(), vout, p.n(), 1, 0);
+		return DMK_OK;
+	});
+}
+
+int dmk_apply_host_d(dmk_handle* handle, const double* vin, double* vout)
+{
+	return guarded([&] {
+		dmk::Plan&   p     = plan_of(handle);
+		const size_t bytes = size_t(p.n()) * sizeof(double);
+		double*      dx    = nullptr;
+		double*      dy    = nullptr;
+		DMK_HIP_CHECK(hipMalloc(&dx, bytes));
+		try {
+			DMK_HIP_CHECK(hipMalloc(&dy, bytes));
+			DMK_HIP_CHECK(hipMemcpyAsync(dx, vin, bytes, hipMemcpyHostToDevice, p.stream()));
+			p.apply(dx, p.n(), dy, p.n(), 1, 0);
+			DMK_HIP_CHECK(hipMemcpyAsync(vout, dy, bytes, hipMemcpyDeviceToHost, p.stream()));
+			DMK_HIP_CHECK(hipStreamSynchronize(p.stream()));
+		} catch (...) {
+			(void)hipStreamSynchronize(p.stream());
+			(void)hipFree(dx);
+			(void)hipFree(dy);
+			throw;
+		}
+		(void)hipFree(dx);
+		(void)hipFree(dy);
+		return DMK_OK;
+	});
+}
+
+int dmk_apply_multi_d(dmk_handle* handle, int nvec, const double* vin, size_t ldin, double* vout,
+                      size_t ldout, int beta)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (beta != 0 && beta != 1)
+			throw std::invalid_argument("dmk_apply_multi_d: beta must be 0 or 1");
+		p.apply(vin, int64_t(ldin), vout, int64_t(ldout), nvec, beta);
+		return DMK_OK;
+	});
+}
+
+int dmk_set_precision(dmk_handle* handle, int precision)
+{
+	return guarded([&] {
+		plan_of(handle).set_precision(precision);
+		return DMK_OK;
+	});
+}
+
+int dmk_get_info(const dmk_handle* handle, dmk_info* info)
+{
+	return guarded([&] {
+		if (!handle || !handle->plan || !info)
+			throw std::invalid_argument("dmk_get_info: null argument");
+		const dmk::PlanStats& s = handle->plan->stats();
+		std::memset(info, 0, sizeof(*info));
+		info->n                 = s.n;
+		info->npatches          = s.npatches;
+		info->noperator         = s.noperator;
+		info->nterms            = s.nterms;
+		info->nterms_identity   = s.nterms_identity;
+		info->stage1_tasks      = s.stage1_tasks;
+		info->stage2_tasks      = s.stage2_tasks;
+		info->stage1_tiles      = s.stage1_tiles;
+		info->stage2_tiles      = s.stage2_tiles;
+		info->max_patch_left    = s.max_patch_left;
+		info->max_patch_right   = s.max_patch_right;
+		info->apack_elems       = s.apack_elems;
+		info->bpack_elems       = s.bpack_elems;
+		info->bx_elems          = s.bx_elems;
+		info->flops_algorithmic = s.flops_algorithmic;
+		info->flops_plugin      = s.flops_plugin;
+		info->flops_executed    = s.flops_executed;
+		info->flops_stage1      = s.flops_stage1;
+		info->flops_stage2      = s.flops_stage2;
+		info->bytes_compulsory  = s.bytes_compulsory;
+		info->bytes_bx          = s.bytes_bx;
+		info->precision         = handle->plan->precision();
+		return DMK_OK;
+	});
+}
+
+int dmk_sync(dmk_handle* handle)
+{
+	return guarded([&] {
+		DMK_HIP_CHECK(hipStreamSynchronize(plan_of(handle).stream()));
+		return DMK_OK;
+	});
+}
+
+int dmk_profile_apply_d(dmk_handle* handle, const double* vin, double* vout, int iters,
+                        double* ms_stage1, double* ms_stage2, double* ms_total)
+{
+	return guarded([&] {
+		plan_of(handle).profile(vin, vout, iters, ms_stage1, ms_stage2, ms_total);
+		return DMK_OK;
+	});
+}
+
+int dmk_device_alloc(size_t bytes, void** ptr)
+{
+	return guarded([&] {
+		if (!ptr)
+			throw std::invalid_argument("dmk_device_alloc: null pointer");
+		*ptr = nullptr;
+		DMK_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+		return DMK_OK;
+	});
+}
+
+int dmk_device_free(void* ptr)
+{
+	return guarded([&] {
+		DMK_HIP_CHECK(hipFree(ptr));
+		return DMK_OK;
+	});
+}
+
+int dmk_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+	return guarded([&] {
+		DMK_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+		return DMK_OK;
+	});
+}
+
+int dmk_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+	return guarded([&] {
+		DMK_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+		return DMK_OK;
+	});
+}
+
+int dmk_device_count(int* count)
+{
+	return guarded([&] {
+		if (!count)
+			throw std::invalid_argument("dmk_device_count: null pointer");
+		*count = 0;
+		hipError_t e = hipGetDeviceCount(count);
+		if (e != hipSuccess)
+			*count = 0;
+		return DMK_OK;
+	});
+}
+
+const char* dmk_last_error(void) { return dmk::last_error(); }
+
+} // extern "C"

@@ -1,0 +1,72 @@
+// Internal descriptors shared by the host planner (kron_plan.cpp) and the HIP kernels
+// (kron_kernels.hip). Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace dmk {
+
+// Operand bases. A segment operand is (base id, element offset); the base table is filled per
+// launch, so the same descriptors serve every call whatever vin/vout the caller passes.
+enum BaseId : int32_t
+{
+	BASE_APACK = 0, // packed left-block factors (A panels, identity-free)
+	BASE_BPACK = 1, // packed right-block factors (row-stacked B blocks per (I,J))
+	BASE_X     = 2, // input vector (patch order) in the compute precision
+	BASE_BX    = 3, // stage-1 products B*X_J, laid out as the stage-2 panels
+	BASE_COUNT = 4
+};
+
+// One contribution C(MxN) += L(MxK) * R(KxN) to a task. L is column major (ldl). R is column
+// major K x N (ldr) when rtrans == 0, or column major N x K used transposed when rtrans == 1.
+struct Seg {
+	int64_t loff;
+	int64_t roff;
+	int32_t lbase;
+	int32_t rbase;
+	int32_t ldl;
+	int32_t ldr;
+	int32_t k;
+	int32_t rtrans;
+};
+
+// One output matrix. Element (r, c) of the MxN result is stored at
+//   C[(r % rsplit) + ((r / rsplit) * N + c) * ldc]
+// which is plain column-major storage when rsplit == M; stage 1 uses rsplit = R_I to lay the
+// row-stacked products B_k*X_J side by side as the stage-2 panel columns.
+struct Task {
+	int64_t coff;      // element offset of C in its output buffer
+	int32_t m;
+	int32_t n;
+	int32_t ldc;
+	int32_t rsplit;
+	int32_t seg_begin;
+	int32_t seg_end;
+	int32_t out_is_y;  // 1: C lives in the caller's output vector, 0: in the BX scratch
+	int32_t pad;
+};
+
+// One output tile of a task; the tile list is ordered by decreasing task cost.
+struct Tile {
+	int32_t task;
+	int32_t m0;
+	int32_t n0;
+	int32_t pad;
+};
+
+// Error reporting for the C ABI.
+void        set_last_error(const std::string& msg);
+const char* last_error();
+
+} // namespace dmk
+
+#define DMK_HIP_CHECK(expr)                                                                        \
+	do {                                                                                       \
+		hipError_t _e = (expr);                                                            \
+		if (_e != hipSuccess) {                                                            \
+			throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+		}                                                                                  \
+	} while (0)

@@ -1,0 +1,285 @@
+// Grouped multi-segment GEMM for CDNA4 (gfx950) on MFMA, and the small conversion kernels
+// around it.
+//
+// One launch computes a list of independent output matrices ("tasks"), each the sum of one or
+// more products ("segments") C += L * op(R). The superblock matvec uses two launches of it:
+//   stage 1: for every (I,J) with general terms, [B_k1; B_k2; ...] * X_J (row-stacked B blocks),
+//            written side by side into the stage-2 panel BX_I;
+//   stage 2: Y_I = BX_I * A_I^T  +  X_I * H_L^T (identity right factor)  +  H_R * X_I (identity
+//            left factor).
+// This is the two-stage evaluation of GPUPlugin/apply_Htarget_sparse_not_ready.cpp:199-449 with
+// the identity factors of H_L (x) I and I (x) H_R contracted directly instead of as dense GEMMs.
+//
+// Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a 64x64 output tile and each
+// wave a 32x32 sub-tile held as 2x2 MFMA 16x16 accumulators. K is walked in chunks of 16 staged
+// through LDS, double buffered: the global loads of chunk c+1 are issued before the MFMAs of
+// chunk c. Every output tile is owned by exactly one workgroup and its contraction order is
+// fixed, so results are bitwise reproducible (no atomics, no split-k).
+#include "dmk_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace dmk {
+
+constexpr int kBM      = 64;
+constexpr int kBN      = 64;
+constexpr int kBK      = 16;
+constexpr int kThreads = 256;
+constexpr int kPad     = 8; // LDS row padding (elements)
+
+template <typename T> struct MfmaOp;
+
+// Operand layouts of the 16x16x4 MFMAs (both types): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]. Result layouts differ (checked on gfx950): f64 holds D[l/16 + 4r][l%16] in
+// register r, f32 holds D[4(l/16) + r][l%16].
+template <> struct MfmaOp<double> {
+	using acc_t = double __attribute__((ext_vector_type(4)));
+	__device__ static inline acc_t mma(double a, double b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+template <> struct MfmaOp<float> {
+	using acc_t = float __attribute__((ext_vector_type(4)));
+	__device__ static inline acc_t mma(float a, float b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T> struct Bases {
+	const T* p[BASE_COUNT];
+};
+
+// Per-thread register staging for one K chunk: 4 elements of L and 4 of R.
+template <typename T> struct Stage {
+	T l[4];
+	T r[4];
+};
+
+template <typename T>
+__device__ inline void load_chunk(Stage<T>& st, const T* __restrict__ L, const T* __restrict__ R,
+                                  const Seg& s, int m0, int n0, int M, int N, int k0, int tid)
+{
+	// L tile 64 x 16: thread -> row tid % 64, columns tid / 64 + 4q (coalesced along rows).
+	{
+		const int m  = m0 + (tid & 63);
+		const int kb = k0 + (tid >> 6);
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const int k = kb + 4 * q;
+			st.l[q]     = (m < M && k < s.k) ? L[int64_t(m) + int64_t(k) * s.ldl] : T(0);
+		}
+	}
+	if (s.rtrans) {
+		// R stored N x K: element (k, n) at R[n + k*ldr]; coalesced along n.
+		const int n  = n0 + (tid & 63);
+		const int kb = k0 + (tid >> 6);
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const int k = kb + 4 * q;
+			st.r[q]     = (n < N && k < s.k) ? R[int64_t(n) + int64_t(k) * s.ldr] : T(0);
+		}
+	} else {
+		// R stored K x N: element (k, n) at R[k + n*ldr]; coalesced along k.
+		const int k  = k0 + (tid & 15);
+		const int nb = n0 + (tid >> 4);
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const int n = nb + 16 * q;
+			st.r[q]     = (n < N && k < s.k) ? R[int64_t(k) + int64_t(n) * s.ldr] : T(0);
+		}
+	}
+}
+
+template <typename T>
+__device__ inline void store_chunk(const Stage<T>& st, T (*As)[kBM + kPad], T (*Bs)[kBN + kPad],
+                                   int rtrans, int tid)
+{
+#pragma unroll
+	for (int q = 0; q < 4; ++q)
+		As[(tid >> 6) + 4 * q][tid & 63] = st.l[q];
+	if (rtrans) {
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			Bs[(tid >> 6) + 4 * q][tid & 63] = st.r[q];
+	} else {
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			Bs[tid & 15][(tid >> 4) + 16 * q] = st.r[q];
+	}
+}
+
+// Advances (seg, k0) to the next K chunk of the task; returns false past the last one.
+__device__ inline bool next_chunk(const Seg* __restrict__ segs, int seg_end, int& seg, int& k0)
+{
+	k0 += kBK;
+	while (seg < seg_end && k0 >= segs[seg].k) {
+		++seg;
+		k0 = 0;
+	}
+	return seg < seg_end;
+}
+
+// T: operand/accumulate type; TO: output element type (the stage-2 output of the FP32 mode is
+// written in FP64 for the FP64 Lanczos recurrence).
+template <typename T, typename TO>
+__global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
+    const Task* __restrict__ tasks, const Seg* __restrict__ segs, const Tile* __restrict__ tiles,
+    Bases<T> bases, int64_t xstride, int64_t bxstride, TO* __restrict__ out, int64_t ostride,
+    int beta)
+{
+	using Op    = MfmaOp<T>;
+	using acc_t = typename Op::acc_t;
+
+	__shared__ T As[2][kBK][kBM + kPad];
+	__shared__ T Bs[2][kBK][kBN + kPad];
+
+	const Tile tl  = tiles[blockIdx.x];
+	const Task tk  = tasks[tl.task];
+	const int  tid = threadIdx.x;
+	const int  vec = blockIdx.y;
+
+	// Per-vector operand bases (multi-vector launches put one right-hand side per grid row).
+	// Selected through pick() rather than a dynamically indexed array so they stay in SGPRs.
+	const T* b0 = bases.p[BASE_APACK];
+	const T* b1 = bases.p[BASE_BPACK];
+	const T* b2 = bases.p[BASE_X] + vec * xstride;
+	const T* b3 = bases.p[BASE_BX] + vec * bxstride;
+	auto     pick = [&](int id) -> const T* {
+                return id == BASE_APACK ? b0 : id == BASE_BPACK ? b1 : id == BASE_X ? b2 : b3;
+	};
+	out += vec * ostride;
+
+	const int lane = tid & 63;
+	const int wave = tid >> 6;
+	const int wm   = (wave & 1) * 32;
+	const int wn   = (wave >> 1) * 32;
+
+	acc_t acc[2][2];
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j)
+			acc[i][j] = acc_t { 0, 0, 0, 0 };
+
+	int seg = tk.seg_begin;
+	int k0  = 0;
+	while (seg < tk.seg_end && segs[seg].k <= 0)
+		++seg;
+
+	if (seg < tk.seg_end) {
+		Stage<T> st;
+		Seg      s = segs[seg];
+		load_chunk<T>(st, pick(s.lbase) + s.loff, pick(s.rbase) + s.roff, s, tl.m0, tl.n0, tk.m,
+		              tk.n, k0, tid);
+		store_chunk<T>(st, As[0], Bs[0], s.rtrans, tid);
+		__syncthreads();
+		int buf = 0;
+		for (;;) {
+			int  nseg    = seg;
+			int  nk0     = k0;
+			bool hasNext = next_chunk(segs, tk.seg_end, nseg, nk0);
+			Seg  ns;
+			if (hasNext) {
+				ns = segs[nseg];
+				load_chunk<T>(st, pick(ns.lbase) + ns.loff, pick(ns.rbase) + ns.roff, ns, tl.m0,
+				              tl.n0, tk.m, tk.n, nk0, tid);
+			}
+#pragma unroll
+			for (int kk = 0; kk < kBK; kk += 4) {
+				const int kr = kk + (lane >> 4);
+				T         a[2], b[2];
+#pragma unroll
+				for (int i = 0; i < 2; ++i)
+					a[i] = As[buf][kr][wm + 16 * i + (lane & 15)];
+#pragma unroll
+				for (int j = 0; j < 2; ++j)
+					b[j] = Bs[buf][kr][wn + 16 * j + (lane & 15)];
+#pragma unroll
+				for (int i = 0; i < 2; ++i)
+#pragma unroll
+					for (int j = 0; j < 2; ++j)
+						acc[i][j] = Op::mma(a[i], b[j], acc[i][j]);
+			}
+			if (!hasNext)
+				break;
+			store_chunk<T>(st, As[buf ^ 1], Bs[buf ^ 1], ns.rtrans, tid);
+			__syncthreads();
+			buf ^= 1;
+			seg = nseg;
+			k0  = nk0;
+		}
+	}
+
+	// Epilogue: register r of lane l holds row Op::row(l, r), column l%16 of each 16x16 block.
+	TO* C = out + tk.coff;
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const int row = tl.m0 + wm + 16 * i + Op::row(lane, r);
+				const int col = tl.n0 + wn + 16 * j + (lane & 15);
+				if (row < tk.m && col < tk.n) {
+					const int     rr   = row % tk.rsplit;
+					const int     blk  = row / tk.rsplit;
+					const int64_t addr = int64_t(rr) + (int64_t(blk) * tk.n + col) * tk.ldc;
+					TO            v    = TO(acc[i][j][r]);
+					if (beta)
+						v += C[addr];
+					C[addr] = v;
+				}
+			}
+}
+
+__global__ void cast_f64_to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, int64_t n)
+{
+	int64_t i      = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	int64_t stride = int64_t(gridDim.x) * blockDim.x;
+	for (; i < n; i += stride)
+		y[i] = float(x[i]);
+}
+
+// Host-side launchers ------------------------------------------------------------------------
+
+template <typename T, typename TO>
+void launch_grouped_gemm(const Task* tasks, const Seg* segs, const Tile* tiles, int ntiles,
+                         const T* const* bases, int64_t xstride, int64_t bxstride, TO* out,
+                         int64_t ostride, int nvec, int beta, hipStream_t stream)
+{
+	if (ntiles <= 0 || nvec <= 0)
+		return;
+	Bases<T> b;
+	for (int i = 0; i < BASE_COUNT; ++i)
+		b.p[i] = bases[i];
+	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
+	hipLaunchKernelGGL((grouped_gemm_kernel<T, TO>), grid, dim3(kThreads), 0, stream, tasks, segs,
+	                   tiles, b, xstride, bxstride, out, ostride, beta);
+}
+
+template void launch_grouped_gemm<double, double>(const Task*, const Seg*, const Tile*, int,
+                                                  const double* const*, int64_t, int64_t,
+                                                  double*, int64_t, int, int, hipStream_t);
+template void launch_grouped_gemm<float, float>(const Task*, const Seg*, const Tile*, int,
+                                                const float* const*, int64_t, int64_t, float*,
+                                                int64_t, int, int, hipStream_t);
+template void launch_grouped_gemm<float, double>(const Task*, const Seg*, const Tile*, int,
+                                                 const float* const*, int64_t, int64_t, double*,
+                                                 int64_t, int, int, hipStream_t);
+
+void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream)
+{
+	if (n <= 0)
+		return;
+	int64_t blocks = (n + 255) / 256;
+	if (blocks > 4096)
+		blocks = 4096;
+	hipLaunchKernelGGL(cast_f64_to_f32_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, x, y, n);
+}
+
+} // namespace dmk

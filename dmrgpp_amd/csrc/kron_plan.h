@@ -1,0 +1,102 @@
+// Host-side plan of one superblock Kron matvec: the packed factors resident in HBM, the task /
+// segment / tile tables of the two grouped-GEMM stages, and the per-handle stream.
+#pragma once
+
+#include "dmk_internal.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace dmk {
+
+enum Precision : int
+{
+	PREC_FP64   = 0,
+	PREC_FP32   = 1,
+	PREC_BF16X3 = 2,
+};
+
+struct PlanStats {
+	int64_t n                = 0; // superblock dimension (vector length)
+	int64_t npatches         = 0;
+	int64_t noperator        = 0;
+	int64_t nterms           = 0; // nonzero (I,J,k) pairs (setup_nC total)
+	int64_t nterms_identity  = 0; // of which one factor is an identity
+	int64_t stage1_tasks     = 0;
+	int64_t stage2_tasks     = 0;
+	int64_t stage1_tiles     = 0;
+	int64_t stage2_tiles     = 0;
+	double  flops_algorithmic = 0; // sum over terms of min(method 1, method 2), identities as one GEMM
+	double  flops_plugin      = 0; // the reference plugin's gflops1+gflops2 count (all terms dense)
+	double  flops_executed    = 0; // what the two stages execute (before tile padding)
+	double  flops_stage1      = 0; // executed by the stage-1 launch
+	double  flops_stage2      = 0; // executed by the stage-2 launch
+	double  bytes_compulsory  = 0; // non-identity factors once + x read + y write, FP64 bytes
+	double  bytes_bx          = 0; // stage-1 scratch written and read back, FP64 bytes
+	int64_t apack_elems       = 0;
+	int64_t bpack_elems       = 0;
+	int64_t bx_elems          = 0;
+	int64_t max_patch_left    = 0;
+	int64_t max_patch_right   = 0;
+};
+
+struct StageTables {
+	std::vector<Task> tasks;
+	std::vector<Seg>  segs;
+	std::vector<Tile> tiles;
+	Task*             d_tasks = nullptr;
+	Seg*              d_segs  = nullptr;
+	Tile*             d_tiles = nullptr;
+	void              upload();
+	void              release();
+};
+
+class Plan {
+public:
+	// Same inputs as BatchedGemm<double>::BatchedGemm (GPUPlugin/BatchedGemm.h:21-28).
+	Plan(int64_t noperator, int64_t npatches, const size_t* left_patch_size,
+	     const size_t* right_patch_size, const double* const* Amatrix, const size_t* ld_Amatrix,
+	     const double* const* Bmatrix, const size_t* ld_Bmatrix);
+	~Plan();
+
+	Plan(const Plan&)            = delete;
+	Plan& operator=(const Plan&) = delete;
+
+	// vout = H vin (beta = 0) or vout += H vin (beta = 1) for nvec vectors; device pointers.
+	void apply(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec, int beta);
+
+	void set_precision(int p);
+	int  precision() const { return precision_; }
+
+	const PlanStats& stats() const { return stats_; }
+	hipStream_t      stream() const { return stream_; }
+	int64_t          n() const { return stats_.n; }
+
+	// Average device time (ms) of each stage's kernel and of the whole matvec over iters calls,
+	// from HIP events recorded on the plan's stream around every launch.
+	void profile(const double* vin, double* vout, int iters, double* ms_stage1, double* ms_stage2,
+	             double* ms_total);
+
+private:
+	void ensure_scratch(int nvec);
+	void ensure_fp32();
+
+	int         precision_ = PREC_FP64;
+	PlanStats   stats_;
+	StageTables st1_, st2_;
+	hipStream_t stream_ = nullptr;
+
+	std::vector<int64_t> off_;
+	double*              d_apack_ = nullptr;
+	double*              d_bpack_ = nullptr;
+	float*               d_apack32_ = nullptr;
+	float*               d_bpack32_ = nullptr;
+	void*                d_bx_      = nullptr; // stage-1 products, bx_bytes_ bytes
+	float*               d_x32_     = nullptr; // FP32 copy of the input, x32_cap_ vectors
+	int                  x32_cap_   = 0; // vectors d_x32_ holds
+	size_t               bx_bytes_  = 0;
+
+	hipEvent_t ev_[3] = { nullptr, nullptr, nullptr };
+};
+
+} // namespace dmk

@@ -1,0 +1,126 @@
+/*
+ * dmrgpp_amd — MI355X (gfx950) superblock Hamiltonian matvec for DMRG++.
+ *
+ * C ABI of the drop-in replacement for DMRG++'s dmrg/GPUPlugin batched-GEMM plugin
+ * (selected in the reference by -DDMRG_BATCHED_GEMM_TYPE=Plugin, dmrg/CMakeLists.txt:8-11, and
+ * at run time by SolverOptions=BatchedGemm, BatchedGemmPluginSc.h:137).
+ *
+ * Conventions shared with the reference plugin:
+ *  - a patch I has left size L_I and right size R_I; its vector segment is an R_I x L_I
+ *    column-major matrix; segments are concatenated in patch order (InitKronHamiltonian.h:211-216);
+ *  - block pointers are indexed I + J*npatches + k*npatches*npatches; A_k(I,J) is L_I x L_J,
+ *    B_k(I,J) is R_I x R_J, both column major with the given leading dimensions, NULL meaning a
+ *    zero block (BatchedGemmPluginSc.h:96-112, setup_nC.cpp:67-101);
+ *  - the product is y_I = sum_{J,k} B_k(I,J) x_J A_k(I,J)^T, i.e. y = sum_k (A_k (x) B_k) x,
+ *    with the plain transpose also for complex scalars (apply_Htarget_sparse_not_ready.cpp:384);
+ *  - dmk_apply overwrites its output (beta = 0); the caller accumulates (KronMatrix.h:142-151).
+ *
+ * All functions return 0 on success and a negative status on failure; dmk_last_error() then
+ * describes the failure (the reference asserts instead, dmrg_malloc_hip.cpp:17-30).
+ * Vectors passed to dmk_apply*, dmk_lanczos_* are device pointers unless a name says _host.
+ * A handle owns one HIP stream; calls on one handle are not thread safe.
+ */
+#ifndef DMRGPP_AMD_H
+#define DMRGPP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dmk_handle dmk_handle;
+
+enum dmk_status
+{
+	DMK_OK             = 0,
+	DMK_ERR_ARGUMENT   = -1,
+	DMK_ERR_DEVICE     = -2,
+	DMK_ERR_UNSUPPORTED = -3,
+	DMK_ERR_NUMERICAL  = -4,
+};
+
+enum dmk_precision
+{
+	DMK_PREC_FP64   = 0, /* FP64 MFMA, FP64 accumulate */
+	DMK_PREC_FP32   = 1, /* FP32 MFMA on FP32 copies of factors and input, FP32 accumulate */
+	DMK_PREC_BF16X3 = 2, /* reserved: BF16 hi/lo split, three BF16 MFMAs, FP32 accumulate */
+};
+
+typedef struct dmk_info {
+	int64_t n;                 /* superblock dimension */
+	int64_t npatches;
+	int64_t noperator;
+	int64_t nterms;            /* nonzero (I,J,k) terms */
+	int64_t nterms_identity;   /* terms whose A or B factor is an identity */
+	int64_t stage1_tasks, stage2_tasks, stage1_tiles, stage2_tiles;
+	int64_t max_patch_left, max_patch_right;
+	int64_t apack_elems, bpack_elems, bx_elems;
+	double  flops_algorithmic; /* sum_terms min(method1, method2); identity terms as one GEMM */
+	double  flops_plugin;      /* reference plugin count gflops1+gflops2 (all terms dense) */
+	double  flops_executed;    /* flops the two stages execute */
+	double  flops_stage1;      /* of which the stage-1 launch (B_k X_J products) */
+	double  flops_stage2;      /* of which the stage-2 launch (panel and identity products) */
+	double  bytes_compulsory;  /* FP64 bytes: non-identity factors + x read + y write */
+	double  bytes_bx;          /* FP64 bytes of the stage-1 scratch written and read */
+	int32_t precision;
+	int32_t pad;
+} dmk_info;
+
+/* Replaces BatchedGemm<double>::BatchedGemm(noperator, npatches, left_patch_size,
+ * right_patch_size, Amatrix, ld_Amatrix, Bmatrix, ld_Bmatrix) — dmrg/GPUPlugin/BatchedGemm.h:21-28,
+ * BatchedGemm.cpp:8-70. Blocks are host pointers; they are scanned, packed and copied to the
+ * device, and may be freed after the call (as setup_sparse_batch.cpp:281-343 copies them). */
+int dmk_create_d(size_t noperator, size_t npatches, const size_t* left_patch_size,
+                 const size_t* right_patch_size, const double* const* Amatrix,
+                 const size_t* ld_Amatrix, const double* const* Bmatrix, const size_t* ld_Bmatrix,
+                 dmk_handle** handle);
+
+/* Replaces BatchedGemm<T>::~BatchedGemm — BatchedGemm.h:33, BatchedGemm.cpp:72-79. */
+void dmk_destroy(dmk_handle* handle);
+
+/* Replaces BatchedGemm<double>::apply_Htarget(vin, vout) — BatchedGemm.h:30-31,
+ * BatchedGemm.cpp:80-113 -> apply_Htarget_sparse. vout = H vin, device pointers, n elements.
+ * Asynchronous on the handle's stream; dmk_sync waits for it. */
+int dmk_apply_d(dmk_handle* handle, const double* vin, double* vout);
+
+/* Same on host vectors (the BatchedGemmPluginSc::matrixVector round trip,
+ * BatchedGemmPluginSc.h:139-161): copies vin in, applies, copies vout out, synchronous. */
+int dmk_apply_host_d(dmk_handle* handle, const double* vin, double* vout);
+
+/* nvec right-hand sides in one launch pair: vout[v*ldout..] = H vin[v*ldin..] (beta = 0) or
+ * += (beta = 1). No reference counterpart (ParallelTriDiag.h:132-161 tridiagonalizes one vector
+ * at a time); used by the batched Krylov time evolution. */
+int dmk_apply_multi_d(dmk_handle* handle, int nvec, const double* vin, size_t ldin, double* vout,
+                      size_t ldout, int beta);
+
+/* Precision of the A_k (x) B_k products (enum dmk_precision). No reference counterpart: the
+ * plugin is FP64/complex-FP64 only (GPUPluginConfig.h.in). */
+int dmk_set_precision(dmk_handle* handle, int precision);
+
+int dmk_get_info(const dmk_handle* handle, dmk_info* info);
+
+/* Waits for all work queued on the handle's stream. */
+int dmk_sync(dmk_handle* handle);
+
+/* Average device time per call (ms) of stage 1, stage 2 and the whole matvec, over iters calls
+ * timed with HIP events on the handle's stream (the plugin prints GFLOP/s with idebug=1,
+ * apply_Htarget_sparse_not_ready.cpp:451-463). */
+int dmk_profile_apply_d(dmk_handle* handle, const double* vin, double* vout, int iters,
+                        double* ms_stage1, double* ms_stage2, double* ms_total);
+
+/* Device memory helpers for callers without their own allocator. */
+int dmk_device_alloc(size_t bytes, void** ptr);
+int dmk_device_free(void* ptr);
+int dmk_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int dmk_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int dmk_device_count(int* count);
+
+const char* dmk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMRGPP_AMD_H */

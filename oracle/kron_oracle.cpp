@@ -1,0 +1,720 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of DMRG++'s superblock Kron matvec path and of the Lanczos recurrence that
+// drives it. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
+// library, and only as the checker. The product (dmrgpp_amd/) never links or calls it.
+//
+// Functions and the reference code they restate (paths relative to the DMRG++ tree):
+//   orc_kron_connections_{d,z}  KronConnections::doTask            Engine/MatrixVectorKron/KronConnections.h:124-181
+//                               kronMult (dense x dense branch)    KronUtil/MatrixDenseOrSparse.h:133-160
+//                               den_kron_mult methods 1/2/3        KronUtil/den_kron_mult.cpp:5-430
+//                               estimate_kron_cost                 KronUtil/estimate_kron_cost.cpp:3-125
+//   orc_plugin_apply_{d,z}      setup_nC                           GPUPlugin/setup_nC.cpp:11-145
+//                               setup_sparse_batch packing         GPUPlugin/setup_sparse_batch.cpp:13-379
+//                               apply_Htarget_sparse (2 stages)    GPUPlugin/apply_Htarget_sparse_not_ready.cpp:14-478
+//   orc_kron_form_{d,z}         explicit H = sum_k A_k (x) B_k     KronUtil/den_kron_form.cpp (used by KronUtil/test1.cpp)
+//   orc_lanczos_{d,z}           LanczosCore::decomposition         PsimagLite/LanczosCore.h:147-272
+//                               LanczosVectors::oneStepDecomposition / reorthoIfNecessary / excitedVector
+//                                                                  PsimagLite/LanczosVectors.h:192-271
+//                               LanczosSolver::computeAllStatesBelow PsimagLite/LanczosSolver.h:68-88
+//   orc_tridiag_eigen           eigen-decomposition of the Lanczos tridiagonal matrix
+//                               (TridiagonalMatrix.h:149-200 builds it; the reference hands it to LAPACK
+//                               dsyev, Matrix.cpp:15-45). Implemented here as implicit-shift QL.
+//
+// Pinning (tests/test_oracle.py): the KronConnections fixture values of test_KronConnections.cpp,
+// the mt19937_64 fixture of test_BatchedGemm.cpp, the dumped Kron problem kron_data31_0_ref.txt
+// checked against brute-force assembly (orc_kron_form) and Hermiticity, closed-form spectra
+// (free fermions, ED) for the Lanczos restatement, numpy.linalg.eigh for the tridiagonal solver.
+//
+// Storage conventions follow the reference: every matrix is column major (PsimagLite::Matrix,
+// Matrix.h:250-262); the vector segment of patch I is an R_I x L_I column-major matrix
+// (InitKronHamiltonian.h:211-216); block pointers of the plugin interface are indexed
+// (I + J*np + k*np*np) (BatchedGemmPluginSc.h:96-101, dmrg_vbatch.cpp:50-59).
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+namespace {
+
+using cplx = std::complex<double>;
+
+inline double conjv(double x) { return x; }
+inline cplx   conjv(const cplx& z) { return std::conj(z); }
+inline double realv(double x) { return x; }
+inline double realv(const cplx& z) { return z.real(); }
+
+// op(M)(i,j) for a column-major matrix with leading dimension ld; trans in {n,t,c}.
+template <typename T> struct OpView {
+	const T* p;
+	int64_t  ld;
+	char     trans;
+	T        operator()(int64_t i, int64_t j) const
+	{
+		if (trans == 'n' || trans == 'N')
+			return p[i + j * ld];
+		if (trans == 't' || trans == 'T')
+			return p[j + i * ld];
+		return conjv(p[j + i * ld]);
+	}
+};
+
+// C(m x n) += alpha * op(A)(m x k) * op(B)(k x n). Column-major. Loop order j, l, i keeps the
+// innermost loop unit-stride on C and (for 'n') on A.
+template <typename T>
+void gemm_acc(char ta, char tb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda,
+              const T* B, int64_t ldb, T* C, int64_t ldc)
+{
+	OpView<T> a { A, lda, ta };
+	OpView<T> b { B, ldb, tb };
+	const bool an = (ta == 'n' || ta == 'N');
+	for (int64_t j = 0; j < n; ++j) {
+		T* cj = C + j * ldc;
+		for (int64_t l = 0; l < k; ++l) {
+			const T blj = alpha * b(l, j);
+			if (blj == T(0))
+				continue;
+			if (an) {
+				const T* al = A + l * lda;
+				for (int64_t i = 0; i < m; ++i)
+					cj[i] += al[i] * blj;
+			} else {
+				for (int64_t i = 0; i < m; ++i)
+					cj[i] += a(i, l) * blj;
+			}
+		}
+	}
+}
+
+template <typename T> int64_t count_nnz(const T* p, int64_t rows, int64_t cols, int64_t ld)
+{
+	if (!p)
+		return 0;
+	int64_t nnz = 0;
+	for (int64_t j = 0; j < cols; ++j)
+		for (int64_t i = 0; i < rows; ++i)
+			nnz += (p[i + j * ld] == T(0)) ? 0 : 1;
+	return nnz;
+}
+
+// estimate_kron_cost.cpp:3-125 with minimize_flops = true.
+int estimate_kron_method(int64_t nrow_A, int64_t ncol_A, int64_t nnz_A_in, int64_t nrow_B,
+                         int64_t ncol_B, int64_t nnz_B_in, double denseFlopDiscount)
+{
+	const double nnz_A      = double(nnz_A_in);
+	const double nnz_B      = double(nnz_B_in);
+	const bool   is_dense_A = (nnz_A_in == nrow_A * ncol_A);
+	const bool   is_dense_B = (nnz_B_in == nrow_B * ncol_B);
+
+	double       discount      = is_dense_B ? denseFlopDiscount : 1;
+	const double flops_BY      = discount * 2.0 * nnz_B * ncol_A;
+	discount                   = is_dense_A ? denseFlopDiscount : 1;
+	double       flops_BYAt    = discount * 2.0 * nnz_A * nrow_B;
+	const double flops_method1 = flops_BY + flops_BYAt;
+
+	discount                   = is_dense_A ? denseFlopDiscount : 1;
+	const double flops_YAt     = discount * 2.0 * nnz_A * ncol_B;
+	discount                   = is_dense_B ? denseFlopDiscount : 1;
+	flops_BYAt                 = discount * 2.0 * nnz_B * nrow_A;
+	const double flops_method2 = flops_YAt + flops_BYAt;
+
+	const double flops_method3 = 2.0 * nnz_A * nnz_B;
+	const double kron_flops    = std::min(flops_method1, std::min(flops_method2, flops_method3));
+
+	int imethod = 3;
+	if (kron_flops == flops_method1)
+		imethod = 1;
+	if (kron_flops == flops_method2)
+		imethod = 2;
+	return imethod;
+}
+
+// den_kron_mult.cpp:5-430: X += kron(op(A), op(B)) * Y for dense A, B.
+// X is nrow_X x ncol_X at xout+offX, Y is nrow_Y x ncol_Y at yin+offY (column major, ld = rows).
+template <typename T>
+void den_kron_mult(int imethod, char transA, char transB, const T* a, int64_t nrow_A, int64_t ncol_A,
+                   int64_t lda, const T* b, int64_t nrow_B, int64_t ncol_B, int64_t ldb,
+                   const T* yin, T* xout)
+{
+	const bool tA = (transA != 'n' && transA != 'N');
+	const bool tB = (transB != 'n' && transB != 'N');
+	const bool cA = (transA == 'c' || transA == 'C');
+	const bool cB = (transB == 'c' || transB == 'C');
+
+	const int64_t nrow_1 = tA ? ncol_A : nrow_A;
+	const int64_t ncol_1 = tA ? nrow_A : ncol_A;
+	const int64_t nrow_2 = tB ? ncol_B : nrow_B;
+	const int64_t ncol_2 = tB ? nrow_B : ncol_B;
+
+	const int64_t nrow_X = nrow_2;
+	const int64_t ncol_X = nrow_1;
+	const int64_t nrow_Y = ncol_2;
+	const int64_t ncol_Y = ncol_1;
+
+	// "transpose(op(A))" as a GEMM op on the stored A: n->t, t->n, c->conj (no transpose).
+	std::vector<T> aconj;
+	const T*       aeff     = a;
+	char           opAt     = tA ? 'n' : 't';
+	if (cA) {
+		aconj.resize(size_t(nrow_A * ncol_A));
+		for (int64_t j = 0; j < ncol_A; ++j)
+			for (int64_t i = 0; i < nrow_A; ++i)
+				aconj[size_t(i + j * nrow_A)] = conjv(a[i + j * lda]);
+		aeff = aconj.data();
+	}
+	const int64_t ldaeff = cA ? nrow_A : lda;
+
+	if (imethod == 1) {
+		// BY = op(B) * Y (nrow_X x ncol_Y); X += BY * transpose(op(A)).
+		std::vector<T> by(size_t(nrow_X * ncol_Y), T(0));
+		gemm_acc<T>(transB, 'n', nrow_X, ncol_Y, nrow_Y, T(1), b, ldb, yin, nrow_Y, by.data(),
+		            nrow_X);
+		gemm_acc<T>('n', opAt, nrow_X, ncol_X, ncol_Y, T(1), by.data(), nrow_X, aeff, ldaeff, xout,
+		            nrow_X);
+	} else if (imethod == 2) {
+		// YAt = Y * transpose(op(A)) (nrow_Y x ncol_X); X += op(B) * YAt.
+		std::vector<T> yat(size_t(nrow_Y * ncol_X), T(0));
+		gemm_acc<T>('n', opAt, nrow_Y, ncol_X, ncol_Y, T(1), yin, nrow_Y, aeff, ldaeff, yat.data(),
+		            nrow_Y);
+		gemm_acc<T>(transB, 'n', nrow_X, ncol_X, nrow_Y, T(1), b, ldb, yat.data(), nrow_Y, xout,
+		            nrow_X);
+	} else {
+		// Method 3: visit every pair of entries (den_kron_mult.cpp:300-372).
+		for (int64_t ia = 0; ia < nrow_A; ++ia)
+			for (int64_t ja = 0; ja < ncol_A; ++ja)
+				for (int64_t ib = 0; ib < nrow_B; ++ib)
+					for (int64_t jb = 0; jb < ncol_B; ++jb) {
+						T aij = a[ia + ja * lda];
+						if (cA)
+							aij = conjv(aij);
+						T bij = b[ib + jb * ldb];
+						if (cB)
+							bij = conjv(bij);
+						const int64_t ix = tB ? jb : ib;
+						const int64_t jx = tA ? ja : ia;
+						const int64_t iy = tB ? ib : jb;
+						const int64_t jy = tA ? ia : ja;
+						xout[ix + jx * nrow_X] += aij * bij * yin[iy + jy * nrow_Y];
+					}
+	}
+}
+
+// The problem as the plugin interface sees it (BatchedGemm.h:21-28).
+template <typename T> struct PluginInput {
+	int64_t          nC;
+	int64_t          np;
+	const size_t*    L; // left_patch_size (indexed by patch)
+	const size_t*    R; // right_patch_size
+	const T* const*  A; // [I + J*np + k*np*np], L_I x L_J, or nullptr
+	const size_t*    ldA;
+	const T* const*  B; // R_I x R_J
+	const size_t*    ldB;
+	size_t           idx(int64_t I, int64_t J, int64_t k) const { return size_t(I + J * np + k * np * np); }
+};
+
+template <typename T> std::vector<size_t> patch_offsets(const PluginInput<T>& in)
+{
+	std::vector<size_t> off(size_t(in.np + 1), 0);
+	for (int64_t I = 0; I < in.np; ++I)
+		off[size_t(I + 1)] = off[size_t(I)] + in.L[I] * in.R[I];
+	return off;
+}
+
+// KronConnections::doTask over all out-patches (KronConnections.h:124-181), dense blocks.
+// useLowerPart: for outPatch < inPatch take the (inPatch, outPatch) blocks with op 't'/'c'.
+template <typename T>
+void kron_connections(const PluginInput<T>& in, const T* yin, T* xout, int applyAdjoint,
+                      int useLowerPart, double denseFlopDiscount)
+{
+	const bool           isComplex = !std::is_same<T, double>::value;
+	std::vector<size_t>  off       = patch_offsets(in);
+	for (int64_t outPatch = 0; outPatch < in.np; ++outPatch) {
+		for (int64_t inPatch = 0; inPatch < in.np; ++inPatch) {
+			for (int64_t ic = 0; ic < in.nC; ++ic) {
+				const bool performTranspose = (!applyAdjoint && useLowerPart && outPatch < inPatch);
+				size_t     id               = (applyAdjoint || performTranspose)
+				                  ? in.idx(inPatch, outPatch, ic)
+				                  : in.idx(outPatch, inPatch, ic);
+				const T* a = in.A[id];
+				const T* b = in.B[id];
+				if (!a || !b)
+					continue;
+				const int64_t rI  = (applyAdjoint || performTranspose) ? inPatch : outPatch;
+				const int64_t cI  = (applyAdjoint || performTranspose) ? outPatch : inPatch;
+				const int64_t nrA = int64_t(in.L[rI]), ncA = int64_t(in.L[cI]);
+				const int64_t nrB = int64_t(in.R[rI]), ncB = int64_t(in.R[cI]);
+				const char    opt = (applyAdjoint || performTranspose) ? (isComplex ? 'c' : 't') : 'n';
+				const int64_t nnzA = count_nnz(a, nrA, ncA, int64_t(in.ldA[id]));
+				const int64_t nnzB = count_nnz(b, nrB, ncB, int64_t(in.ldB[id]));
+				const int     imethod
+				    = estimate_kron_method(nrA, ncA, nnzA, nrB, ncB, nnzB, denseFlopDiscount);
+				den_kron_mult<T>(imethod, opt, opt, a, nrA, ncA, int64_t(in.ldA[id]), b, nrB, ncB,
+				                 int64_t(in.ldB[id]), yin + off[size_t(inPatch)],
+				                 xout + off[size_t(outPatch)]);
+			}
+		}
+	}
+}
+
+inline size_t ld32(size_t x) { return 32 * ((x + 31) / 32); }
+
+// setup_nC + setup_sparse_batch + apply_Htarget_sparse: vout = H vin (beta = 0).
+template <typename T> void plugin_apply(const PluginInput<T>& in, const T* vin, T* vout)
+{
+	const int64_t np = in.np;
+	std::vector<size_t> off = patch_offsets(in);
+	// setup_nC.cpp:47-124: a pair counts when both factors have a nonzero entry; NULL = zero.
+	std::vector<int64_t> nC(size_t(np * np), 0);
+	std::vector<char>    work(size_t(np * np * in.nC), 0);
+	for (int64_t k = 0; k < in.nC; ++k)
+		for (int64_t J = 0; J < np; ++J)
+			for (int64_t I = 0; I < np; ++I) {
+				size_t  id   = in.idx(I, J, k);
+				int64_t nnzA = count_nnz(in.A[id], int64_t(in.L[I]), int64_t(in.L[J]), int64_t(in.ldA[id]));
+				int64_t nnzB = count_nnz(in.B[id], int64_t(in.R[I]), int64_t(in.R[J]), int64_t(in.ldB[id]));
+				if (nnzA > 0 && nnzB > 0) {
+					nC[size_t(I + J * np)] += 1;
+					work[id] = 1;
+				}
+			}
+	for (int64_t I = 0; I < np; ++I) {
+		const size_t LI = in.L[I], RI = in.R[I];
+		const size_t ldA = ld32(LI), ldB = ld32(RI);
+		// Packed panels of row I in (J outer, k inner) order (setup_sparse_batch.cpp:281-343).
+		size_t K = 0;
+		for (int64_t J = 0; J < np; ++J)
+			K += size_t(nC[size_t(I + J * np)]) * in.L[J];
+		std::vector<T> Apanel(ldA * std::max<size_t>(K, 1), T(0));
+		std::vector<T> BX(ldB * std::max<size_t>(K, 1), T(0));
+		size_t         col = 0;
+		for (int64_t J = 0; J < np; ++J) {
+			for (int64_t k = 0; k < in.nC; ++k) {
+				size_t id = in.idx(I, J, k);
+				if (!work[id])
+					continue;
+				const size_t LJ = in.L[J], RJ = in.R[J];
+				for (size_t c = 0; c < LJ; ++c)
+					for (size_t r = 0; r < LI; ++r)
+						Apanel[r + (col + c) * ldA] = in.A[id][r + c * in.ldA[id]];
+				// Stage 1 (apply_Htarget_sparse_not_ready.cpp:199-311): BX = B * X_J.
+				gemm_acc<T>('n', 'n', int64_t(RI), int64_t(LJ), int64_t(RJ), T(1), in.B[id],
+				            int64_t(in.ldB[id]), vin + off[size_t(J)], int64_t(RJ),
+				            BX.data() + col * ldB, int64_t(ldB));
+				col += LJ;
+			}
+		}
+		// Stage 2 (:334-449): Y_I = BX_I * transpose(A_I), beta = 0.
+		T* Y = vout + off[size_t(I)];
+		std::fill(Y, Y + RI * LI, T(0));
+		gemm_acc<T>('n', 't', int64_t(RI), int64_t(LI), int64_t(K), T(1), BX.data(), int64_t(ldB),
+		            Apanel.data(), int64_t(ldA), Y, int64_t(RI));
+	}
+}
+
+// Dense n x n H, column major: H[(off_I + r + l R_I), (off_J + r' + l' R_J)] = sum_k A(l,l') B(r,r').
+template <typename T> void kron_form(const PluginInput<T>& in, T* H)
+{
+	std::vector<size_t> off = patch_offsets(in);
+	const size_t        n   = off[size_t(in.np)];
+	std::fill(H, H + n * n, T(0));
+	for (int64_t k = 0; k < in.nC; ++k)
+		for (int64_t J = 0; J < in.np; ++J)
+			for (int64_t I = 0; I < in.np; ++I) {
+				size_t id = in.idx(I, J, k);
+				if (!in.A[id] || !in.B[id])
+					continue;
+				const size_t LI = in.L[I], RI = in.R[I], LJ = in.L[J], RJ = in.R[J];
+				for (size_t lp = 0; lp < LJ; ++lp)
+					for (size_t rp = 0; rp < RJ; ++rp) {
+						const size_t col = off[size_t(J)] + rp + lp * RJ;
+						for (size_t l = 0; l < LI; ++l) {
+							const T a = in.A[id][l + lp * in.ldA[id]];
+							if (a == T(0))
+								continue;
+							for (size_t r = 0; r < RI; ++r) {
+								const size_t row = off[size_t(I)] + r + l * RI;
+								H[row + col * n] += a * in.B[id][r + rp * in.ldB[id]];
+							}
+						}
+					}
+			}
+}
+
+// Symmetric tridiagonal eigenproblem by implicit-shift QL iterations (Wilkinson shift, Givens
+// rotations). d[0..n) diagonal, e[0..n) with e[i] = T(i, i+1) for i < n-1. On return d holds the
+// eigenvalues in ascending order and z (n x n, column major, may be null) the eigenvectors.
+int tridiag_ql(int n, double* d, const double* e_in, double* z)
+{
+	std::vector<double> e(size_t(std::max(n, 1)), 0.0);
+	for (int i = 0; i + 1 < n; ++i)
+		e[size_t(i)] = e_in[i];
+	if (z) {
+		std::fill(z, z + size_t(n) * size_t(n), 0.0);
+		for (int i = 0; i < n; ++i)
+			z[i + size_t(i) * n] = 1.0;
+	}
+	for (int l = 0; l < n; ++l) {
+		int iter = 0;
+		for (;;) {
+			int m = l;
+			for (; m + 1 < n; ++m) {
+				const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+				if (std::fabs(e[size_t(m)]) <= 1e-300 + 2.220446049250313e-16 * dd * 0.5)
+					break;
+			}
+			if (m == l)
+				break;
+			if (++iter > 200)
+				return -1;
+			// Wilkinson shift from the trailing 2x2 of the unreduced block starting at l.
+			double g = (d[l + 1] - d[l]) / (2.0 * e[size_t(l)]);
+			double r = std::hypot(g, 1.0);
+			g        = d[m] - d[l] + e[size_t(l)] / (g + (g >= 0 ? r : -r));
+			double s = 1.0, c = 1.0, p = 0.0;
+			int    i = m - 1;
+			for (; i >= l; --i) {
+				double f = s * e[size_t(i)];
+				double b = c * e[size_t(i)];
+				r        = std::hypot(f, g);
+				e[size_t(i + 1)] = r;
+				if (r == 0.0) {
+					d[i + 1] -= p;
+					e[size_t(m)] = 0.0;
+					break;
+				}
+				s        = f / r;
+				c        = g / r;
+				g        = d[i + 1] - p;
+				r        = (d[i] - g) * s + 2.0 * c * b;
+				p        = s * r;
+				d[i + 1] = g + p;
+				g        = c * r - b;
+				if (z) {
+					for (int kk = 0; kk < n; ++kk) {
+						double* zi  = z + size_t(i) * n;
+						double* zi1 = z + size_t(i + 1) * n;
+						f           = zi1[kk];
+						zi1[kk]     = s * zi[kk] + c * f;
+						zi[kk]      = c * zi[kk] - s * f;
+					}
+				}
+			}
+			if (r == 0.0 && i >= l)
+				continue;
+			d[l] -= p;
+			e[size_t(l)] = g;
+			e[size_t(m)] = 0.0;
+		}
+	}
+	// Sort ascending (selection sort keeps eigenvector columns paired).
+	for (int i = 0; i + 1 < n; ++i) {
+		int    k = i;
+		double p = d[i];
+		for (int j = i + 1; j < n; ++j)
+			if (d[j] < p) {
+				k = j;
+				p = d[j];
+			}
+		if (k != i) {
+			d[k] = d[i];
+			d[i] = p;
+			if (z)
+				for (int kk = 0; kk < n; ++kk)
+					std::swap(z[kk + size_t(i) * n], z[kk + size_t(k) * n]);
+		}
+	}
+	return 0;
+}
+
+template <typename T> double dotre(size_t n, const T* x, const T* y)
+{
+	// Re <x|y> = sum Re(y_i * conj(x_i)) as LanczosCore.h:192-196.
+	double s = 0;
+	for (size_t i = 0; i < n; ++i)
+		s += realv(y[i] * conjv(x[i]));
+	return s;
+}
+
+// LanczosCore::decomposition (LanczosCore.h:147-272) + computeAllStatesBelow (LanczosSolver.h:68-88)
+// for the lowest state with lotaMemory = true. matvec(x, y) computes x += H y.
+template <typename T>
+int lanczos(size_t n, const std::function<void(T*, const T*)>& matvec, const T* init, int maxSteps,
+            int minSteps, double tol, int reortho, double* energy, T* z, double* alpha,
+            double* beta, int* stepsOut)
+{
+	size_t max_nstep = size_t(maxSteps);
+	if (max_nstep > n)
+		max_nstep = n;
+	std::vector<T>      V0(init, init + n), V1(n, T(0)), V2(n, T(0));
+	std::vector<double> a(max_nstep, 0.0), b(max_nstep, 0.0);
+	std::vector<T>      data; // n x max_nstep saved vectors
+	data.resize(n * max_nstep);
+
+	double atmp = 0;
+	for (size_t i = 0; i < n; ++i)
+		atmp += realv(V0[i] * conjv(V0[i]));
+	atmp = 1.0 / std::sqrt(atmp);
+	for (size_t i = 0; i < n; ++i)
+		V0[i] *= atmp;
+
+	b[0] = 0.0;
+	std::copy(V0.begin(), V0.end(), data.begin());
+	matvec(V1.data(), V0.data());
+	atmp = dotre(n, V0.data(), V1.data());
+	a[0] = atmp;
+	double btmp = 0.0;
+	for (size_t i = 0; i < n; ++i) {
+		V1[i] -= atmp * V0[i];
+		btmp += realv(V1[i] * conjv(V1[i]));
+	}
+	btmp = std::sqrt(btmp);
+	if (max_nstep > 1)
+		b[1] = btmp;
+	if (btmp > 0) {
+		btmp = 1.0 / btmp;
+		for (size_t i = 0; i < n; ++i)
+			V1[i] *= btmp;
+	}
+	if (1 < max_nstep)
+		std::copy(V1.begin(), V1.end(), data.begin() + n);
+
+	double eold   = 0;
+	size_t j      = 1;
+	bool   broke  = false;
+	for (j = 1; j < max_nstep; ++j) {
+		// oneStepDecomposition (LanczosVectors.h:203-237)
+		std::fill(V2.begin(), V2.end(), T(0));
+		matvec(V2.data(), V1.data());
+		a[j] = dotre(n, V1.data(), V2.data());
+		btmp = 0.0;
+		for (size_t h = 0; h < n; ++h) {
+			V2[h] = V2[h] - a[j] * V1[h] - b[j] * V0[h];
+			btmp += realv(V2[h] * conjv(V2[h]));
+		}
+		btmp = std::sqrt(btmp);
+		if (j + 1 < max_nstep)
+			b[j + 1] = btmp;
+		if (btmp > 0) {
+			btmp = 1.0 / btmp;
+			for (size_t i = 0; i < n; ++i)
+				V2[i] *= btmp;
+		}
+		if (reortho) {
+			// reorthoIfNecessary (LanczosVectors.h:243-271): classical Gram-Schmidt against V_0..V_j.
+			for (size_t i = 0; i < j + 1; ++i) {
+				const T* vi  = data.data() + i * n;
+				T        rij = T(0);
+				for (size_t h = 0; h < n; ++h)
+					rij += conjv(vi[h]) * V2[h];
+				for (size_t h = 0; h < n; ++h)
+					V2[h] = V2[h] - vi[h] * rij;
+			}
+			double ntmp = 0.0;
+			for (size_t h = 0; h < n; ++h)
+				ntmp += realv(V2[h] * conjv(V2[h]));
+			ntmp = 1.0 / std::sqrt(ntmp);
+			for (size_t h = 0; h < n; ++h)
+				V2[h] = V2[h] * ntmp;
+		}
+		// ab.diag(tmpEigs, j+1): lowest eigenvalue of T_{j+1}.
+		std::vector<double> dd(a.begin(), a.begin() + long(j + 1)), ee(j + 1, 0.0);
+		for (size_t i = 0; i < j; ++i)
+			ee[i] = b[i + 1];
+		if (tridiag_ql(int(j + 1), dd.data(), ee.data(), nullptr) != 0)
+			return -2;
+		const double delta = std::fabs(dd[0] - eold);
+		eold               = dd[0];
+		bool exitFlag      = (delta < tol);
+		if (j == max_nstep - 1)
+			exitFlag = true;
+		if (exitFlag && n <= 4) {
+			broke = true;
+			break;
+		}
+		if (exitFlag && j >= size_t(minSteps)) {
+			broke = true;
+			break;
+		}
+		if (j + 1 < max_nstep)
+			std::copy(V2.begin(), V2.end(), data.begin() + long((j + 1) * n));
+		V0.swap(V1);
+		V1.swap(V2);
+	}
+	// LanczosCore.h:252-255: a decomposition stopped at step j keeps j+1 vectors.
+	const size_t nsteps = (broke || j < max_nstep) ? j + 1 : max_nstep;
+	// Ritz: eigenvectors of T (buildDenseMatrix + diag), ground vector z = sum_j ritz(j,0) V_j.
+	std::vector<double> dd(a.begin(), a.begin() + long(nsteps)), ee(nsteps, 0.0), ritz(nsteps * nsteps);
+	for (size_t i = 0; i + 1 < nsteps; ++i)
+		ee[i] = b[i + 1];
+	if (tridiag_ql(int(nsteps), dd.data(), ee.data(), ritz.data()) != 0)
+		return -3;
+	*energy = dd[0];
+	std::fill(z, z + n, T(0));
+	for (size_t jj = 0; jj < nsteps; ++jj) {
+		const double c  = ritz[jj];
+		const T*     vj = data.data() + jj * n;
+		for (size_t i = 0; i < n; ++i)
+			z[i] += c * vj[i];
+	}
+	for (size_t i = 0; i < nsteps; ++i) {
+		if (alpha)
+			alpha[i] = a[i];
+		if (beta)
+			beta[i] = b[i];
+	}
+	*stepsOut = int(nsteps);
+	return 0;
+}
+
+template <typename T>
+PluginInput<T> make_input(int64_t nC, int64_t np, const size_t* L, const size_t* R, const T* const* A,
+                          const size_t* ldA, const T* const* B, const size_t* ldB)
+{
+	PluginInput<T> in;
+	in.nC  = nC;
+	in.np  = np;
+	in.L   = L;
+	in.R   = R;
+	in.A   = A;
+	in.ldA = ldA;
+	in.B   = B;
+	in.ldB = ldB;
+	return in;
+}
+
+} // namespace
+
+extern "C" {
+
+int orc_kron_connections_d(int64_t nC, int64_t np, const size_t* L, const size_t* R,
+                           const double* const* A, const size_t* ldA, const double* const* B,
+                           const size_t* ldB, const double* yin, double* xout, int applyAdjoint,
+                           int useLowerPart, double denseFlopDiscount)
+{
+	kron_connections<double>(make_input<double>(nC, np, L, R, A, ldA, B, ldB), yin, xout,
+	                         applyAdjoint, useLowerPart, denseFlopDiscount);
+	return 0;
+}
+
+int orc_kron_connections_z(int64_t nC, int64_t np, const size_t* L, const size_t* R,
+                           const double* const* A, const size_t* ldA, const double* const* B,
+                           const size_t* ldB, const double* yin, double* xout, int applyAdjoint,
+                           int useLowerPart, double denseFlopDiscount)
+{
+	kron_connections<cplx>(make_input<cplx>(nC, np, L, R, reinterpret_cast<const cplx* const*>(A), ldA,
+	                                        reinterpret_cast<const cplx* const*>(B), ldB),
+	                       reinterpret_cast<const cplx*>(yin), reinterpret_cast<cplx*>(xout),
+	                       applyAdjoint, useLowerPart, denseFlopDiscount);
+	return 0;
+}
+
+int orc_plugin_apply_d(int64_t nC, int64_t np, const size_t* L, const size_t* R,
+                       const double* const* A, const size_t* ldA, const double* const* B,
+                       const size_t* ldB, const double* vin, double* vout)
+{
+	plugin_apply<double>(make_input<double>(nC, np, L, R, A, ldA, B, ldB), vin, vout);
+	return 0;
+}
+
+int orc_plugin_apply_z(int64_t nC, int64_t np, const size_t* L, const size_t* R,
+                       const double* const* A, const size_t* ldA, const double* const* B,
+                       const size_t* ldB, const double* vin, double* vout)
+{
+	plugin_apply<cplx>(make_input<cplx>(nC, np, L, R, reinterpret_cast<const cplx* const*>(A), ldA,
+	                                    reinterpret_cast<const cplx* const*>(B), ldB),
+	                   reinterpret_cast<const cplx*>(vin), reinterpret_cast<cplx*>(vout));
+	return 0;
+}
+
+int orc_kron_form_d(int64_t nC, int64_t np, const size_t* L, const size_t* R, const double* const* A,
+                    const size_t* ldA, const double* const* B, const size_t* ldB, double* H)
+{
+	kron_form<double>(make_input<double>(nC, np, L, R, A, ldA, B, ldB), H);
+	return 0;
+}
+
+int orc_kron_form_z(int64_t nC, int64_t np, const size_t* L, const size_t* R, const double* const* A,
+                    const size_t* ldA, const double* const* B, const size_t* ldB, double* H)
+{
+	kron_form<cplx>(make_input<cplx>(nC, np, L, R, reinterpret_cast<const cplx* const*>(A), ldA,
+	                                 reinterpret_cast<const cplx* const*>(B), ldB),
+	                reinterpret_cast<cplx*>(H));
+	return 0;
+}
+
+// den_kron_mult with an explicit method (1, 2 or 3) and op characters, for the KronUtil
+// test1.cpp-style sweep: X(nrow_X x ncol_X) += kron(op(A), op(B)) Y.
+int orc_den_kron_mult_d(int imethod, char transA, char transB, const double* a, int64_t nrow_A,
+                        int64_t ncol_A, const double* b, int64_t nrow_B, int64_t ncol_B,
+                        const double* yin, double* xout)
+{
+	den_kron_mult<double>(imethod, transA, transB, a, nrow_A, ncol_A, nrow_A, b, nrow_B, ncol_B,
+	                      nrow_B, yin, xout);
+	return 0;
+}
+
+int orc_den_kron_mult_z(int imethod, char transA, char transB, const double* a, int64_t nrow_A,
+                        int64_t ncol_A, const double* b, int64_t nrow_B, int64_t ncol_B,
+                        const double* yin, double* xout)
+{
+	den_kron_mult<cplx>(imethod, transA, transB, reinterpret_cast<const cplx*>(a), nrow_A, ncol_A,
+	                    nrow_A, reinterpret_cast<const cplx*>(b), nrow_B, ncol_B, nrow_B,
+	                    reinterpret_cast<const cplx*>(yin), reinterpret_cast<cplx*>(xout));
+	return 0;
+}
+
+int orc_estimate_kron_method(int64_t nrow_A, int64_t ncol_A, int64_t nnz_A, int64_t nrow_B,
+                             int64_t ncol_B, int64_t nnz_B, double denseFlopDiscount)
+{
+	return estimate_kron_method(nrow_A, ncol_A, nnz_A, nrow_B, ncol_B, nnz_B, denseFlopDiscount);
+}
+
+int orc_tridiag_eigen(int n, double* d, const double* e, double* z)
+{
+	return tridiag_ql(n, d, e, z);
+}
+
+// Lanczos ground state of H given in the plugin layout; the matvec is the plugin restatement.
+int orc_lanczos_d(int64_t nC, int64_t np, const size_t* L, const size_t* R, const double* const* A,
+                  const size_t* ldA, const double* const* B, const size_t* ldB, const double* init,
+                  int maxSteps, int minSteps, double tol, int reortho, double* energy, double* z,
+                  double* alpha, double* beta, int* steps)
+{
+	PluginInput<double> in  = make_input<double>(nC, np, L, R, A, ldA, B, ldB);
+	std::vector<size_t> off = patch_offsets(in);
+	const size_t        n   = off[size_t(np)];
+	std::vector<double> tmp(n);
+	auto                mv = [&](double* x, const double* y) {
+                plugin_apply<double>(in, y, tmp.data());
+                for (size_t i = 0; i < n; ++i)
+                        x[i] += tmp[i];
+	};
+	return lanczos<double>(n, mv, init, maxSteps, minSteps, tol, reortho, energy, z, alpha, beta,
+	                       steps);
+}
+
+int orc_lanczos_z(int64_t nC, int64_t np, const size_t* L, const size_t* R, const double* const* A,
+                  const size_t* ldA, const double* const* B, const size_t* ldB, const double* init,
+                  int maxSteps, int minSteps, double tol, int reortho, double* energy, double* z,
+                  double* alpha, double* beta, int* steps)
+{
+	PluginInput<cplx> in = make_input<cplx>(nC, np, L, R, reinterpret_cast<const cplx* const*>(A),
+	                                        ldA, reinterpret_cast<const cplx* const*>(B), ldB);
+	std::vector<size_t> off = patch_offsets(in);
+	const size_t        n   = off[size_t(np)];
+	std::vector<cplx>   tmp(n);
+	auto                mv = [&](cplx* x, const cplx* y) {
+                plugin_apply<cplx>(in, y, tmp.data());
+                for (size_t i = 0; i < n; ++i)
+                        x[i] += tmp[i];
+	};
+	return lanczos<cplx>(n, mv, reinterpret_cast<const cplx*>(init), maxSteps, minSteps, tol, reortho,
+	                     energy, reinterpret_cast<cplx*>(z), alpha, beta, steps);
+}
+
+} // extern "C"

@@ -1,0 +1,196 @@
+"""Parity of the HIP matvec (through the C ABI) with the CPU oracle, plus the properties that
+hold at full size. GPU tests are marked ``gpu``; the C-ABI load/export and argument-error tests
+run on the CPU."""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+import dmrgpp_amd as dm
+from dmrgpp_amd import problems
+from oracle import oracle
+from tests import fixtures
+
+HEADER = "include/dmrgpp_amd.h"
+
+
+def rel_err(got, want):
+    return np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300)
+
+
+# ----------------------------------------------------------------------------- CPU tests
+
+
+def test_library_exports_every_declared_symbol():
+    import os
+    src = open(os.path.join(os.path.dirname(os.path.dirname(__file__)), HEADER)).read()
+    declared = set(re.findall(r"\b(dmk_[a-z0-9_]+)\s*\(", src))
+    assert declared, "no declarations parsed"
+    lib = ctypes.CDLL(dm.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(dm.SIGNATURES), declared ^ set(dm.SIGNATURES)
+
+
+def test_create_rejects_bad_arguments():
+    L = (ctypes.c_size_t * 2)(3, 0)
+    R = (ctypes.c_size_t * 2)(2, 2)
+    A = (ctypes.c_void_p * 4)()
+    ld = (ctypes.c_size_t * 4)(3, 3, 3, 3)
+    h = ctypes.c_void_p()
+    rc = dm.lib().dmk_create_d(1, 2, ctypes.addressof(L), ctypes.addressof(R),
+                               ctypes.addressof(A), ctypes.addressof(ld), ctypes.addressof(A),
+                               ctypes.addressof(ld), ctypes.byref(h))
+    assert rc == -1
+    assert b"patch sizes" in dm.lib().dmk_last_error()
+    rc = dm.lib().dmk_create_d(0, 2, ctypes.addressof(L), ctypes.addressof(R),
+                               ctypes.addressof(A), ctypes.addressof(ld), ctypes.addressof(A),
+                               ctypes.addressof(ld), ctypes.byref(h))
+    assert rc == -1
+    assert h.value is None
+    assert dm.lib().dmk_apply_d(None, None, None) == -1
+
+
+# ----------------------------------------------------------------------------- GPU tests
+
+
+def gpu_apply(prob, x, precision="fp64"):
+    op = dm.BatchedGemm(prob, precision)
+    dx = dm.DeviceArray.from_host(x)
+    dy = dm.DeviceArray(prob.n)
+    op.apply_Htarget(dx, dy)
+    op.sync()
+    y = dy.download()
+    op.close()
+    return y
+
+
+@pytest.mark.gpu
+def test_gpu_batchedgemm_fixture():
+    prob, vin, expected, tol = fixtures.batchedgemm_fixture()
+    got = gpu_apply(prob, vin)
+    np.testing.assert_allclose(got, expected, rtol=tol, atol=0)
+
+
+@pytest.mark.gpu
+def test_gpu_kron31_fixture():
+    full, _, y, _ = fixtures.kron31_problem()
+    want = oracle.plugin_apply(full, y)
+    got = gpu_apply(full, y)
+    assert rel_err(got, want) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,L,R,nC,density", [
+    (1, [1, 1, 1], [1, 1, 1], 2, 1.0),
+    (2, [3, 70, 5, 130], [65, 2, 64, 9], 3, 0.7),
+    (3, [64, 128, 63, 1], [64, 1, 129, 200], 4, 0.5),
+    (4, [17] * 9, [33] * 9, 5, 0.3),
+    (5, [200, 7], [3, 150], 2, 1.0),
+])
+def test_gpu_random_problems_match_oracle(seed, L, R, nC, density):
+    prob = problems.random_problem(L, R, nC, density=density, seed=seed)
+    x = np.random.default_rng(seed).uniform(-1, 1, prob.n)
+    want = oracle.plugin_apply(prob, x)
+    got = gpu_apply(prob, x)
+    assert rel_err(got, want) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_patch_without_terms_is_zeroed():
+    # Patch 1 has no (1, J, k) block at all: its output segment must be overwritten with zeros.
+    rng = np.random.default_rng(0)
+    from dmrgpp_amd.problem import KronProblem
+    blocks = {(0, 0, 0): (rng.uniform(-1, 1, (5, 5)), rng.uniform(-1, 1, (4, 4))),
+              (0, 1, 0): (rng.uniform(-1, 1, (5, 3)), rng.uniform(-1, 1, (4, 6)))}
+    prob = KronProblem(L=[5, 3], R=[4, 6], nC=1, blocks=blocks)
+    x = rng.uniform(-1, 1, prob.n)
+    op = dm.BatchedGemm(prob)
+    dx = dm.DeviceArray.from_host(x)
+    dy = dm.DeviceArray.from_host(np.full(prob.n, 7.0))
+    op.apply_Htarget(dx, dy)
+    op.sync()
+    got = dy.download()
+    assert np.all(got[20:] == 0.0)
+    assert rel_err(got, oracle.plugin_apply(prob, x)) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_identity_and_general_terms_qn_problem():
+    # QN-structured problem with identity factors (H_L (x) I, I (x) H_R) and link terms.
+    left = {-3: 7, -1: 40, 1: 70, 3: 20}
+    prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(2), seed=5)
+    x = np.random.default_rng(2).uniform(-1, 1, prob.n)
+    want = oracle.plugin_apply(prob, x)
+    op = dm.BatchedGemm(prob)
+    info = op.info()
+    assert info["nterms_identity"] == 2 * prob.npatches
+    op.close()
+    got = gpu_apply(prob, x)
+    assert rel_err(got, want) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_ladder_full_size_properties():
+    # Full-size m=2048 ladder shape: linearity, symmetry <x|Hy> = <Hx|y>, bitwise determinism,
+    # and agreement with per-block BLAS products.
+    prob = problems.ladder_m2048_problem()
+    rng = np.random.default_rng(1234)
+    x = rng.uniform(-1, 1, prob.n)
+    y = rng.uniform(-1, 1, prob.n)
+    op = dm.BatchedGemm(prob)
+    dx, dy, dz = dm.DeviceArray.from_host(x), dm.DeviceArray.from_host(y), dm.DeviceArray(prob.n)
+    hx_d, hy_d = dm.DeviceArray(prob.n), dm.DeviceArray(prob.n)
+    op.apply_Htarget(dx, hx_d)
+    op.apply_Htarget(dy, hy_d)
+    a, b = 0.75, -1.25
+    dz.upload(a * x + b * y)
+    hz_d = dm.DeviceArray(prob.n)
+    op.apply_Htarget(dz, hz_d)
+    op.sync()
+    hx, hy, hz = hx_d.download(), hy_d.download(), hz_d.download()
+    assert rel_err(hz, a * hx + b * hy) < 1e-12
+    assert abs(x @ hy - hx @ y) < 1e-10 * abs(x @ hy)
+    op.apply_Htarget(dx, hz_d)
+    op.sync()
+    assert np.array_equal(hz_d.download(), hx)
+    want = prob.apply_numpy(x)
+    assert rel_err(hx, want) < 1e-12
+    op.close()
+
+
+@pytest.mark.gpu
+def test_gpu_fp32_mode_error_bound():
+    left = {-2: 90, 0: 150, 2: 90}
+    prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(3), seed=9)
+    x = np.random.default_rng(3).uniform(-1, 1, prob.n)
+    want = oracle.plugin_apply(prob, x)
+    got = gpu_apply(prob, x, "fp32")
+    err = rel_err(got, want)
+    # FP32 products with FP32 accumulation: error ~ eps32 * sqrt(K), far below 1e-4.
+    assert 1e-10 < err < 1e-5, err
+
+
+@pytest.mark.gpu
+def test_gpu_multi_vector_and_accumulate():
+    prob = problems.random_problem([30, 70, 12], [50, 9, 66], 3, density=0.8, seed=4)
+    nvec = 4
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (nvec, prob.n))
+    op = dm.BatchedGemm(prob)
+    dX = dm.DeviceArray.from_host(X.reshape(-1))
+    dY = dm.DeviceArray(nvec * prob.n)
+    op.apply_multi(dX, dY, nvec)
+    op.sync()
+    Y = dY.download().reshape(nvec, prob.n)
+    for v in range(nvec):
+        assert np.array_equal(Y[v], gpu_apply(prob, X[v]))
+    # beta = 1 accumulates on top of the previous result.
+    op.apply_multi(dX, dY, nvec, beta=1)
+    op.sync()
+    Y2 = dY.download().reshape(nvec, prob.n)
+    np.testing.assert_allclose(Y2, 2 * Y, rtol=1e-15, atol=0)
+    # Host round trip gives the same numbers.
+    assert np.array_equal(op.apply_host(X[0]), Y[0])
+    op.close()

@@ -1,0 +1,164 @@
+"""Pins the CPU oracle (oracle/) against the reference's own fixtures and against mathematics
+that does not depend on the oracle's code paths (explicit Kronecker products, closed forms,
+LAPACK via numpy)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests import fixtures
+
+
+def test_batchedgemm_fixture():
+    # dmrg/Engine/tests/test_BatchedGemm.cpp:234-258, Approx(...).epsilon(1e-12).
+    prob, vin, expected, tol = fixtures.batchedgemm_fixture()
+    for got in (oracle.plugin_apply(prob, vin), oracle.kron_connections(prob, vin)):
+        np.testing.assert_allclose(got, expected, rtol=tol, atol=0)
+
+
+def test_kron_connections_fixture():
+    # dmrg/Engine/tests/test_KronConnections.cpp:132-178, hard-coded expected values.
+    prob = fixtures.kron_connections_fixture()
+    inp = np.array([1 + 1j, 2 - 1j])
+    x0 = np.array([3 + 1j, -1 + 2j])
+    fwd = oracle.kron_connections(prob, inp, x0=x0)
+    np.testing.assert_allclose(fwd, [10 + 15j, 1 + 6j], atol=1e-12)
+    matrix = np.array([[3 + 1j, 5j], [5 - 5j, -4]])
+    np.testing.assert_allclose(oracle.kron_form(prob), matrix, atol=1e-12)
+    np.testing.assert_allclose(matrix @ inp, [7 + 14j, 2 + 4j], atol=1e-12)
+    adj = oracle.kron_connections(prob, inp, x0=x0, apply_adjoint=True)
+    np.testing.assert_allclose(adj, x0 + matrix.conj().T @ inp, atol=1e-12)
+    # The adjoint is not the transpose for this non-Hermitian complex factor.
+    assert np.all(np.abs(matrix.conj().T @ inp - matrix.T @ inp) > 1e-12)
+    # A^H A as forward followed by adjoint.
+    hy = oracle.kron_connections(prob, inp)
+    fact = oracle.kron_connections(prob, hy, x0=x0, apply_adjoint=True)
+    np.testing.assert_allclose(fact, x0 + matrix.conj().T @ (matrix @ inp), atol=1e-12)
+    # Plugin layout computes the plain product (no conjugation, BatchedGemm semantics).
+    np.testing.assert_allclose(oracle.plugin_apply(prob, inp), matrix @ inp, atol=1e-12)
+
+
+def _np_op(m, t):
+    return {"n": m, "t": m.T, "c": m.conj().T}[t]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_den_kron_mult_methods_vs_explicit_kron(cplx):
+    # KronUtil/test1.cpp:10-60 sweep: sizes {1,4,7} x {1,4,7,10}, op in {n,t,c}, methods 1-3,
+    # against the explicit Kronecker product (den_kron_form), tol 1e-9 there, 1e-12 here.
+    rng = np.random.default_rng(7)
+    sizes_a = [1, 4, 7]
+    sizes_b = [1, 4, 7, 10]
+    for nra, nca, nrb, ncb in itertools.product(sizes_a, sizes_a, sizes_b, sizes_b):
+        if rng.random() > 0.35:
+            continue
+        a = rng.uniform(-1, 1, (nra, nca))
+        b = rng.uniform(-1, 1, (nrb, ncb))
+        if cplx:
+            a = a + 1j * rng.uniform(-1, 1, a.shape)
+            b = b + 1j * rng.uniform(-1, 1, b.shape)
+        for ta, tb in itertools.product("ntc" if cplx else "nt", repeat=2):
+            if (ta == "c") != (tb == "c"):
+                continue  # KronConnections always passes the same op for A and B
+            K = np.kron(_np_op(a, ta), _np_op(b, tb))
+            y = rng.uniform(-1, 1, K.shape[1])
+            x0 = rng.uniform(-1, 1, K.shape[0])
+            if cplx:
+                y = y + 1j * rng.uniform(-1, 1, y.shape)
+            want = x0 + K @ y
+            for m in (1, 2, 3):
+                got = oracle.den_kron_mult(m, ta, tb, a, b, y, x0)
+                np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(1, np.abs(want).max()))
+
+
+def test_estimate_kron_cost_choices():
+    # estimate_kron_cost.cpp: dense square blocks prefer method 1 when B x Y is cheaper, method 2
+    # when Y x A^T is, and very sparse blocks go to method 3.
+    assert oracle.estimate_kron_method(10, 10, 100, 10, 10, 100) == 2  # tie -> method 2 wins last
+    # A 40x10 dense, B 4x4 dense: method 1 = 2*16*10 + 2*400*4 = 3520 < method 2 = 4480.
+    assert oracle.estimate_kron_method(40, 10, 400, 4, 4, 16) == 1
+    # A 10x40: method 1 = 2*16*40 + 2*400*4 = 4480 > method 2 = 2*400*4 + 2*16*10 = 3520.
+    assert oracle.estimate_kron_method(10, 40, 400, 4, 4, 16) == 2
+    assert oracle.estimate_kron_method(100, 100, 3, 100, 100, 2) == 3
+
+
+def test_kron31_fixture_products_agree():
+    full, lower, y, ents = fixtures.kron31_problem()
+    assert full.n == len(y) == 70
+    H = oracle.kron_form(full)
+    want = H @ y
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(oracle.plugin_apply(full, y), want, atol=1e-13 * scale)
+    # The reference's own call pattern: useLowerPart with 't' on the mirrored blocks.
+    got = oracle.kron_connections(lower, y, use_lower_part=True)
+    np.testing.assert_allclose(got, want, atol=1e-13 * scale)
+    # Offsets of the fixture match the patch layout.
+    off = full.offsets
+    for e in ents:
+        assert off[e["out"]] == e["offsetX"] and off[e["in"]] == e["offsetY"]
+    # Hermiticity invariant of the assembled superblock Hamiltonian.
+    assert np.abs(H - H.T).max() < 1e-12
+
+
+def test_kron31_lanczos_matches_exact_diagonalization():
+    # input31 is the Heisenberg chain L=8; the dumped superblock is the full Sz=0 sector
+    # (70 = C(8,4) states), so its lowest eigenvalue is the ED value -3.374932599
+    # (dmrg/CMakeLists.txt:214-221 checks -3.37493 at 2e-5).
+    full, _, y, _ = fixtures.kron31_problem()
+    H = oracle.kron_form(full)
+    e_exact = np.linalg.eigvalsh(0.5 * (H + H.T))[0]
+    assert abs(e_exact - (-3.374932598687897)) < 1e-9
+    init = np.random.default_rng(1234).uniform(-1, 1, full.n)
+    e, z, steps, a, b = oracle.lanczos(full, init, max_steps=200, min_steps=4, tol=1e-12)
+    assert abs(e - e_exact) < 1e-10 * abs(e_exact)
+    zn = z / np.linalg.norm(z)
+    assert np.linalg.norm(H @ zn - e * zn) < 1e-5
+
+
+def test_tridiag_eigen_vs_lapack():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3, 10, 57, 200):
+        d = rng.uniform(-2, 2, n)
+        e = rng.uniform(-1, 1, max(n - 1, 0))
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        w, v = np.linalg.eigh(T)
+        got, z = oracle.tridiag_eigen(d, e)
+        np.testing.assert_allclose(got, w, atol=1e-12 * max(1, np.abs(w).max()))
+        # Eigenvectors up to sign.
+        for j in range(n):
+            s = np.sign(z[:, j] @ v[:, j])
+            np.testing.assert_allclose(s * z[:, j], v[:, j], atol=1e-8)
+    # Degenerate spectrum (identity) and a zero off-diagonal split.
+    got, _ = oracle.tridiag_eigen(np.ones(5), np.zeros(4))
+    np.testing.assert_allclose(got, np.ones(5))
+
+
+def test_lanczos_free_fermion_spectrum():
+    # A diagonal operator with a known spectrum written as a Kron problem: H = D_L (x) I + I (x) D_R
+    # has eigenvalues dL_i + dR_j; the lowest is min(dL) + min(dR).
+    rng = np.random.default_rng(11)
+    dl = np.sort(rng.uniform(-1, 1, 6))
+    dr = np.sort(rng.uniform(-1, 1, 5))
+    from dmrgpp_amd.problem import KronProblem
+    # One patch holding everything; dense rotated factors so that nothing is trivially diagonal.
+    ql, _ = np.linalg.qr(rng.standard_normal((6, 6)))
+    qr, _ = np.linalg.qr(rng.standard_normal((5, 5)))
+    HL = ql @ np.diag(dl) @ ql.T
+    HR = qr @ np.diag(dr) @ qr.T
+    prob = KronProblem(L=[6], R=[5], nC=2, blocks={(0, 0, 0): (HL, np.eye(5)),
+                                                   (0, 0, 1): (np.eye(6), HR)})
+    e, z, steps, _, _ = oracle.lanczos(prob, rng.uniform(-1, 1, 30), tol=1e-14, max_steps=30)
+    assert abs(e - (dl[0] + dr[0])) < 1e-12
+
+
+def test_plugin_matches_connections_on_random_problems():
+    from dmrgpp_amd import problems
+    for seed, cplx in [(1, False), (2, True)]:
+        prob = problems.random_problem([3, 5, 2, 7], [4, 1, 6, 3], nC=3, density=0.6, seed=seed,
+                                       dtype=np.complex128 if cplx else np.float64)
+        x = np.random.default_rng(seed).uniform(-1, 1, prob.n).astype(prob.dtype)
+        want = oracle.kron_form(prob) @ x
+        np.testing.assert_allclose(oracle.plugin_apply(prob, x), want, atol=1e-12)
+        np.testing.assert_allclose(oracle.kron_connections(prob, x), want, atol=1e-12)
+        np.testing.assert_allclose(prob.apply_numpy(x), want, atol=1e-12)
