@@ -61,6 +61,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
+                    help="only run the plain copy kernel on N doubles 5 times (HBM counter "
+                         "calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported (asked for {args.gpus}); this build runs the "
@@ -72,6 +75,17 @@ def main():
 
     if dm.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
+    if args.copy_calibration:
+        import ctypes
+
+        n = args.copy_calibration
+        src = dm.DeviceArray.from_host(np.ones(n))
+        dst = dm.DeviceArray(n)
+        for _ in range(5):
+            dm._check(dm.lib().dmk_device_copy_d(dst.ptr, src.ptr, ctypes.c_size_t(n)),
+                      "dmk_device_copy_d")
+        log(f"copy calibration: 5 copies of {n} doubles ({8 * n} bytes read + written each)")
+        return
 
     t0 = time.perf_counter()
     prob = problems.ladder_m2048_problem(seed=1234)

@@ -48,6 +48,25 @@ class DmkInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+class LanczosParams(ctypes.Structure):
+    _fields_ = [("max_steps", ctypes.c_int32), ("min_steps", ctypes.c_int32),
+                ("tolerance", ctypes.c_double), ("reortho", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("refine", ctypes.c_int32),
+                ("refine_min_steps", ctypes.c_int32), ("refine_tolerance", ctypes.c_double)]
+
+
+class LanczosResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("residual", ctypes.c_double),
+                ("steps", ctypes.c_int32), ("matvecs", ctypes.c_int32),
+                ("energy_first", ctypes.c_double), ("steps_first", ctypes.c_int32),
+                ("matvecs_first", ctypes.c_int32), ("precision_first", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("ms_total", ctypes.c_double),
+                ("ms_matvec", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 _lib = None
 
 # Every symbol include/dmrgpp_amd.h declares, with its ctypes signature.
@@ -66,11 +85,16 @@ SIGNATURES = {
     "dmk_sync": (_I, [_VP]),
     "dmk_profile_apply_d": (_I, [_VP, _VP, _VP, _I, ctypes.POINTER(_D), ctypes.POINTER(_D),
                                  ctypes.POINTER(_D)]),
+    "dmk_lanczos_default_params": (_I, [ctypes.POINTER(LanczosParams)]),
+    "dmk_lanczos_ground_d": (_I, [_VP, _VP, _VP, ctypes.POINTER(LanczosParams),
+                                  ctypes.POINTER(LanczosResult)]),
+    "dmk_tridiag_lowest": (_I, [_I, _VP, _VP, ctypes.POINTER(_D), _VP]),
     "dmk_device_alloc": (_I, [_SZ, ctypes.POINTER(_VP)]),
     "dmk_device_free": (_I, [_VP]),
     "dmk_memcpy_h2d": (_I, [_VP, _VP, _SZ]),
     "dmk_memcpy_d2h": (_I, [_VP, _VP, _SZ]),
     "dmk_device_count": (_I, [ctypes.POINTER(_I)]),
+    "dmk_device_copy_d": (_I, [_VP, _VP, _SZ]),
     "dmk_last_error": (ctypes.c_char_p, []),
 }
 
@@ -101,6 +125,29 @@ def device_count() -> int:
     c = ctypes.c_int(0)
     _check(lib().dmk_device_count(ctypes.byref(c)), "dmk_device_count")
     return c.value
+
+
+def tridiag_lowest(a, b):
+    """Lowest eigenpair of the Lanczos tridiagonal matrix (diagonal a, off-diagonal b[1:])."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m = a.size
+    assert b.size >= m
+    ev = ctypes.c_double()
+    vec = np.zeros(m)
+    _check(lib().dmk_tridiag_lowest(m, a.ctypes.data, b.ctypes.data, ctypes.byref(ev),
+                                    vec.ctypes.data), "dmk_tridiag_lowest")
+    return ev.value, vec
+
+
+def lanczos_params(**kw) -> LanczosParams:
+    p = LanczosParams()
+    _check(lib().dmk_lanczos_default_params(ctypes.byref(p)), "dmk_lanczos_default_params")
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        setattr(p, k, v)
+    return p
 
 
 class DeviceArray:
@@ -188,6 +235,14 @@ class BatchedGemm:
         inf = DmkInfo()
         _check(lib().dmk_get_info(self.handle, ctypes.byref(inf)), "dmk_get_info")
         return inf.as_dict()
+
+    def lanczos_ground(self, init: DeviceArray, z: DeviceArray, **kw) -> dict:
+        """Device-resident Lanczos ground state (LanczosSolver::computeAllStatesBelow)."""
+        p = lanczos_params(**kw)
+        r = LanczosResult()
+        _check(lib().dmk_lanczos_ground_d(self.handle, init.ptr, z.ptr, ctypes.byref(p),
+                                          ctypes.byref(r)), "dmk_lanczos_ground_d")
+        return r.as_dict()
 
     def profile(self, vin: DeviceArray, vout: DeviceArray, iters: int) -> tuple:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -1,6 +1,8 @@
 // extern "C" layer of dmrgpp_amd (declarations and reference citations in include/dmrgpp_amd.h).
 #include "dmrgpp_amd.h"
 #include "kron_plan.h"
+#include "lanczos.h"
+#include "tridiag.h"
 
 #include <hip/hip_runtime.h>
 
@@ -8,12 +10,16 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 struct dmk_handle {
-	std::unique_ptr<dmk::Plan> plan;
+	std::unique_ptr<dmk::Plan>             plan;
+	std::unique_ptr<dmk::LanczosWorkspace> lanczos;
 };
 
 namespace dmk {
+
+void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
 
 namespace {
 thread_local std::string g_last_error;
@@ -184,6 +190,73 @@ int dmk_profile_apply_d(dmk_handle* handle, const double* vin, double* vout, int
 	});
 }
 
+int dmk_lanczos_default_params(dmk_lanczos_params* params)
+{
+	return guarded([&] {
+		if (!params)
+			throw std::invalid_argument("dmk_lanczos_default_params: null pointer");
+		dmk::LanczosParams d;
+		std::memset(params, 0, sizeof(*params));
+		params->max_steps        = d.max_steps;
+		params->min_steps        = d.min_steps;
+		params->tolerance        = d.tolerance;
+		params->reortho          = d.reortho;
+		params->precision        = d.precision;
+		params->refine           = d.refine;
+		params->refine_min_steps = d.refine_min_steps;
+		params->refine_tolerance = d.refine_tolerance;
+		return DMK_OK;
+	});
+}
+
+int dmk_lanczos_ground_d(dmk_handle* handle, const double* init, double* z,
+                         const dmk_lanczos_params* params, dmk_lanczos_result* result)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (!params || !result)
+			throw std::invalid_argument("dmk_lanczos_ground_d: null argument");
+		dmk::LanczosParams lp;
+		lp.max_steps        = params->max_steps;
+		lp.min_steps        = params->min_steps;
+		lp.tolerance        = params->tolerance;
+		lp.reortho          = params->reortho;
+		lp.precision        = params->precision;
+		lp.refine           = params->refine;
+		lp.refine_min_steps = params->refine_min_steps;
+		lp.refine_tolerance = params->refine_tolerance;
+		if (!handle->lanczos)
+			handle->lanczos = std::make_unique<dmk::LanczosWorkspace>();
+		dmk::LanczosReport r = dmk::lanczos_ground(p, *handle->lanczos, init, z, lp);
+		std::memset(result, 0, sizeof(*result));
+		result->energy          = r.energy;
+		result->residual        = r.residual;
+		result->steps           = r.steps;
+		result->matvecs         = r.matvecs;
+		result->energy_first    = r.energy_first;
+		result->steps_first     = r.steps_first;
+		result->matvecs_first   = r.matvecs_first;
+		result->precision_first = r.precision_first;
+		result->ms_total        = r.ms_total;
+		result->ms_matvec       = r.ms_matvec;
+		return DMK_OK;
+	});
+}
+
+int dmk_tridiag_lowest(int m, const double* a, const double* b, double* eval, double* evec)
+{
+	return guarded([&] {
+		if (m < 1 || !a || !b || !eval)
+			throw std::invalid_argument("dmk_tridiag_lowest: bad arguments");
+		*eval = dmk::tridiag_eigenvalue(a, b, m, 0);
+		if (evec) {
+			std::vector<double> v = dmk::tridiag_eigenvector(a, b, m, *eval);
+			std::memcpy(evec, v.data(), size_t(m) * sizeof(double));
+		}
+		return DMK_OK;
+	});
+}
+
 int dmk_device_alloc(size_t bytes, void** ptr)
 {
 	return guarded([&] {
@@ -228,6 +301,18 @@ int dmk_device_count(int* count)
 		hipError_t e = hipGetDeviceCount(count);
 		if (e != hipSuccess)
 			*count = 0;
+		return DMK_OK;
+	});
+}
+
+int dmk_device_copy_d(double* dst, const double* src, size_t n)
+{
+	return guarded([&] {
+		if (!dst || !src)
+			throw std::invalid_argument("dmk_device_copy_d: null pointer");
+		dmk::launch_copy(dst, src, int64_t(n), nullptr);
+		DMK_HIP_CHECK(hipGetLastError());
+		DMK_HIP_CHECK(hipDeviceSynchronize());
 		return DMK_OK;
 	});
 }

@@ -110,12 +110,61 @@ int dmk_sync(dmk_handle* handle);
 int dmk_profile_apply_d(dmk_handle* handle, const double* vin, double* vout, int iters,
                         double* ms_stage1, double* ms_stage2, double* ms_total);
 
+/* Parameters of the device-resident Lanczos ground-state solver. Defaults (from
+ * dmk_lanczos_default_params) are those of PsimagLite ParametersForSolver
+ * (ParametersForSolver.h:97-128): 200 steps, at least 4, tolerance 1e-12, no re-orthogonalization. */
+typedef struct dmk_lanczos_params {
+	int32_t max_steps;
+	int32_t min_steps;
+	double  tolerance;        /* stop when the lowest Ritz value moves less than this */
+	int32_t reortho;          /* 1: full Gram-Schmidt each step (LanczosVectors.h:243-271) */
+	int32_t precision;        /* matvec precision of the first run (enum dmk_precision) */
+	int32_t refine;           /* 1: after a non-FP64 run, restart in FP64 from its Ritz vector */
+	int32_t refine_min_steps;
+	double  refine_tolerance;
+} dmk_lanczos_params;
+
+typedef struct dmk_lanczos_result {
+	double  energy;           /* lowest eigenvalue (after refinement when it ran) */
+	double  residual;         /* ||H z - E z|| / ||z|| evaluated with the FP64 matvec */
+	int32_t steps;            /* Lanczos steps of the last run */
+	int32_t matvecs;          /* all matvecs of the solve (both runs) */
+	double  energy_first;     /* energy of the first run, in its own precision */
+	int32_t steps_first;
+	int32_t matvecs_first;
+	int32_t precision_first;
+	int32_t pad;
+	double  ms_total;         /* wall time of the solve */
+	double  ms_matvec;        /* device time inside the matvec launches */
+} dmk_lanczos_result;
+
+int dmk_lanczos_default_params(dmk_lanczos_params* params);
+
+/* Lowest eigenpair of the handle's H: LanczosSolver::computeAllStatesBelow for the ground state
+ * (PsimagLite/LanczosSolver.h:68-88) over LanczosCore::decomposition (LanczosCore.h:147-272) and
+ * LanczosVectors::oneStepDecomposition / excitedVector (LanczosVectors.h:192-237), with the
+ * Lanczos vectors resident on the device. init and z are device vectors of n elements;
+ * z receives the (unnormalized Ritz combination of unit Lanczos vectors) ground vector. */
+int dmk_lanczos_ground_d(dmk_handle* handle, const double* init, double* z,
+                         const dmk_lanczos_params* params, dmk_lanczos_result* result);
+
+/* Lowest eigenvalue and unit eigenvector of the m x m symmetric tridiagonal matrix with diagonal
+ * a[0..m) and off-diagonal b[1..m) (T(i,i+1) = b[i+1]), as TridiagonalMatrix::buildDenseMatrix
+ * lays it out (TridiagonalMatrix.h:149-163); replaces its LAPACK dsyev call (Matrix.cpp:15-45)
+ * for the ground state. evec may be NULL. Host arrays. */
+int dmk_tridiag_lowest(int m, const double* a, const double* b, double* eval, double* evec);
+
 /* Device memory helpers for callers without their own allocator. */
 int dmk_device_alloc(size_t bytes, void** ptr);
 int dmk_device_free(void* ptr);
 int dmk_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int dmk_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int dmk_device_count(int* count);
+
+/* dst[i] = src[i], i < n, by a plain grid-stride copy kernel on the default stream (synchronous).
+ * No reference counterpart: it calibrates HBM byte counters (FETCH_SIZE / WRITE_SIZE) against a
+ * kernel of known traffic. */
+int dmk_device_copy_d(double* dst, const double* src, size_t n);
 
 const char* dmk_last_error(void);
 
