@@ -1,17 +1,18 @@
 """Benchmark of the superblock Kron matvec on one MI355X.
 
-Workload: the superblock of the Heisenberg 32x2 two-leg ladder at m = 2048 kept states,
-Sz_total = 0 (the m = 2048 ladder configuration): 17 patches, 11 connections (H_L (x) I,
-I (x) H_R, three Heisenberg bonds across the cut), block sizes from the symmetry-sector
-histogram, values U(-1,1) from seed 1234 (synthetic; no DMRG run produces it here).
+Workload: the Heisenberg two-leg ladder 32x2 (J_leg = J_rung = 1, Sz = 0) at m = 2048 kept
+states. The sweep driver grows the system by the infinite-size algorithm with the eigensolve on
+the GPU; the superblock of its last step (the middle of the 64-site ladder: left block S+s and
+right block e+E of 2m states each) is the matvec workload. Untimed setup.
 
-A step is one matvec y = H x with x and y resident in HBM. value = matvecs per second in the
-FP64 path; the FP32-MFMA rate and its error against FP64 are reported beside it. The roofline
-entry is the dominant grouped-GEMM launch: algorithmic flops of that launch / its average
-device time from HIP events on the launch stream. cpu_baseline times the KronConnections
-restatement with numpy BLAS GEMMs on a bounded number of matvecs.
+A step is one matvec y = H x with x and y resident in HBM; value = FP64 matvecs per second.
+Beside it: the FP32-MFMA matvec rate and its error against FP64; the ground state of the same
+superblock by the device Lanczos in FP64 and in FP32 + FP64 refinement (energy difference, the
+|dE| < 1e-10 gate); a bounded finite-sweep segment in both modes with the estimated full-sweep
+time; the roofline of the dominant grouped-GEMM launch from HIP events on its stream; and the
+CPU baseline (KronConnections restatement with numpy BLAS) on a bounded number of matvecs.
 
-Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--sweep-steps 6] [--no-cpu]
 """
 import argparse
 import json
@@ -25,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+M = 2048
+LX = 32
 
 
 def log(*a):
@@ -50,9 +53,25 @@ def cpu_baseline(prob, x, budget_s=15.0, max_calls=20):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": calls / dt, "unit": "matvecs/s", "cores": int(threads), "kind": "port",
+    return {"value": round(calls / dt, 4), "unit": "matvecs/s", "cores": int(threads),
+            "kind": "port",
             "sample": f"{calls} matvecs of the same m=2048 ladder superblock (KronConnections "
                       f"restatement, numpy BLAS GEMMs, {threads} BLAS threads), {dt:.1f} s"}
+
+
+class Recorder:
+    """GPU solver that keeps the last superblock it solved."""
+
+    def __init__(self, inner):
+        self.inner = inner
+        self.prob = None
+        self.energy = None
+
+    def __call__(self, prob):
+        self.prob = prob
+        e, z, r = self.inner(prob)
+        self.energy = e
+        return e, z, r
 
 
 def main():
@@ -60,6 +79,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--sweep-steps", type=int, default=6,
+                    help="finite-sweep DMRG steps timed per precision mode (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel on N doubles 5 times (HBM counter "
@@ -71,7 +92,7 @@ def main():
         sys.exit(2)
 
     import dmrgpp_amd as dm
-    from dmrgpp_amd import problems
+    from dmrgpp_amd import models, sweep
 
     if dm.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
@@ -87,9 +108,16 @@ def main():
         log(f"copy calibration: 5 copies of {n} doubles ({8 * n} bytes read + written each)")
         return
 
+    # ---- setup: infinite-size growth of the 32x2 ladder at m=2048 (untimed)
     t0 = time.perf_counter()
-    prob = problems.ladder_m2048_problem(seed=1234)
-    log(f"problem {prob.name}: {prob.histogram()} ({time.perf_counter() - t0:.1f} s)")
+    model, lattice = models.heisenberg(1.0), models.ladder(LX, 2)
+    rec = Recorder(sweep.GpuSolver(precision="fp64"))
+    state = {}
+    e_inf, recs_inf = sweep.run(model, lattice, lambda N: (0,), M, [], rec, log=log, state=state)
+    prob = rec.prob
+    t_setup = time.perf_counter() - t0
+    log(f"setup: infinite growth {len(recs_inf)} steps in {t_setup:.1f} s; superblock "
+        f"{prob.histogram()}")
     x = np.random.default_rng(1234).uniform(-1.0, 1.0, prob.n)
 
     op = dm.BatchedGemm(prob, "fp64")
@@ -101,6 +129,7 @@ def main():
     dx = dm.DeviceArray.from_host(x)
     dy = dm.DeviceArray(prob.n)
 
+    # ---- timed matvecs
     results = {}
     for prec in ("fp64", "fp32"):
         op.set_precision(prec)
@@ -120,11 +149,41 @@ def main():
             f"stage2 {ms2:.3f} ms, total {mst:.3f} ms)")
     y64 = results["fp64"]["y"]
     fp32_err = float(np.linalg.norm(results["fp32"]["y"] - y64) / np.linalg.norm(y64))
+
+    # ---- ground state of the same superblock: FP64 vs FP32 + FP64 refinement
+    dz = dm.DeviceArray(prob.n)
+    op.set_precision("fp64")
+    lz = {}
+    for mode, kw in (("fp64", {}), ("fp32+refine", {"precision": "fp32", "refine": 1})):
+        r = op.lanczos_ground(dx, dz, **kw)
+        lz[mode] = r
+        log(f"lanczos {mode}: E={r['energy']:.14f} steps={r['steps']} matvecs={r['matvecs']} "
+            f"residual={r['residual']:.2e} {r['ms_total']:.1f} ms")
+    dE = abs(lz["fp32+refine"]["energy"] - lz["fp64"]["energy"])
     op.close()
+
+    # ---- bounded finite-sweep segment in both modes, continuing from the grown blocks
+    sweep_out = None
+    if args.sweep_steps > 0:
+        sweep_out = {}
+        steps_per_sweep = 2 * (lattice.nsites - 3)
+        for mode, solver in (("fp64", sweep.GpuSolver(precision="fp64")),
+                             ("fp32+refine", sweep.GpuSolver(precision="fp32", refine=1))):
+            st = sweep.clone_state(state)
+            t = time.perf_counter()
+            e, recs = sweep.run(model, lattice, lambda N: (0,), M, [M], solver, log=log,
+                                state=st, resume=True, stop_after=args.sweep_steps)
+            dt = time.perf_counter() - t
+            per = dt / max(1, len(recs))
+            sweep_out[mode] = {"steps": len(recs), "s_per_step": round(per, 4),
+                               "sweep_s_est": round(per * steps_per_sweep, 2),
+                               "lanczos_matvecs": int(sum(r.matvecs for r in recs)),
+                               "energy_last": e}
+            log(f"sweep segment {mode}: {len(recs)} steps, {per:.3f} s/step, "
+                f"full sweep ~{per * steps_per_sweep:.1f} s")
 
     r = results["fp64"]
     value = 1e3 / r["ms_per_step"]
-    # Dominant kernel: the longer of the two grouped-GEMM launches.
     if r["ms_stage2"] >= r["ms_stage1"]:
         kname, kms, kflops = "stage2", r["ms_stage2"], info["flops_stage2"]
     else:
@@ -152,8 +211,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (symmetry-sector histogram of the m=2048 ladder superblock, "
-                "U(-1,1) block values, seed 1234)",
+        "data": "synthetic model run: the superblock of the last infinite-growth step of the "
+                "Heisenberg 32x2 ladder at m=2048 (GPU Lanczos, seed-1234 U[-1,1) start vectors)",
         "config": {"workload": "heisenberg-ladder-32x2-m2048-superblock-matvec", "n": prob.n,
                    "npatches": prob.npatches, "nC": prob.nC, "nterms": len(prob.blocks),
                    "flops_per_matvec": info["flops_algorithmic"]},
@@ -166,6 +225,18 @@ def main():
                                       / 1e12, 3)},
         "fp64_tflops_per_matvec": round(info["flops_algorithmic"] / (r["ms_total_events"] * 1e-3)
                                         / 1e12, 3),
+        "ground_state": {"energy_fp64": lz["fp64"]["energy"],
+                         "energy_mixed": lz["fp32+refine"]["energy"],
+                         "abs_dE": dE, "rel_dE": dE / abs(lz["fp64"]["energy"]),
+                         "gate_1e-10": bool(dE < 1e-10 * abs(lz["fp64"]["energy"])),
+                         "ms_fp64": round(lz["fp64"]["ms_total"], 2),
+                         "ms_mixed": round(lz["fp32+refine"]["ms_total"], 2),
+                         "matvecs_fp64": lz["fp64"]["matvecs"],
+                         "matvecs_mixed": lz["fp32+refine"]["matvecs"],
+                         "residual_fp64": lz["fp64"]["residual"],
+                         "residual_mixed": lz["fp32+refine"]["residual"]},
+        "sweep": sweep_out,
+        "setup_s": round(t_setup, 1),
     }
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,80 @@
+"""The benchmark workloads as sweep-driver settings, and a command-line runner.
+
+  python -m dmrgpp_amd.configs --config heisenberg-chain-32 [--precision fp64|fp32] [--sweeps N]
+                               [--stop-after K] [--json out.json]
+
+Each run prints one line per DMRG step and, with --json, writes the step records and the total
+sweep wall time.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+from . import models, sweep
+
+CONFIGS = {
+    # Spin-1/2 Heisenberg chain, L=32, OBC, J=1, Sz=0; infinite growth then 3 finite sweeps at
+    # m=128; Lanczos tol 1e-12, <= 200 steps, initial vectors U[-1,1) from seed 1234. The
+    # reference's own pin of this model is input3100 (-13.9973 at 2e-5, dmrg/CMakeLists.txt:343).
+    "heisenberg-chain-32": dict(model=lambda: models.heisenberg(1.0), lattice=lambda: models.chain(32),
+                                target=lambda N: (0,), m=128, sweeps=3),
+    # Fermi-Hubbard chain L=48, U=8, t=1, half filling, m=1024, 4 sweeps.
+    "hubbard-chain-48": dict(model=lambda: models.hubbard(8.0), lattice=lambda: models.chain(48),
+                             target=lambda N: (N // 2, N // 2), m=1024, sweeps=4),
+    # Heisenberg two-leg ladder 32x2, J_leg = J_rung = 1, Sz=0, m=2048.
+    "heisenberg-ladder-32x2": dict(model=lambda: models.heisenberg(1.0),
+                                   lattice=lambda: models.ladder(32, 2), target=lambda N: (0,),
+                                   m=2048, sweeps=2),
+    # Small cases used by the tests.
+    "heisenberg-chain-8": dict(model=lambda: models.heisenberg(1.0), lattice=lambda: models.chain(8),
+                               target=lambda N: (0,), m=16, sweeps=1),
+    "heisenberg-ladder-4x2": dict(model=lambda: models.heisenberg(1.0),
+                                  lattice=lambda: models.ladder(4, 2), target=lambda N: (0,),
+                                  m=16, sweeps=1),
+    "hubbard-chain-8-U0": dict(model=lambda: models.hubbard(0.0), lattice=lambda: models.chain(8),
+                               target=lambda N: (N // 2, N // 2), m=256, sweeps=1),
+}
+
+
+def run_config(name: str, precision="fp64", m=None, sweeps=None, stop_after=None, log=print,
+               solver=None):
+    c = CONFIGS[name]
+    m = m or c["m"]
+    nsw = c["sweeps"] if sweeps is None else sweeps
+    solver = solver or sweep.GpuSolver(precision=precision)
+    t0 = time.perf_counter()
+    e, recs = sweep.run(c["model"](), c["lattice"](), c["target"], m, [m] * nsw, solver, log=log,
+                        stop_after=stop_after)
+    wall = time.perf_counter() - t0
+    return e, recs, wall
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", required=True, choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--m", type=int, default=None)
+    ap.add_argument("--sweeps", type=int, default=None)
+    ap.add_argument("--stop-after", type=int, default=None)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    e, recs, wall = run_config(a.config, a.precision, a.m, a.sweeps, a.stop_after,
+                               log=lambda s: print(s, flush=True))
+    sweep_steps = [r for r in recs if r.phase != "infinite"]
+    out = {"config": a.config, "precision": a.precision, "energy": e, "wall_s": wall,
+           "steps": len(recs), "matvecs": sum(r.matvecs for r in recs),
+           "ms_matvec": sum(r.ms_matvec for r in recs),
+           "ms_lanczos": sum(r.ms_lanczos for r in recs),
+           "sweep_wall_s_est": sum(r.ms_lanczos for r in sweep_steps) / 1e3,
+           "records": [r.__dict__ for r in recs]}
+    print(json.dumps({k: v for k, v in out.items() if k != "records"}), flush=True)
+    if a.json:
+        with open(a.json, "w") as fh:
+            json.dump(out, fh)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -31,8 +31,10 @@ class KronProblem:
         assert np.all(self.L >= 1) and np.all(self.R >= 1)
         for key, (a, b) in list(self.blocks.items()):
             I, J, k = key
-            a = np.asfortranarray(a, dtype=self.dtype)
-            b = np.asfortranarray(b, dtype=self.dtype)
+            if not (a.flags.f_contiguous and a.dtype == self.dtype):
+                a = np.asfortranarray(a, dtype=self.dtype)
+            if not (b.flags.f_contiguous and b.dtype == self.dtype):
+                b = np.asfortranarray(b, dtype=self.dtype)
             assert a.shape == (self.L[I], self.L[J]), (key, a.shape)
             assert b.shape == (self.R[I], self.R[J]), (key, b.shape)
             assert 0 <= k < self.nC
@@ -105,8 +107,6 @@ class PluginArgs:
             self.ldB[i] = int(prob.R[I])
         for (I, J, k), (a, b) in prob.blocks.items():
             i = prob.index(I, J, k)
-            a = np.asfortranarray(a)
-            b = np.asfortranarray(b)
             self._keep += [a, b]
             self.A[i] = a.ctypes.data
             self.B[i] = b.ctypes.data

@@ -1,0 +1,320 @@
+"""A minimal two-site DMRG sweep driver that produces the superblock Kron problems and the sweep
+schedules of the benchmark configurations, with the eigensolve on the GPU through the C ABI.
+
+This is the caller of the hot path, not part of it: DMRG++'s own driver (DmrgSolver.h:219-603,
+Diagonalization.h, Truncation.h) is out of scope. The step structure follows it: the left block
+S+s and the right block e+E (InitKronHamiltonian.h:114-145), connections H_L (x) I, I (x) H_R,
+then one term per link crossing the cut with the fermion sign folded into A
+(InitKronBase.h:392-413), the lowest state of the target sector by Lanczos, then the reduced
+density matrix of the grown block diagonalized per quantum-number group in FP64 and truncated to
+the m largest eigenvalues.
+
+Infinite-size growth builds left blocks from site 0 and right blocks from site L-1
+simultaneously; finite sweeps then move the two-site center right to the end, left to the
+beginning and right back to the middle.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .blocks import Basis, BSOp, Product, sub_qn
+from .models import Lattice, SiteModel
+from .problem import KronProblem
+
+
+@dataclass
+class Block:
+    sites: list
+    basis: Basis
+    H: BSOp
+    ops: dict       # (site, name) -> BSOp
+    parity: np.ndarray
+
+    @property
+    def dim(self):
+        return self.basis.dim
+
+
+def _parity_of(model: SiteModel, qn: tuple) -> float:
+    if model.name == "hubbard":
+        return -1.0 if sum(qn) % 2 else 1.0
+    return 1.0
+
+
+def site_block(model: SiteModel, site: int) -> Block:
+    ops = {(site, name): op for name, op in model.ops.items()}
+    return Block([site], model.basis, model.h_site, ops,
+                 np.array([_parity_of(model, q) for q in model.basis.qns]))
+
+
+def cross_bonds(lattice: Lattice, A: Block, B: Block, offset: int = 0) -> list:
+    """Bonds (i, j, value) from a site of A to a site of B that both carry operators. B's sites
+    sit at j - offset in the (possibly shorter) system the superblock represents."""
+    sa = sorted({s for s, _ in A.ops})
+    sb = sorted({s for s, _ in B.ops})
+    out = []
+    for i in sa:
+        for j in sb:
+            v = lattice.bond_value(i, j - offset)
+            if v is not None and j - offset > i:
+                out.append((i, j, v))
+    return out
+
+
+def grow(A: Block, B: Block, model: SiteModel, lattice: Lattice, keep) -> Block:
+    """A (x) B with A's modes first: H, and the operators of the sites for which keep(s)."""
+    P = Product(A.basis, B.basis)
+    H = P.kron(A.H, None)
+    H.add_(P.kron(None, B.H))
+    for i, j, v in cross_bonds(lattice, A, B):
+        for oi, oj, c, ferm in model.bond_terms(v):
+            a = A.ops[(i, oi)]
+            if ferm:
+                a = a.times_parity_in(A.parity)
+            H.add_(P.kron(a, B.ops[(j, oj)], c))
+    ops = {}
+    pA = BSOp(A.basis, A.basis, {(g, g): np.eye(s) * A.parity[g] for g, s in enumerate(A.basis.sizes)})
+    for (s, name), op in A.ops.items():
+        if keep(s):
+            ops[(s, name)] = P.kron(op, None)
+            ops[(s, name)].fermionic = op.fermionic
+    for (s, name), op in B.ops.items():
+        if keep(s):
+            ops[(s, name)] = P.kron(pA if op.fermionic else None, op)
+            ops[(s, name)].fermionic = op.fermionic
+    parity = np.array([_parity_of(model, q) for q in P.basis.qns])
+    return Block(sorted(A.sites + B.sites), P.basis, H, ops, parity)
+
+
+def kron_problem(Lb: Block, Rb: Block, model: SiteModel, lattice: Lattice, target: tuple,
+                 name: str = "", offset: int = 0):
+    """The superblock decomposition for target sector `target` (patches, A_k, B_k). The right
+    block's sites sit at site - offset in the system the superblock represents."""
+    patches = []
+    for gL, qL in enumerate(Lb.basis.qns):
+        gR = Rb.basis.index.get(sub_qn(target, qL))
+        if gR is not None:
+            patches.append((gL, gR))
+    if not patches:
+        raise ValueError(f"no patch for target {target}")
+    pl = {gL: I for I, (gL, _) in enumerate(patches)}
+    pr = {gR: I for I, (_, gR) in enumerate(patches)}
+    L = [Lb.basis.sizes[gL] for gL, _ in patches]
+    R = [Rb.basis.sizes[gR] for _, gR in patches]
+    conns = [(Lb.H, None, 1.0), (None, Rb.H, 1.0)]
+    for i, j, v in cross_bonds(lattice, Lb, Rb, offset):
+        for oi, oj, c, ferm in model.bond_terms(v):
+            a = Lb.ops[(i, oi)]
+            if ferm:
+                a = a.times_parity_in(Lb.parity)
+            conns.append((a, Rb.ops[(j, oj)], c))
+    eyes = {}
+
+    def eye(nn):
+        # Identity factors are shared between blocks (read-only, column-major).
+        if nn not in eyes:
+            eyes[nn] = np.asfortranarray(np.eye(nn))
+        return eyes[nn]
+
+    def fortran(a, c):
+        return np.multiply(a, c, order="F") if c != 1.0 else np.asfortranarray(a)
+
+    blocks = {}
+    for k, (A, B, c) in enumerate(conns):
+        if A is None:
+            for (go, gi), b in B.blocks.items():
+                I, J = pr.get(go), pr.get(gi)
+                if I is None or J is None or I != J:
+                    continue
+                blocks[(I, J, k)] = (c * eye(L[I]) if c != 1.0 else eye(L[I]), fortran(b, 1.0))
+            continue
+        for (go, gi), a in A.blocks.items():
+            I, J = pl.get(go), pl.get(gi)
+            if I is None or J is None:
+                continue
+            if B is None:
+                if I == J:
+                    blocks[(I, J, k)] = (fortran(a, c), eye(R[I]))
+                continue
+            b = B.blocks.get((patches[I][1], patches[J][1]))
+            if b is not None:
+                blocks[(I, J, k)] = (fortran(a, c), fortran(b, 1.0))
+    return KronProblem(L=L, R=R, nC=len(conns), blocks=blocks, name=name), patches
+
+
+def truncate(block: Block, psi_blocks: list, patches: list, side: str, m: int,
+             model: SiteModel):
+    """Keeps the m largest eigenvalues of the block's reduced density matrix.
+
+    psi_blocks[I] is the R_I x L_I ground-state segment of patch I. For the left block
+    rho_g = X^T X, for the right block rho_g = X X^T (each group meets one patch).
+    Returns (new block, discarded weight)."""
+    cand = []
+    vecs = {}
+    for I, (gL, gR) in enumerate(patches):
+        X = psi_blocks[I]
+        if side == "left":
+            g, rho = gL, X.T @ X
+        else:
+            g, rho = gR, X @ X.T
+        w, v = np.linalg.eigh(0.5 * (rho + rho.T))
+        vecs[g] = v
+        for t, val in enumerate(w):
+            cand.append((val, g, t))
+    cand.sort(key=lambda t: -t[0])
+    kept = cand[:m]
+    # Weight of the dropped states, summed directly (1 - kept cancels at m = 2048).
+    discarded = float(sum(max(c[0], 0.0) for c in cand[m:]))
+    per = {}
+    for val, g, t in kept:
+        per.setdefault(g, []).append(t)
+    groups = sorted(per, key=lambda g: block.basis.qns[g])
+    nb = Basis([block.basis.qns[g] for g in groups], [len(per[g]) for g in groups])
+    W = {}
+    for ng, g in enumerate(groups):
+        cols = sorted(per[g])
+        W[g] = (vecs[g][:, cols], (ng, len(cols)))
+    H = block.H.transform(W, nb, W, nb)
+    ops = {}
+    for key, op in block.ops.items():
+        ops[key] = op.transform(W, nb, W, nb)
+    parity = np.array([_parity_of(model, q) for q in nb.qns])
+    return Block(block.sites, nb, H, ops, parity), discarded
+
+
+@dataclass
+class StepRecord:
+    phase: str
+    center: int
+    m: int
+    n: int
+    npatches: int
+    nterms: int
+    energy: float
+    lanczos_steps: int
+    matvecs: int
+    ms_lanczos: float
+    ms_matvec: float
+    discarded: float
+    info: dict = field(default_factory=dict)
+
+
+class GpuSolver:
+    """Ground state of a Kron problem on the GPU (C-ABI plugin + device Lanczos)."""
+
+    def __init__(self, precision="fp64", refine=1, max_steps=200, min_steps=4, tol=1e-12,
+                 seed=1234):
+        self.precision, self.refine = precision, refine
+        self.max_steps, self.min_steps, self.tol, self.seed = max_steps, min_steps, tol, seed
+
+    def __call__(self, prob: KronProblem):
+        from . import BatchedGemm, DeviceArray
+
+        init = np.random.default_rng(self.seed).uniform(-1.0, 1.0, prob.n)
+        op = BatchedGemm(prob, "fp64")
+        info = op.info()
+        dinit = DeviceArray.from_host(init)
+        dz = DeviceArray(prob.n)
+        r = op.lanczos_ground(dinit, dz, max_steps=self.max_steps, min_steps=self.min_steps,
+                              tolerance=self.tol, precision=self.precision, refine=self.refine,
+                              refine_tolerance=self.tol)
+        z = dz.download()
+        op.close()
+        dinit.free()
+        dz.free()
+        r["plan"] = {k: info[k] for k in ("nterms", "flops_algorithmic", "stage1_tiles",
+                                          "stage2_tiles")}
+        return r["energy"], z, r
+
+
+def clone_state(state: dict) -> dict:
+    """A copy of the block stores that a later run can advance without touching the original
+    (blocks themselves are never modified in place)."""
+    return {"left": dict(state["left"]), "right": dict(state["right"])}
+
+
+def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: list,
+        solver, log=print, stop_after=None, state: dict | None = None, resume: bool = False):
+    """Infinite growth to the full lattice, then finite sweeps.
+
+    target_of(nsites) gives the target quantum number of a superblock of nsites sites;
+    sweeps is a list of m values, one per finite sweep (right to the end, left to the
+    beginning, right back to the middle). state, when given, receives the left/right block
+    stores; with resume=True the infinite growth is skipped and the sweeps start from them.
+    Returns (final energy, records)."""
+    Ls = lattice.nsites
+    assert Ls % 2 == 0 and Ls >= 4
+    sites = [site_block(model, s) for s in range(Ls)]
+    if state is None:
+        state = {}
+    if resume:
+        left, right = state["left"], state["right"]
+    else:
+        left = {1: sites[0]}
+        right = {1: sites[Ls - 1]}
+        state["left"], state["right"] = left, right
+    records = []
+    energy = None
+
+    W = lattice.max_range
+
+    def step(phase, lsize, rsize, m, trunc):
+        nonlocal energy
+        S = left[lsize]
+        E = right[rsize]
+        l = lsize  # the left block's added site
+        r = Ls - rsize - 1  # the right block's added site
+        # Operators are kept for sites within the bond range of the block's open end.
+        Lb = grow(S, sites[l], model, lattice, lambda s: l - s < W)
+        Rb = grow(sites[r], E, model, lattice, lambda s: s - r < W)
+        nsuper = lsize + rsize + 2
+        target = target_of(lsize + rsize + 2)
+        prob, patches = kron_problem(Lb, Rb, model, lattice, target,
+                                     name=f"{model.name}-{phase}-{l}", offset=Ls - nsuper)
+        t0 = time.perf_counter()
+        e, z, rep = solver(prob)
+        t1 = time.perf_counter()
+        off = prob.offsets
+        psi = [z[off[I]:off[I + 1]].reshape((prob.R[I], prob.L[I]), order="F")
+               for I in range(prob.npatches)]
+        nrm = np.linalg.norm(z)
+        psi = [p / nrm for p in psi]
+        disc = 0.0
+        if "left" in trunc:
+            left[lsize + 1], d = truncate(Lb, psi, patches, "left", m, model)
+            disc = max(disc, d)
+        if "right" in trunc:
+            right[rsize + 1], d = truncate(Rb, psi, patches, "right", m, model)
+            disc = max(disc, d)
+        energy = e
+        rec = StepRecord(phase, l, m, prob.n, prob.npatches, len(prob.blocks), e,
+                         rep.get("steps", 0), rep.get("matvecs", 0), 1e3 * (t1 - t0),
+                         rep.get("ms_matvec", 0.0), disc, rep)
+        records.append(rec)
+        log(f"{phase:8s} center={l:3d} m={m:5d} n={prob.n:9d} patches={prob.npatches:4d} "
+            f"terms={len(prob.blocks):5d} E={e:.12f} steps={rec.lanczos_steps:3d} "
+            f"lanczos={rec.ms_lanczos:8.1f}ms disc={disc:.2e}")
+        return rec
+
+    # Infinite-size growth: superblock of 2*lsize + 2 sites.
+    lsize = 1
+    while not resume and 2 * lsize + 2 <= Ls:
+        step("infinite", lsize, lsize, m_infinite, ("left", "right"))
+        lsize += 1
+        if stop_after and len(records) >= stop_after:
+            return energy, records
+    # Finite sweeps. The center's left block has lsize sites, right block Ls - lsize - 2.
+    half = Ls // 2 - 1
+    schedule = []
+    for m in sweeps:
+        schedule += [("sweep>", l, m, ("left",)) for l in range(half, Ls - 3)]   # to the right end
+        schedule += [("sweep<", l, m, ("right",)) for l in range(Ls - 3, 1, -1)]  # to the left end
+        schedule += [("sweep>", l, m, ("left",)) for l in range(1, half)]       # back to the middle
+    for phase, l, m, trunc in schedule:
+        step(phase, l, Ls - l - 2, m, trunc)
+        if stop_after and len(records) >= stop_after:
+            break
+    return energy, records

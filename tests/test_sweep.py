@@ -1,0 +1,119 @@
+"""The sweep driver that produces real superblock Kron problems: exact energies on small
+systems (ED, free-fermion closed form), structure of the decomposition, and on the GPU the same
+energies through the C-ABI plugin and device Lanczos."""
+import numpy as np
+import pytest
+
+from dmrgpp_amd import configs, models, sweep
+from oracle import oracle
+
+
+class OracleSolver:
+    """Lanczos of the oracle (LanczosCore restatement) on the plugin-layout problem."""
+
+    def __call__(self, prob):
+        init = np.random.default_rng(1234).uniform(-1, 1, prob.n)
+        e, z, s, _, _ = oracle.lanczos(prob, init)
+        return e, z, {"steps": s, "matvecs": s}
+
+
+E_HEIS_L8 = -3.374932598687897        # ED (dmrg/CMakeLists.txt:199-200 pins -3.37493)
+E_LADDER_4x2 = -4.293066457           # ED (dmrg/CMakeLists.txt:212-213 pins -4.29307)
+E_HUB_U0_L8 = 2 * sum(-2 * np.cos(k * np.pi / 9) for k in range(1, 5))  # -9.517540966
+
+
+def test_sweep_heisenberg_chain8_exact():
+    e, recs, _ = configs.run_config("heisenberg-chain-8", solver=OracleSolver(), log=lambda s: None)
+    assert abs(e - E_HEIS_L8) < 1e-10
+
+
+def test_sweep_heisenberg_ladder_exact():
+    e, recs, _ = configs.run_config("heisenberg-ladder-4x2", solver=OracleSolver(),
+                                    log=lambda s: None)
+    assert abs(e - E_LADDER_4x2) < 1e-8
+
+
+def test_sweep_hubbard_free_fermions_exact():
+    # U = 0, 8 sites, 4 up 4 down: 2 * sum_{k=1..4} -2 cos(k pi / 9) (input2 / input2_batchedgemm).
+    e, recs, _ = configs.run_config("hubbard-chain-8-U0", solver=OracleSolver(), log=lambda s: None)
+    assert abs(e - E_HUB_U0_L8) < 1e-10
+
+
+def _ed_hubbard_chain(L, U, t=1.0):
+    nm = 2 * L
+    dim = 2 ** nm
+
+    def c(mode):
+        M = np.zeros((dim, dim))
+        for s in range(dim):
+            if s >> mode & 1:
+                M[s ^ (1 << mode), s] = (-1) ** bin(s & ((1 << mode) - 1)).count("1")
+        return M
+
+    C = [c(m) for m in range(nm)]
+    H = np.zeros((dim, dim))
+    for i in range(L - 1):
+        for sp in range(2):
+            a, b = C[2 * i + sp], C[2 * (i + 1) + sp]
+            H += -t * (a.T @ b + b.T @ a)
+    for i in range(L):
+        H += U * (C[2 * i].T @ C[2 * i]) @ (C[2 * i + 1].T @ C[2 * i + 1])
+    nup = sum(C[2 * i].T @ C[2 * i] for i in range(L)).diagonal()
+    ndn = sum(C[2 * i + 1].T @ C[2 * i + 1] for i in range(L)).diagonal()
+    idx = np.where((nup == L // 2) & (ndn == L // 2))[0]
+    return np.linalg.eigvalsh(H[np.ix_(idx, idx)])[0]
+
+
+def test_sweep_hubbard_interacting_matches_ed():
+    e_ed = _ed_hubbard_chain(6, 4.0)
+    e, _ = sweep.run(models.hubbard(4.0), models.chain(6), lambda N: (N // 2, N // 2), 64, [64],
+                     OracleSolver(), log=lambda s: None)
+    assert abs(e - e_ed) < 1e-10
+
+
+def test_kron_problem_structure_and_hermiticity():
+    # A mid-sweep superblock of the Heisenberg ladder: connections H_L (x) I, I (x) H_R and three
+    # bonds x three terms; the assembled H is symmetric and its blocks respect the patches.
+    model, lat = models.heisenberg(), models.ladder(4, 2)
+    captured = {}
+
+    class Capture(OracleSolver):
+        def __call__(self, prob):
+            captured.setdefault("prob", prob)
+            return super().__call__(prob)
+
+    sweep.run(model, lat, lambda N: (0,), 16, [], Capture(), log=lambda s: None, stop_after=2)
+    prob = captured["prob"]
+    H = oracle.kron_form(prob)
+    assert np.abs(H - H.T).max() < 1e-12
+    for (I, J, k), (a, b) in prob.blocks.items():
+        if k == 0:
+            assert I == J and np.array_equal(b, np.eye(prob.R[I]))
+        if k == 1:
+            assert I == J and np.array_equal(a, np.eye(prob.L[I]))
+
+
+def test_lattice_bonds():
+    lat = models.ladder(3, 2)
+    assert sorted(lat.bonds) == [(0, 1, 1.0), (0, 2, 1.0), (1, 3, 1.0), (2, 3, 1.0), (2, 4, 1.0),
+                                 (3, 5, 1.0), (4, 5, 1.0)]
+    cyl = models.ladder(2, 4, periodic_y=True)
+    assert (0, 3, 1.0) in cyl.bonds and (4, 7, 1.0) in cyl.bonds and len(cyl.bonds) == 12
+
+
+@pytest.mark.gpu
+def test_gpu_sweep_small_systems_exact():
+    e, _, _ = configs.run_config("heisenberg-chain-8", log=lambda s: None)
+    assert abs(e - E_HEIS_L8) < 1e-10
+    e, _, _ = configs.run_config("hubbard-chain-8-U0", log=lambda s: None)
+    assert abs(e - E_HUB_U0_L8) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_sweep_heisenberg_chain32_reference_pin():
+    # The reference's own pin of this model: input3100, -13.9973 at 2e-5 (dmrg/CMakeLists.txt:343).
+    e, recs, _ = configs.run_config("heisenberg-chain-32", sweeps=1, log=lambda s: None)
+    assert abs(e - (-13.9973)) < 2e-5
+    e32, _, _ = configs.run_config("heisenberg-chain-32", precision="fp32", sweeps=1,
+                                   log=lambda s: None)
+    assert abs(e32 - e) < 1e-10 * abs(e)
