@@ -42,10 +42,20 @@ class DmkInfo(ctypes.Structure):
         ("flops_executed", ctypes.c_double), ("flops_stage1", ctypes.c_double),
         ("flops_stage2", ctypes.c_double), ("bytes_compulsory", ctypes.c_double),
         ("bytes_bx", ctypes.c_double), ("precision", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("split_tasks", ctypes.c_int64), ("ws_elems", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("stage1_tile_m", ctypes.c_int32), ("stage1_tile_n", ctypes.c_int32),
+                ("stage2_tile_m", ctypes.c_int32), ("stage2_tile_n", ctypes.c_int32),
+                ("splitk", ctypes.c_int32), ("xcd_order", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class LanczosParams(ctypes.Structure):
@@ -81,6 +91,9 @@ SIGNATURES = {
     "dmk_apply_host_d": (_I, [_VP, _VP, _VP]),
     "dmk_apply_multi_d": (_I, [_VP, _I, _VP, _SZ, _VP, _SZ, _I]),
     "dmk_set_precision": (_I, [_VP, _I]),
+    "dmk_get_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
+    "dmk_set_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
+    "dmk_autotune_d": (_I, [_VP, _VP, _VP, _I, ctypes.POINTER(Tuning)]),
     "dmk_get_info": (_I, [_VP, ctypes.POINTER(DmkInfo)]),
     "dmk_sync": (_I, [_VP]),
     "dmk_profile_apply_d": (_I, [_VP, _VP, _VP, _I, ctypes.POINTER(_D), ctypes.POINTER(_D),
@@ -230,6 +243,24 @@ class BatchedGemm:
 
     def sync(self):
         _check(lib().dmk_sync(self.handle), "dmk_sync")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().dmk_get_tuning(self.handle, ctypes.byref(t)), "dmk_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        _check(lib().dmk_get_tuning(self.handle, ctypes.byref(t)), "dmk_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().dmk_set_tuning(self.handle, ctypes.byref(t)), "dmk_set_tuning")
+
+    def autotune(self, vin: DeviceArray, vout: DeviceArray, iters: int = 3) -> dict:
+        t = Tuning()
+        _check(lib().dmk_autotune_d(self.handle, vin.ptr, vout.ptr, iters, ctypes.byref(t)),
+               "dmk_autotune_d")
+        return t.as_dict()
 
     def info(self) -> dict:
         inf = DmkInfo()

@@ -169,6 +169,60 @@ int dmk_get_info(const dmk_handle* handle, dmk_info* info)
 		info->bytes_compulsory  = s.bytes_compulsory;
 		info->bytes_bx          = s.bytes_bx;
 		info->precision         = handle->plan->precision();
+		info->split_tasks       = s.split_tasks;
+		info->ws_elems          = s.ws_elems;
+		return DMK_OK;
+	});
+}
+
+namespace {
+void to_c(const dmk::Tuning& t, dmk_tuning* c)
+{
+	c->stage1_tile_m = t.bm1;
+	c->stage1_tile_n = t.bn1;
+	c->stage2_tile_m = t.bm2;
+	c->stage2_tile_n = t.bn2;
+	c->splitk        = t.splitk;
+	c->xcd_order     = t.xcd;
+}
+} // namespace
+
+int dmk_get_tuning(const dmk_handle* handle, dmk_tuning* tuning)
+{
+	return guarded([&] {
+		if (!handle || !handle->plan || !tuning)
+			throw std::invalid_argument("dmk_get_tuning: null argument");
+		to_c(handle->plan->tuning(), tuning);
+		return DMK_OK;
+	});
+}
+
+int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning)
+{
+	return guarded([&] {
+		if (!tuning)
+			throw std::invalid_argument("dmk_set_tuning: null tuning");
+		dmk::Tuning t;
+		t.bm1    = tuning->stage1_tile_m;
+		t.bn1    = tuning->stage1_tile_n;
+		t.bm2    = tuning->stage2_tile_m;
+		t.bn2    = tuning->stage2_tile_n;
+		t.splitk = tuning->splitk;
+		t.xcd    = tuning->xcd_order;
+		plan_of(handle).retune(t);
+		return DMK_OK;
+	});
+}
+
+int dmk_autotune_d(dmk_handle* handle, const double* vin, double* vout, int iters,
+                   dmk_tuning* chosen)
+{
+	return guarded([&] {
+		if (!vin || !vout)
+			throw std::invalid_argument("dmk_autotune_d: null vector");
+		dmk::Tuning t = plan_of(handle).autotune(vin, vout, iters);
+		if (chosen)
+			to_c(t, chosen);
 		return DMK_OK;
 	});
 }

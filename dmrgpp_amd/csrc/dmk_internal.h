@@ -45,8 +45,26 @@ struct Task {
 	int32_t rsplit;
 	int32_t seg_begin;
 	int32_t seg_end;
-	int32_t out_is_y;  // 1: C lives in the caller's output vector, 0: in the BX scratch
+	int32_t out_is_y;  // 1: C lives in the caller's output vector (type TO), 0: in the
+	                   // workspace (BX or split-k partials, compute type)
 	int32_t pad;
+};
+
+// One fixed-order reduction of split-k partials into the output: for e < count,
+//   out[yoff + e] (+)= sum_{s < nsplit} ws[wsoff + s * count + e].
+struct Reduce {
+	int64_t yoff;
+	int64_t wsoff;
+	int64_t count;
+	int32_t nsplit;
+	int32_t pad;
+};
+
+// One workgroup of the reduction launch: elements [start, start + len) of reduce entry r.
+struct ReduceChunk {
+	int32_t r;
+	int32_t len;
+	int64_t start;
 };
 
 // One output tile of a task; the tile list is ordered by decreasing task cost.

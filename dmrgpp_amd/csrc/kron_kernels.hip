@@ -10,8 +10,8 @@
 // This is the two-stage evaluation of GPUPlugin/apply_Htarget_sparse_not_ready.cpp:199-449 with
 // the identity factors of H_L (x) I and I (x) H_R contracted directly instead of as dense GEMMs.
 //
-// Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a 64x64 output tile and each
-// wave a 32x32 sub-tile held as 2x2 MFMA 16x16 accumulators. K is walked in chunks of 16 staged
+// Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a BM x BN output tile (64 or
+// 128 each way) and each wave a (BM/2) x (BN/2) sub-tile held as MFMA 16x16 accumulators. K is walked in chunks of 16 staged
 // through LDS, double buffered: the global loads of chunk c+1 are issued before the MFMAs of
 // chunk c. Every output tile is owned by exactly one workgroup and its contraction order is
 // fixed, so results are bitwise reproducible (no atomics, no split-k).
@@ -19,13 +19,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 namespace dmk {
 
-constexpr int kBM      = 64;
-constexpr int kBN      = 64;
 constexpr int kBK      = 16;
-constexpr int kThreads = 256;
-constexpr int kPad     = 8; // LDS row padding (elements)
+constexpr int kThreads = 256; // 4 wave64s in a 2x2 grid
+constexpr int kPad     = 8;   // LDS row padding (elements)
 
 template <typename T> struct MfmaOp;
 
@@ -54,33 +54,37 @@ template <typename T> struct Bases {
 	const T* p[BASE_COUNT];
 };
 
-// Per-thread register staging for one K chunk: 4 elements of L and 4 of R.
-template <typename T> struct Stage {
-	T l[4];
-	T r[4];
+// Per-thread register staging for one K chunk of a BM x BN tile.
+template <typename T, int BM, int BN> struct Stage {
+	static constexpr int NL = BM * kBK / kThreads;
+	static constexpr int NR = BN * kBK / kThreads;
+	T                    l[NL];
+	T                    r[NR];
 };
 
-template <typename T>
-__device__ inline void load_chunk(Stage<T>& st, const T* __restrict__ L, const T* __restrict__ R,
-                                  const Seg& s, int m0, int n0, int M, int N, int k0, int tid)
+template <typename T, int BM, int BN>
+__device__ inline void load_chunk(Stage<T, BM, BN>& st, const T* __restrict__ L,
+                                  const T* __restrict__ R, const Seg& s, int m0, int n0, int M,
+                                  int N, int k0, int tid)
 {
-	// L tile 64 x 16: thread -> row tid % 64, columns tid / 64 + 4q (coalesced along rows).
+	using S = Stage<T, BM, BN>;
+	// L tile BM x 16: thread -> row tid % BM, columns tid / BM + (256 / BM) q (coalesced rows).
 	{
-		const int m  = m0 + (tid & 63);
-		const int kb = k0 + (tid >> 6);
+		const int m  = m0 + (tid % BM);
+		const int kb = k0 + (tid / BM);
 #pragma unroll
-		for (int q = 0; q < 4; ++q) {
-			const int k = kb + 4 * q;
+		for (int q = 0; q < S::NL; ++q) {
+			const int k = kb + (kThreads / BM) * q;
 			st.l[q]     = (m < M && k < s.k) ? L[int64_t(m) + int64_t(k) * s.ldl] : T(0);
 		}
 	}
 	if (s.rtrans) {
 		// R stored N x K: element (k, n) at R[n + k*ldr]; coalesced along n.
-		const int n  = n0 + (tid & 63);
-		const int kb = k0 + (tid >> 6);
+		const int n  = n0 + (tid % BN);
+		const int kb = k0 + (tid / BN);
 #pragma unroll
-		for (int q = 0; q < 4; ++q) {
-			const int k = kb + 4 * q;
+		for (int q = 0; q < S::NR; ++q) {
+			const int k = kb + (kThreads / BN) * q;
 			st.r[q]     = (n < N && k < s.k) ? R[int64_t(n) + int64_t(k) * s.ldr] : T(0);
 		}
 	} else {
@@ -88,28 +92,29 @@ __device__ inline void load_chunk(Stage<T>& st, const T* __restrict__ L, const T
 		const int k  = k0 + (tid & 15);
 		const int nb = n0 + (tid >> 4);
 #pragma unroll
-		for (int q = 0; q < 4; ++q) {
-			const int n = nb + 16 * q;
+		for (int q = 0; q < S::NR; ++q) {
+			const int n = nb + (kThreads / 16) * q;
 			st.r[q]     = (n < N && k < s.k) ? R[int64_t(k) + int64_t(n) * s.ldr] : T(0);
 		}
 	}
 }
 
-template <typename T>
-__device__ inline void store_chunk(const Stage<T>& st, T (*As)[kBM + kPad], T (*Bs)[kBN + kPad],
-                                   int rtrans, int tid)
+template <typename T, int BM, int BN>
+__device__ inline void store_chunk(const Stage<T, BM, BN>& st, T (*As)[BM + kPad],
+                                   T (*Bs)[BN + kPad], int rtrans, int tid)
 {
+	using S = Stage<T, BM, BN>;
 #pragma unroll
-	for (int q = 0; q < 4; ++q)
-		As[(tid >> 6) + 4 * q][tid & 63] = st.l[q];
+	for (int q = 0; q < S::NL; ++q)
+		As[(tid / BM) + (kThreads / BM) * q][tid % BM] = st.l[q];
 	if (rtrans) {
 #pragma unroll
-		for (int q = 0; q < 4; ++q)
-			Bs[(tid >> 6) + 4 * q][tid & 63] = st.r[q];
+		for (int q = 0; q < S::NR; ++q)
+			Bs[(tid / BN) + (kThreads / BN) * q][tid % BN] = st.r[q];
 	} else {
 #pragma unroll
-		for (int q = 0; q < 4; ++q)
-			Bs[tid & 15][(tid >> 4) + 16 * q] = st.r[q];
+		for (int q = 0; q < S::NR; ++q)
+			Bs[tid & 15][(tid >> 4) + (kThreads / 16) * q] = st.r[q];
 	}
 }
 
@@ -125,45 +130,52 @@ __device__ inline bool next_chunk(const Seg* __restrict__ segs, int seg_end, int
 }
 
 // T: operand/accumulate type; TO: output element type (the stage-2 output of the FP32 mode is
-// written in FP64 for the FP64 Lanczos recurrence).
-template <typename T, typename TO>
+// written in FP64 for the FP64 Lanczos recurrence). BM x BN output tile per workgroup, each of
+// the 4 waves owning a (BM/2) x (BN/2) sub-tile of (BM/32) x (BN/32) MFMA 16x16 blocks.
+template <typename T, typename TO, int BM, int BN>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
     const Task* __restrict__ tasks, const Seg* __restrict__ segs, const Tile* __restrict__ tiles,
-    Bases<T> bases, int64_t xstride, int64_t bxstride, TO* __restrict__ out, int64_t ostride,
-    int beta)
+    Bases<T> bases, int64_t xstride, int64_t bxstride, TO* __restrict__ out_y, int64_t ystride,
+    T* __restrict__ out_ws, int64_t wsstride, int beta)
 {
 	using Op    = MfmaOp<T>;
 	using acc_t = typename Op::acc_t;
+	constexpr int WM = BM / 2, WN = BN / 2;
+	constexpr int FI = WM / 16, FJ = WN / 16;
+	static_assert(BM * kBK % kThreads == 0 && BN * kBK % kThreads == 0, "tile / thread mismatch");
 
-	__shared__ T As[2][kBK][kBM + kPad];
-	__shared__ T Bs[2][kBK][kBN + kPad];
+	__shared__ T As[2][kBK][BM + kPad];
+	__shared__ T Bs[2][kBK][BN + kPad];
 
-	const Tile tl  = tiles[blockIdx.x];
+	const Tile tl = tiles[blockIdx.x];
+	if (tl.task < 0)
+		return; // padding slot of the XCD-aware order
 	const Task tk  = tasks[tl.task];
 	const int  tid = threadIdx.x;
 	const int  vec = blockIdx.y;
 
 	// Per-vector operand bases (multi-vector launches put one right-hand side per grid row).
 	// Selected through pick() rather than a dynamically indexed array so they stay in SGPRs.
-	const T* b0 = bases.p[BASE_APACK];
-	const T* b1 = bases.p[BASE_BPACK];
-	const T* b2 = bases.p[BASE_X] + vec * xstride;
-	const T* b3 = bases.p[BASE_BX] + vec * bxstride;
+	const T* b0   = bases.p[BASE_APACK];
+	const T* b1   = bases.p[BASE_BPACK];
+	const T* b2   = bases.p[BASE_X] + vec * xstride;
+	const T* b3   = bases.p[BASE_BX] + vec * bxstride;
 	auto     pick = [&](int id) -> const T* {
                 return id == BASE_APACK ? b0 : id == BASE_BPACK ? b1 : id == BASE_X ? b2 : b3;
 	};
-	out += vec * ostride;
+	out_y += vec * ystride;
+	out_ws += vec * wsstride;
 
 	const int lane = tid & 63;
 	const int wave = tid >> 6;
-	const int wm   = (wave & 1) * 32;
-	const int wn   = (wave >> 1) * 32;
+	const int wm   = (wave & 1) * WM;
+	const int wn   = (wave >> 1) * WN;
 
-	acc_t acc[2][2];
+	acc_t acc[FI][FJ];
 #pragma unroll
-	for (int i = 0; i < 2; ++i)
+	for (int i = 0; i < FI; ++i)
 #pragma unroll
-		for (int j = 0; j < 2; ++j)
+		for (int j = 0; j < FJ; ++j)
 			acc[i][j] = acc_t { 0, 0, 0, 0 };
 
 	int seg = tk.seg_begin;
@@ -172,11 +184,11 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 		++seg;
 
 	if (seg < tk.seg_end) {
-		Stage<T> st;
-		Seg      s = segs[seg];
-		load_chunk<T>(st, pick(s.lbase) + s.loff, pick(s.rbase) + s.roff, s, tl.m0, tl.n0, tk.m,
-		              tk.n, k0, tid);
-		store_chunk<T>(st, As[0], Bs[0], s.rtrans, tid);
+		Stage<T, BM, BN> st;
+		Seg              s = segs[seg];
+		load_chunk<T, BM, BN>(st, pick(s.lbase) + s.loff, pick(s.rbase) + s.roff, s, tl.m0, tl.n0,
+		                      tk.m, tk.n, k0, tid);
+		store_chunk<T, BM, BN>(st, As[0], Bs[0], s.rtrans, tid);
 		__syncthreads();
 		int buf = 0;
 		for (;;) {
@@ -186,28 +198,28 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 			Seg  ns;
 			if (hasNext) {
 				ns = segs[nseg];
-				load_chunk<T>(st, pick(ns.lbase) + ns.loff, pick(ns.rbase) + ns.roff, ns, tl.m0,
-				              tl.n0, tk.m, tk.n, nk0, tid);
+				load_chunk<T, BM, BN>(st, pick(ns.lbase) + ns.loff, pick(ns.rbase) + ns.roff, ns,
+				                      tl.m0, tl.n0, tk.m, tk.n, nk0, tid);
 			}
 #pragma unroll
 			for (int kk = 0; kk < kBK; kk += 4) {
 				const int kr = kk + (lane >> 4);
-				T         a[2], b[2];
+				T         a[FI], b[FJ];
 #pragma unroll
-				for (int i = 0; i < 2; ++i)
+				for (int i = 0; i < FI; ++i)
 					a[i] = As[buf][kr][wm + 16 * i + (lane & 15)];
 #pragma unroll
-				for (int j = 0; j < 2; ++j)
+				for (int j = 0; j < FJ; ++j)
 					b[j] = Bs[buf][kr][wn + 16 * j + (lane & 15)];
 #pragma unroll
-				for (int i = 0; i < 2; ++i)
+				for (int i = 0; i < FI; ++i)
 #pragma unroll
-					for (int j = 0; j < 2; ++j)
+					for (int j = 0; j < FJ; ++j)
 						acc[i][j] = Op::mma(a[i], b[j], acc[i][j]);
 			}
 			if (!hasNext)
 				break;
-			store_chunk<T>(st, As[buf ^ 1], Bs[buf ^ 1], ns.rtrans, tid);
+			store_chunk<T, BM, BN>(st, As[buf ^ 1], Bs[buf ^ 1], ns.rtrans, tid);
 			__syncthreads();
 			buf ^= 1;
 			seg = nseg;
@@ -216,11 +228,13 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 	}
 
 	// Epilogue: register r of lane l holds row Op::row(l, r), column l%16 of each 16x16 block.
-	TO* C = out + tk.coff;
+	// Tasks writing the caller's vector convert to TO (and accumulate when beta); workspace
+	// tasks (stage-1 panels, split-k partials) store the compute type and always overwrite.
+	const bool to_y = tk.out_is_y != 0;
 #pragma unroll
-	for (int i = 0; i < 2; ++i)
+	for (int i = 0; i < FI; ++i)
 #pragma unroll
-		for (int j = 0; j < 2; ++j)
+		for (int j = 0; j < FJ; ++j)
 #pragma unroll
 			for (int r = 0; r < 4; ++r) {
 				const int row = tl.m0 + wm + 16 * i + Op::row(lane, r);
@@ -228,13 +242,39 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 				if (row < tk.m && col < tk.n) {
 					const int     rr   = row % tk.rsplit;
 					const int     blk  = row / tk.rsplit;
-					const int64_t addr = int64_t(rr) + (int64_t(blk) * tk.n + col) * tk.ldc;
-					TO            v    = TO(acc[i][j][r]);
-					if (beta)
-						v += C[addr];
-					C[addr] = v;
+					const int64_t addr = tk.coff + int64_t(rr) + (int64_t(blk) * tk.n + col) * tk.ldc;
+					if (to_y) {
+						TO v = TO(acc[i][j][r]);
+						if (beta)
+							v += out_y[addr];
+						out_y[addr] = v;
+					} else {
+						out_ws[addr] = acc[i][j][r];
+					}
 				}
 			}
+}
+
+// Sums split-k partials in a fixed order (s = 0, 1, ...) into the output, accumulating in TO.
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const Reduce* __restrict__ red,
+                                                            const ReduceChunk* __restrict__ chunks,
+                                                            const T* __restrict__ ws,
+                                                            int64_t wsstride, TO* __restrict__ out,
+                                                            int64_t ystride, int beta)
+{
+	const ReduceChunk ch = chunks[blockIdx.x];
+	const Reduce      rd = red[ch.r];
+	ws += int64_t(blockIdx.y) * wsstride + rd.wsoff;
+	out += int64_t(blockIdx.y) * ystride + rd.yoff;
+	for (int64_t e = ch.start + threadIdx.x; e < ch.start + ch.len; e += blockDim.x) {
+		TO v = 0;
+		for (int s = 0; s < rd.nsplit; ++s)
+			v += TO(ws[int64_t(s) * rd.count + e]);
+		if (beta)
+			v += out[e];
+		out[e] = v;
+	}
 }
 
 __global__ void cast_f64_to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, int64_t n)
@@ -248,9 +288,10 @@ __global__ void cast_f64_to_f32_kernel(const double* __restrict__ x, float* __re
 // Host-side launchers ------------------------------------------------------------------------
 
 template <typename T, typename TO>
-void launch_grouped_gemm(const Task* tasks, const Seg* segs, const Tile* tiles, int ntiles,
-                         const T* const* bases, int64_t xstride, int64_t bxstride, TO* out,
-                         int64_t ostride, int nvec, int beta, hipStream_t stream)
+void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
+                         int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
+                         TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
+                         int beta, hipStream_t stream)
 {
 	if (ntiles <= 0 || nvec <= 0)
 		return;
@@ -258,19 +299,45 @@ void launch_grouped_gemm(const Task* tasks, const Seg* segs, const Tile* tiles, 
 	for (int i = 0; i < BASE_COUNT; ++i)
 		b.p[i] = bases[i];
 	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
-	hipLaunchKernelGGL((grouped_gemm_kernel<T, TO>), grid, dim3(kThreads), 0, stream, tasks, segs,
-	                   tiles, b, xstride, bxstride, out, ostride, beta);
+#define DMK_LAUNCH(BM, BN)                                                                          \
+	hipLaunchKernelGGL((grouped_gemm_kernel<T, TO, BM, BN>), grid, dim3(kThreads), 0, stream,  \
+	                   tasks, segs, tiles, b, xstride, bxstride, out_y, ystride, out_ws,          \
+	                   wsstride, beta)
+	if (bm == 64 && bn == 64)
+		DMK_LAUNCH(64, 64);
+	else if (bm == 128 && bn == 64)
+		DMK_LAUNCH(128, 64);
+	else if (bm == 64 && bn == 128)
+		DMK_LAUNCH(64, 128);
+	else if (bm == 128 && bn == 128)
+		DMK_LAUNCH(128, 128);
+	else
+		throw std::runtime_error("grouped gemm: unsupported tile shape");
+#undef DMK_LAUNCH
 }
 
-template void launch_grouped_gemm<double, double>(const Task*, const Seg*, const Tile*, int,
-                                                  const double* const*, int64_t, int64_t,
-                                                  double*, int64_t, int, int, hipStream_t);
-template void launch_grouped_gemm<float, float>(const Task*, const Seg*, const Tile*, int,
-                                                const float* const*, int64_t, int64_t, float*,
-                                                int64_t, int, int, hipStream_t);
-template void launch_grouped_gemm<float, double>(const Task*, const Seg*, const Tile*, int,
-                                                 const float* const*, int64_t, int64_t, double*,
-                                                 int64_t, int, int, hipStream_t);
+template <typename T, typename TO>
+void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchunks, const T* ws,
+                          int64_t wsstride, TO* out, int64_t ystride, int nvec, int beta,
+                          hipStream_t stream)
+{
+	if (nchunks <= 0 || nvec <= 0)
+		return;
+	const dim3 grid { unsigned(nchunks), unsigned(nvec), 1u };
+	hipLaunchKernelGGL((splitk_reduce_kernel<T, TO>), grid, dim3(256), 0, stream, red, chunks, ws,
+	                   wsstride, out, ystride, beta);
+}
+
+#define DMK_INST(T, TO)                                                                            \
+	template void launch_grouped_gemm<T, TO>(int, int, const Task*, const Seg*, const Tile*, int, \
+	                                         const T* const*, int64_t, int64_t, TO*, int64_t, T*,  \
+	                                         int64_t, int, int, hipStream_t);                     \
+	template void launch_splitk_reduce<T, TO>(const Reduce*, const ReduceChunk*, int, const T*,   \
+	                                          int64_t, TO*, int64_t, int, int, hipStream_t);
+DMK_INST(double, double)
+DMK_INST(float, float)
+DMK_INST(float, double)
+#undef DMK_INST
 
 void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream)
 {

@@ -19,7 +19,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -27,15 +30,19 @@
 namespace dmk {
 
 template <typename T, typename TO>
-void launch_grouped_gemm(const Task* tasks, const Seg* segs, const Tile* tiles, int ntiles,
-                         const T* const* bases, int64_t xstride, int64_t bxstride, TO* out,
-                         int64_t ostride, int nvec, int beta, hipStream_t stream);
+void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
+                         int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
+                         TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
+                         int beta, hipStream_t stream);
+template <typename T, typename TO>
+void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchunks, const T* ws,
+                          int64_t wsstride, TO* out, int64_t ystride, int nvec, int beta,
+                          hipStream_t stream);
 void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream);
 
 namespace {
 
-constexpr int kTileM = 64;
-constexpr int kTileN = 64;
+constexpr int kXcds = 8; // MI355X: 8 accelerator complex dies, 32 CUs each
 
 int64_t count_nnz(const double* p, int64_t rows, int64_t cols, int64_t ld)
 {
@@ -74,10 +81,11 @@ template <typename X> X* dev_alloc_copy(const std::vector<X>& h)
 }
 
 // Orders tasks by decreasing cost and lists their 64x64 tiles in that order (a flop-sorted
-// work list: the dispatcher hands out workgroups in blockIdx order, so the largest tiles
-// start first and the small ones fill the tail).
-void build_tiles(StageTables& st, const std::vector<double>& cost)
+// work list: the dispatcher hands out workgroups in blockIdx order, so the largest tiles start
+// first and the small ones fill the tail).
+void build_tiles_plain(StageTables& st, const std::vector<double>& cost)
 {
+	const int kTileM = st.bm, kTileN = st.bn;
 	std::vector<int> order(st.tasks.size());
 	std::iota(order.begin(), order.end(), 0);
 	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
@@ -90,7 +98,107 @@ void build_tiles(StageTables& st, const std::vector<double>& cost)
 	}
 }
 
+// XCD-aware order. Workgroup b runs on XCD b % kXcds, and the XCDs do not share L2. A task with
+// at least kXcds tiles is cut into kXcds rectangular regions of its tile grid, one per XCD, so
+// the tiles that read the same row panel (and column panel) of the operands share one L2;
+// smaller tasks go whole to the least loaded XCD. Each XCD's list stays in decreasing task
+// cost; slot i of XCD x becomes blockIdx kXcds * i + x, and holes are empty tiles (task -1).
+void build_tiles_xcd(StageTables& st, const std::vector<double>& cost)
+{
+	const int kTileM = st.bm, kTileN = st.bn;
+	std::vector<int> order(st.tasks.size());
+	std::iota(order.begin(), order.end(), 0);
+	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+	std::vector<std::vector<Tile>> lists(kXcds);
+	std::vector<double>            load(kXcds, 0.0);
+	for (int t : order) {
+		const Task& tk = st.tasks[t];
+		const int   tr = std::max(1, (tk.m + kTileM - 1) / kTileM);
+		const int   tc = std::max(1, (tk.n + kTileN - 1) / kTileN);
+		const double per_tile = cost[t] / double(tr * tc);
+		if (tr * tc >= kXcds) {
+			// Region grid pr x pc = kXcds with regions as square as the tile grid allows.
+			int    best_pr = 1;
+			double best    = 1e300;
+			for (int pr = 1; pr <= kXcds; pr *= 2) {
+				const int pc = kXcds / pr;
+				if (pr > tr || pc > tc)
+					continue;
+				const double rh = double(tr) / pr, rw = double(tc) / pc;
+				const double score = std::max(rh, rw) / std::min(rh, rw);
+				if (score < best) {
+					best    = score;
+					best_pr = pr;
+				}
+			}
+			int pr = best_pr, pc = kXcds / best_pr;
+			if (pr > tr || pc > tc) {
+				// Degenerate grid (e.g. 1 x 9): cut the longer side only.
+				pr = (tr >= tc) ? std::min(tr, kXcds) : 1;
+				pc = (tr >= tc) ? 1 : std::min(tc, kXcds);
+			}
+			// XCDs ordered by current load so uneven regions even out across tasks.
+			std::vector<int> xs(kXcds);
+			std::iota(xs.begin(), xs.end(), 0);
+			std::stable_sort(xs.begin(), xs.end(), [&](int a, int b) { return load[a] < load[b]; });
+			int r = 0;
+			for (int ri = 0; ri < pr; ++ri) {
+				const int r0 = (tr * ri) / pr, r1 = (tr * (ri + 1)) / pr;
+				for (int ci = 0; ci < pc; ++ci, ++r) {
+					const int c0 = (tc * ci) / pc, c1 = (tc * (ci + 1)) / pc;
+					const int x  = xs[r % kXcds];
+					for (int i = r0; i < r1; ++i)
+						for (int j = c0; j < c1; ++j)
+							lists[x].push_back(Tile { t, i * kTileM, j * kTileN, 0 });
+					load[x] += per_tile * double((r1 - r0) * (c1 - c0));
+				}
+			}
+		} else {
+			const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+			for (int i = 0; i < tr; ++i)
+				for (int j = 0; j < tc; ++j)
+					lists[x].push_back(Tile { t, i * kTileM, j * kTileN, 0 });
+			load[x] += cost[t];
+		}
+	}
+	size_t longest = 0;
+	for (const auto& l : lists)
+		longest = std::max(longest, l.size());
+	st.tiles.assign(longest * kXcds, Tile { -1, 0, 0, 0 });
+	for (int x = 0; x < kXcds; ++x)
+		for (size_t i = 0; i < lists[x].size(); ++i)
+			st.tiles[i * kXcds + size_t(x)] = lists[x][i];
+}
+
+void build_tiles(StageTables& st, const std::vector<double>& cost, int xcd)
+{
+	if (xcd)
+		build_tiles_xcd(st, cost);
+	else
+		build_tiles_plain(st, cost);
+}
+
+// Autotuner results of this process, keyed by the block-size histogram signature.
+std::mutex                              g_tune_mutex;
+std::unordered_map<std::string, Tuning> g_tune_cache;
+
 } // namespace
+
+bool cached_tuning(const std::string& key, Tuning& t)
+{
+	std::lock_guard<std::mutex> lock(g_tune_mutex);
+	auto                        it = g_tune_cache.find(key);
+	if (it == g_tune_cache.end())
+		return false;
+	t = it->second;
+	return true;
+}
+
+void store_tuning(const std::string& key, const Tuning& t)
+{
+	std::lock_guard<std::mutex> lock(g_tune_mutex);
+	g_tune_cache[key] = t;
+}
 
 void StageTables::upload()
 {
@@ -203,11 +311,12 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	}
 	stats_.bx_elems = bx_off[np];
 
-	st1_.tasks.clear();
-	st1_.segs.clear();
-	st2_.tasks.clear();
-	st2_.segs.clear();
-	std::vector<double> cost1, cost2;
+	t1_tasks_.clear();
+	t1_segs_.clear();
+	t2_tasks_.clear();
+	t2_segs_.clear();
+	cost1_.clear();
+	cost2_.clear();
 
 	for (int64_t I = 0; I < np; ++I) {
 		// A panel for the stage-2 GEMM of patch I.
@@ -244,7 +353,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			tk.n         = int32_t(L[J]);
 			tk.ldc       = int32_t(R[I]);
 			tk.rsplit    = int32_t(R[I]);
-			tk.seg_begin = int32_t(st1_.segs.size());
+			tk.seg_begin = int32_t(t1_segs_.size());
 			Seg s {};
 			s.lbase = BASE_BPACK;
 			s.loff  = boff;
@@ -254,12 +363,12 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			s.ldr   = int32_t(R[J]);
 			s.k     = int32_t(R[J]);
 			s.rtrans = 0;
-			st1_.segs.push_back(s);
-			tk.seg_end  = int32_t(st1_.segs.size());
+			t1_segs_.push_back(s);
+			tk.seg_end  = int32_t(t1_segs_.size());
 			tk.out_is_y = 0;
-			st1_.tasks.push_back(tk);
+			t1_tasks_.push_back(tk);
 			const double f = 2.0 * double(q * R[I]) * double(R[J]) * double(L[J]);
-			cost1.push_back(f);
+			cost1_.push_back(f);
 			stats_.flops_executed += f;
 			stats_.flops_stage1 += f;
 			col += q * L[J];
@@ -272,7 +381,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 		tk.n         = int32_t(L[I]);
 		tk.ldc       = int32_t(R[I]);
 		tk.rsplit    = int32_t(R[I]);
-		tk.seg_begin = int32_t(st2_.segs.size());
+		tk.seg_begin = int32_t(t2_segs_.size());
 		tk.out_is_y  = 1;
 		double cost  = 0;
 		if (K[I] > 0) {
@@ -285,7 +394,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			s.ldr    = int32_t(L[I]);
 			s.k      = int32_t(K[I]);
 			s.rtrans = 1;
-			st2_.segs.push_back(s);
+			t2_segs_.push_back(s);
 			cost += 2.0 * double(R[I]) * double(L[I]) * double(K[I]);
 			stats_.bytes_bx += 2.0 * 8.0 * double(R[I]) * double(K[I]);
 		}
@@ -309,7 +418,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 					s.ldr    = int32_t(L[I]);
 					s.k      = int32_t(L[J]);
 					s.rtrans = 1;
-					st2_.segs.push_back(s);
+					t2_segs_.push_back(s);
 					cost += 2.0 * double(R[I]) * double(L[I]) * double(L[J]);
 				} else if (kind[id] == IDENT_A) {
 					// Y_I += B * X_J: L = B (R_I x R_J), R = X_J (R_J x L_J).
@@ -328,23 +437,19 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 					s.ldr    = int32_t(R[J]);
 					s.k      = int32_t(R[J]);
 					s.rtrans = 0;
-					st2_.segs.push_back(s);
+					t2_segs_.push_back(s);
 					cost += 2.0 * double(R[I]) * double(R[J]) * double(L[I]);
 				}
 			}
-		tk.seg_end = int32_t(st2_.segs.size());
-		st2_.tasks.push_back(tk);
-		cost2.push_back(cost + 1.0); // empty tasks still write zeros
+		tk.seg_end = int32_t(t2_segs_.size());
+		t2_tasks_.push_back(tk);
+		cost2_.push_back(cost + 1.0); // empty tasks still write zeros
 		stats_.flops_executed += cost;
 		stats_.flops_stage2 += cost;
 	}
 
-	build_tiles(st1_, cost1);
-	build_tiles(st2_, cost2);
-	stats_.stage1_tasks = int64_t(st1_.tasks.size());
-	stats_.stage2_tasks = int64_t(st2_.tasks.size());
-	stats_.stage1_tiles = int64_t(st1_.tiles.size());
-	stats_.stage2_tiles = int64_t(st2_.tiles.size());
+	stats_.stage1_tasks = int64_t(t1_tasks_.size());
+	stats_.stage2_tasks = int64_t(t2_tasks_.size());
 	stats_.apack_elems  = int64_t(apack.size());
 	stats_.bpack_elems  = int64_t(bpack.size());
 	if (stats_.n > (int64_t(1) << 31) * 64)
@@ -355,9 +460,10 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 		DMK_HIP_CHECK(hipEventCreate(&e));
 	d_apack_ = dev_alloc_copy(apack);
 	d_bpack_ = dev_alloc_copy(bpack);
-	st1_.upload();
-	st2_.upload();
-	ensure_scratch(1);
+	// A tuning the autotuner chose for this histogram earlier in the process, else the default.
+	Tuning t;
+	cached_tuning(histogram_key(), t);
+	retune(t);
 }
 
 Plan::~Plan()
@@ -369,7 +475,10 @@ Plan::~Plan()
 	(void)hipFree(d_apack32_);
 	(void)hipFree(d_bpack32_);
 	(void)hipFree(d_bx_);
+	(void)hipFree(d_ws_);
 	(void)hipFree(d_x32_);
+	(void)hipFree(d_reduce_);
+	(void)hipFree(d_chunks_);
 	st1_.release();
 	st2_.release();
 	for (auto& e : ev_)
@@ -377,6 +486,145 @@ Plan::~Plan()
 			(void)hipEventDestroy(e);
 	if (stream_)
 		(void)hipStreamDestroy(stream_);
+}
+
+std::string Plan::histogram_key() const
+{
+	// Stage-2 task shapes (R_I, L_I, contraction length) in log2 bins, plus the stage-1 shape
+	// histogram: plans with the same key have the same launch geometry up to rounding.
+	auto bin = [](int64_t v) {
+		int b = 0;
+		while (v > 1) {
+			v >>= 1;
+			++b;
+		}
+		return b;
+	};
+	std::vector<std::string> items;
+	for (size_t t = 0; t < t2_tasks_.size(); ++t) {
+		int64_t k = 0;
+		for (int s = t2_tasks_[t].seg_begin; s < t2_tasks_[t].seg_end; ++s)
+			k += t2_segs_[size_t(s)].k;
+		items.push_back(std::to_string(bin(t2_tasks_[t].m)) + "." + std::to_string(bin(t2_tasks_[t].n))
+		                + "." + std::to_string(bin(k)));
+	}
+	for (const Task& tk : t1_tasks_)
+		items.push_back("s" + std::to_string(bin(tk.m)) + "." + std::to_string(bin(tk.n)));
+	std::sort(items.begin(), items.end());
+	std::string key;
+	for (const auto& it : items)
+		key += it + ";";
+	return key;
+}
+
+void Plan::retune(const Tuning& t)
+{
+	auto valid = [](int bm, int bn) {
+		return (bm == 64 || bm == 128) && (bn == 64 || bn == 128);
+	};
+	if (!valid(t.bm1, t.bn1) || !valid(t.bm2, t.bn2) || t.splitk < 0)
+		throw std::invalid_argument("dmk tuning: tile shapes must be 64 or 128, splitk >= 0");
+	if (stream_)
+		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+	tuning_ = t;
+
+	// Stage 1 as planned.
+	st1_.release();
+	st1_.bm    = t.bm1;
+	st1_.bn    = t.bn1;
+	st1_.tasks = t1_tasks_;
+	st1_.segs  = t1_segs_;
+	build_tiles(st1_, cost1_, t.xcd);
+
+	// Stage 2: tasks with a long contraction are cut into nsplit pieces along K; each piece
+	// writes a partial of the whole output patch to the workspace, and a reduction sums the
+	// pieces in order. Piece boundaries are multiples of the 16-deep K chunk.
+	st2_.release();
+	st2_.bm = t.bm2;
+	st2_.bn = t.bn2;
+	st2_.tasks.clear();
+	st2_.segs.clear();
+	reduce_.clear();
+	std::vector<double> cost;
+	int64_t             ws = 0;
+	stats_.split_tasks     = 0;
+	for (size_t ti = 0; ti < t2_tasks_.size(); ++ti) {
+		const Task& tk   = t2_tasks_[ti];
+		int64_t     ktot = 0;
+		for (int s = tk.seg_begin; s < tk.seg_end; ++s)
+			ktot += t2_segs_[size_t(s)].k;
+		int nsplit = 1;
+		if (t.splitk > 0 && ktot >= 2 * int64_t(t.splitk))
+			nsplit = int(std::min<int64_t>(16, ktot / t.splitk));
+		if (nsplit <= 1) {
+			Task c      = tk;
+			c.seg_begin = int32_t(st2_.segs.size());
+			for (int s = tk.seg_begin; s < tk.seg_end; ++s)
+				st2_.segs.push_back(t2_segs_[size_t(s)]);
+			c.seg_end = int32_t(st2_.segs.size());
+			st2_.tasks.push_back(c);
+			cost.push_back(cost2_[ti]);
+			continue;
+		}
+		stats_.split_tasks++;
+		const int64_t count = int64_t(tk.m) * int64_t(tk.n);
+		Reduce        rd {};
+		rd.yoff   = tk.coff;
+		rd.wsoff  = ws;
+		rd.count  = count;
+		rd.nsplit = nsplit;
+		reduce_.push_back(rd);
+		// Cut the concatenated K range [0, ktot) at multiples of 16.
+		std::vector<int64_t> cuts(size_t(nsplit) + 1, 0);
+		for (int p = 1; p < nsplit; ++p)
+			cuts[size_t(p)] = ((ktot * p / nsplit) / 16) * 16;
+		cuts[size_t(nsplit)] = ktot;
+		for (int p = 0; p < nsplit; ++p) {
+			Task c      = tk;
+			c.coff      = ws + int64_t(p) * count;
+			c.ldc       = tk.m;
+			c.rsplit    = tk.m;
+			c.out_is_y  = 0;
+			c.seg_begin = int32_t(st2_.segs.size());
+			int64_t base = 0; // K offset of the current segment in the concatenation
+			for (int s = tk.seg_begin; s < tk.seg_end; ++s) {
+				const Seg& sg = t2_segs_[size_t(s)];
+				const int64_t a = std::max(cuts[size_t(p)], base);
+				const int64_t b = std::min(cuts[size_t(p) + 1], base + sg.k);
+				if (b > a) {
+					Seg piece = sg;
+					const int64_t ka = a - base;
+					piece.k          = int32_t(b - a);
+					piece.loff += ka * sg.ldl;                       // L is M x K: column ka
+					piece.roff += sg.rtrans ? ka * sg.ldr : ka;      // R is N x K or K x N
+					st2_.segs.push_back(piece);
+				}
+				base += sg.k;
+			}
+			c.seg_end = int32_t(st2_.segs.size());
+			st2_.tasks.push_back(c);
+			cost.push_back(cost2_[ti] * double(cuts[size_t(p) + 1] - cuts[size_t(p)]) / double(ktot));
+		}
+		ws += int64_t(nsplit) * count;
+	}
+	stats_.ws_elems = ws;
+	build_tiles(st2_, cost, t.xcd);
+	chunks_.clear();
+	for (size_t r = 0; r < reduce_.size(); ++r)
+		for (int64_t s0 = 0; s0 < reduce_[r].count; s0 += 4096)
+			chunks_.push_back(ReduceChunk { int32_t(r), int32_t(std::min<int64_t>(4096, reduce_[r].count - s0)), s0 });
+	(void)hipFree(d_reduce_);
+	(void)hipFree(d_chunks_);
+	d_reduce_ = dev_alloc_copy(reduce_);
+	d_chunks_ = dev_alloc_copy(chunks_);
+	st1_.upload();
+	st2_.upload();
+	stats_.stage1_tiles = int64_t(st1_.tiles.size());
+	stats_.stage2_tiles = int64_t(st2_.tiles.size());
+	ws_bytes_           = 0; // force the workspace re-check
+	(void)hipFree(d_ws_);
+	d_ws_ = nullptr;
+	ensure_scratch(1);
 }
 
 void Plan::ensure_scratch(int nvec)
@@ -390,6 +638,15 @@ void Plan::ensure_scratch(int nvec)
 		bx_bytes_ = 0;
 		DMK_HIP_CHECK(hipMalloc(&d_bx_, bytes));
 		bx_bytes_ = bytes;
+	}
+	const size_t wbytes = size_t(std::max<int64_t>(stats_.ws_elems, 1)) * elem * size_t(nvec);
+	if (wbytes > ws_bytes_) {
+		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+		(void)hipFree(d_ws_);
+		d_ws_     = nullptr;
+		ws_bytes_ = 0;
+		DMK_HIP_CHECK(hipMalloc(&d_ws_, wbytes));
+		ws_bytes_ = wbytes;
 	}
 	if (precision_ != PREC_FP64 && nvec > x32_cap_) {
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -428,6 +685,47 @@ void Plan::set_precision(int p)
 	ensure_scratch(1);
 }
 
+// Stage 1, stage 2 and the split-k reduction on the plan's stream; an event is recorded
+// between the stages when mid is given.
+void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec,
+                  int beta, hipEvent_t mid)
+{
+	const int64_t bxs = stats_.bx_elems;
+	const int64_t wss = stats_.ws_elems;
+	if (precision_ == PREC_FP64) {
+		double*       bx                = static_cast<double*>(d_bx_);
+		double*       ws                = static_cast<double*>(d_ws_);
+		const double* bases[BASE_COUNT] = { d_apack_, d_bpack_, vin, bx };
+		launch_grouped_gemm<double, double>(st1_.bm, st1_.bn, st1_.d_tasks, st1_.d_segs,
+		                                    st1_.d_tiles, int(st1_.tiles.size()), bases, ldin, bxs,
+		                                    vout, ldout, bx, bxs, nvec, 0, stream_);
+		if (mid)
+			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
+		launch_grouped_gemm<double, double>(st2_.bm, st2_.bn, st2_.d_tasks, st2_.d_segs,
+		                                    st2_.d_tiles, int(st2_.tiles.size()), bases, ldin, bxs,
+		                                    vout, ldout, ws, wss, nvec, beta, stream_);
+		launch_splitk_reduce<double, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
+		                                     vout, ldout, nvec, beta, stream_);
+	} else {
+		for (int v = 0; v < nvec; ++v)
+			launch_cast_f64_to_f32(vin + v * ldin, d_x32_ + v * stats_.n, stats_.n, stream_);
+		float*       bx                = static_cast<float*>(d_bx_);
+		float*       ws                = static_cast<float*>(d_ws_);
+		const float* bases[BASE_COUNT] = { d_apack32_, d_bpack32_, d_x32_, bx };
+		launch_grouped_gemm<float, float>(st1_.bm, st1_.bn, st1_.d_tasks, st1_.d_segs,
+		                                  st1_.d_tiles, int(st1_.tiles.size()), bases, stats_.n,
+		                                  bxs, bx, bxs, bx, bxs, nvec, 0, stream_);
+		if (mid)
+			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
+		launch_grouped_gemm<float, double>(st2_.bm, st2_.bn, st2_.d_tasks, st2_.d_segs,
+		                                   st2_.d_tiles, int(st2_.tiles.size()), bases, stats_.n,
+		                                   bxs, vout, ldout, ws, wss, nvec, beta, stream_);
+		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
+		                                    vout, ldout, nvec, beta, stream_);
+	}
+	DMK_HIP_CHECK(hipGetLastError());
+}
+
 void Plan::apply(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec, int beta)
 {
 	if (nvec < 1)
@@ -437,27 +735,7 @@ void Plan::apply(const double* vin, int64_t ldin, double* vout, int64_t ldout, i
 	if (nvec > 1 && (ldin < stats_.n || ldout < stats_.n))
 		throw std::runtime_error("dmk_apply: vector stride smaller than n");
 	ensure_scratch(nvec);
-	const int64_t bxs = stats_.bx_elems;
-	if (precision_ == PREC_FP64) {
-		const double* bases[BASE_COUNT] = { d_apack_, d_bpack_, vin, static_cast<double*>(d_bx_) };
-		launch_grouped_gemm<double, double>(st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
-		                                    int(st1_.tiles.size()), bases, ldin, bxs,
-		                                    static_cast<double*>(d_bx_), bxs, nvec, 0, stream_);
-		launch_grouped_gemm<double, double>(st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
-		                                    int(st2_.tiles.size()), bases, ldin, bxs, vout, ldout,
-		                                    nvec, beta, stream_);
-	} else {
-		for (int v = 0; v < nvec; ++v)
-			launch_cast_f64_to_f32(vin + v * ldin, d_x32_ + v * stats_.n, stats_.n, stream_);
-		const float* bases[BASE_COUNT] = { d_apack32_, d_bpack32_, d_x32_, static_cast<float*>(d_bx_) };
-		launch_grouped_gemm<float, float>(st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
-		                                  int(st1_.tiles.size()), bases, stats_.n, bxs,
-		                                  static_cast<float*>(d_bx_), bxs, nvec, 0, stream_);
-		launch_grouped_gemm<float, double>(st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
-		                                   int(st2_.tiles.size()), bases, stats_.n, bxs, vout, ldout,
-		                                   nvec, beta, stream_);
-	}
-	DMK_HIP_CHECK(hipGetLastError());
+	launch(vin, ldin, vout, ldout, nvec, beta, nullptr);
 }
 
 void Plan::profile(const double* vin, double* vout, int iters, double* ms1, double* ms2,
@@ -467,30 +745,9 @@ void Plan::profile(const double* vin, double* vout, int iters, double* ms1, doub
 		throw std::runtime_error("dmk_profile: iters must be >= 1");
 	double s1 = 0, s2 = 0, tot = 0;
 	ensure_scratch(1);
-	const int64_t bxs = stats_.bx_elems;
 	for (int it = 0; it < iters; ++it) {
 		DMK_HIP_CHECK(hipEventRecord(ev_[0], stream_));
-		if (precision_ == PREC_FP64) {
-			const double* bases[BASE_COUNT] = { d_apack_, d_bpack_, vin, static_cast<double*>(d_bx_) };
-			launch_grouped_gemm<double, double>(st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
-			                                    int(st1_.tiles.size()), bases, 0, bxs,
-			                                    static_cast<double*>(d_bx_), 0, 1, 0, stream_);
-			DMK_HIP_CHECK(hipEventRecord(ev_[1], stream_));
-			launch_grouped_gemm<double, double>(st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
-			                                    int(st2_.tiles.size()), bases, 0, bxs, vout, 0, 1, 0,
-			                                    stream_);
-		} else {
-			launch_cast_f64_to_f32(vin, d_x32_, stats_.n, stream_);
-			const float* bases[BASE_COUNT] = { d_apack32_, d_bpack32_, d_x32_,
-				                           static_cast<float*>(d_bx_) };
-			launch_grouped_gemm<float, float>(st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
-			                                  int(st1_.tiles.size()), bases, 0, bxs,
-			                                  static_cast<float*>(d_bx_), 0, 1, 0, stream_);
-			DMK_HIP_CHECK(hipEventRecord(ev_[1], stream_));
-			launch_grouped_gemm<float, double>(st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
-			                                   int(st2_.tiles.size()), bases, 0, bxs, vout, 0, 1, 0,
-			                                   stream_);
-		}
+		launch(vin, stats_.n, vout, stats_.n, 1, 0, ev_[1]);
 		DMK_HIP_CHECK(hipEventRecord(ev_[2], stream_));
 		DMK_HIP_CHECK(hipEventSynchronize(ev_[2]));
 		float a = 0, b = 0, c = 0;
@@ -507,6 +764,40 @@ void Plan::profile(const double* vin, double* vout, int iters, double* ms1, doub
 		*ms2 = s2 / iters;
 	if (mstot)
 		*mstot = tot / iters;
+}
+
+Tuning Plan::autotune(const double* vin, double* vout, int iters)
+{
+	if (iters < 1)
+		throw std::invalid_argument("dmk_autotune: iters must be >= 1");
+	const Tuning start = tuning_;
+	std::vector<Tuning> cands;
+	for (int bm1 : { 64, 128 })
+		for (int bm2 : { 64, 128 })
+			for (int sk : { 0, 512, 1024, 2048 }) {
+				Tuning t;
+				t.bm1    = bm1;
+				t.bn1    = 64;
+				t.bm2    = bm2;
+				t.bn2    = 64;
+				t.splitk = sk;
+				cands.push_back(t);
+			}
+	Tuning best     = start;
+	double best_ms  = 1e300;
+	for (const Tuning& t : cands) {
+		retune(t);
+		double ms = 0;
+		profile(vin, vout, 1, nullptr, nullptr, &ms); // warm-up
+		profile(vin, vout, iters, nullptr, nullptr, &ms);
+		if (ms < best_ms) {
+			best_ms = ms;
+			best    = t;
+		}
+	}
+	retune(best);
+	store_tuning(histogram_key(), best);
+	return best;
 }
 
 } // namespace dmk

@@ -5,6 +5,7 @@
 #include "dmk_internal.h"
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace dmk {
@@ -17,15 +18,16 @@ enum Precision : int
 };
 
 struct PlanStats {
-	int64_t n                = 0; // superblock dimension (vector length)
-	int64_t npatches         = 0;
-	int64_t noperator        = 0;
-	int64_t nterms           = 0; // nonzero (I,J,k) pairs (setup_nC total)
-	int64_t nterms_identity  = 0; // of which one factor is an identity
-	int64_t stage1_tasks     = 0;
-	int64_t stage2_tasks     = 0;
-	int64_t stage1_tiles     = 0;
-	int64_t stage2_tiles     = 0;
+	int64_t n                 = 0; // superblock dimension (vector length)
+	int64_t npatches          = 0;
+	int64_t noperator         = 0;
+	int64_t nterms            = 0; // nonzero (I,J,k) pairs (setup_nC total)
+	int64_t nterms_identity   = 0; // of which one factor is an identity
+	int64_t stage1_tasks      = 0;
+	int64_t stage2_tasks      = 0;
+	int64_t stage1_tiles      = 0;
+	int64_t stage2_tiles      = 0;
+	int64_t split_tasks       = 0; // stage-2 tasks cut into split-k pieces
 	double  flops_algorithmic = 0; // sum over terms of min(method 1, method 2), identities as one GEMM
 	double  flops_plugin      = 0; // the reference plugin's gflops1+gflops2 count (all terms dense)
 	double  flops_executed    = 0; // what the two stages execute (before tile padding)
@@ -36,11 +38,25 @@ struct PlanStats {
 	int64_t apack_elems       = 0;
 	int64_t bpack_elems       = 0;
 	int64_t bx_elems          = 0;
+	int64_t ws_elems          = 0; // split-k partials
 	int64_t max_patch_left    = 0;
 	int64_t max_patch_right   = 0;
 };
 
+// Launch shape of a plan: output tile per stage, split-k granularity of stage 2 (target
+// contraction length per piece; 0 disables), and the tile order.
+struct Tuning {
+	int bm1    = 64;
+	int bn1    = 64;
+	int bm2    = 64;
+	int bn2    = 64;
+	int splitk = 1024;
+	int xcd    = 0; // 1: XCD-aware tile order
+};
+
 struct StageTables {
+	int               bm = 64; // output tile shape of this stage's launch
+	int               bn = 64;
 	std::vector<Task> tasks;
 	std::vector<Seg>  segs;
 	std::vector<Tile> tiles;
@@ -50,6 +66,10 @@ struct StageTables {
 	void              upload();
 	void              release();
 };
+
+// Process-wide autotuner cache (histogram signature -> tuning).
+bool cached_tuning(const std::string& key, Tuning& t);
+void store_tuning(const std::string& key, const Tuning& t);
 
 class Plan {
 public:
@@ -68,6 +88,17 @@ public:
 	void set_precision(int p);
 	int  precision() const { return precision_; }
 
+	// Rebuilds the launch tables for a new tuning (tile shapes, split-k, order).
+	void          retune(const Tuning& t);
+	const Tuning& tuning() const { return tuning_; }
+
+	// Times candidate tunings on this problem (iters matvecs each, current precision) and keeps
+	// the fastest; the choice is cached per block-size histogram signature for later plans.
+	Tuning autotune(const double* vin, double* vout, int iters);
+
+	// Signature of the block-size histogram (log2 bins of the stage-2 task shapes).
+	std::string histogram_key() const;
+
 	const PlanStats& stats() const { return stats_; }
 	hipStream_t      stream() const { return stream_; }
 	int64_t          n() const { return stats_.n; }
@@ -80,21 +111,36 @@ public:
 private:
 	void ensure_scratch(int nvec);
 	void ensure_fp32();
+	void launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec, int beta,
+	            hipEvent_t mid);
 
 	int         precision_ = PREC_FP64;
+	Tuning      tuning_;
 	PlanStats   stats_;
 	StageTables st1_, st2_;
 	hipStream_t stream_ = nullptr;
 
+	// Unsplit tasks of both stages as planned from the blocks (retune builds the tables).
+	std::vector<Task>   t1_tasks_, t2_tasks_;
+	std::vector<Seg>    t1_segs_, t2_segs_;
+	std::vector<double> cost1_, cost2_;
+
+	std::vector<Reduce>      reduce_;
+	std::vector<ReduceChunk> chunks_;
+	Reduce*                  d_reduce_ = nullptr;
+	ReduceChunk*             d_chunks_ = nullptr;
+
 	std::vector<int64_t> off_;
-	double*              d_apack_ = nullptr;
-	double*              d_bpack_ = nullptr;
+	double*              d_apack_   = nullptr;
+	double*              d_bpack_   = nullptr;
 	float*               d_apack32_ = nullptr;
 	float*               d_bpack32_ = nullptr;
 	void*                d_bx_      = nullptr; // stage-1 products, bx_bytes_ bytes
+	void*                d_ws_      = nullptr; // split-k partials, ws_bytes_ bytes
 	float*               d_x32_     = nullptr; // FP32 copy of the input, x32_cap_ vectors
-	int                  x32_cap_   = 0; // vectors d_x32_ holds
+	int                  x32_cap_   = 0;       // vectors d_x32_ holds
 	size_t               bx_bytes_  = 0;
+	size_t               ws_bytes_  = 0;
 
 	hipEvent_t ev_[3] = { nullptr, nullptr, nullptr };
 };

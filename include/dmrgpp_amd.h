@@ -66,7 +66,20 @@ typedef struct dmk_info {
 	double  bytes_bx;          /* FP64 bytes of the stage-1 scratch written and read */
 	int32_t precision;
 	int32_t pad;
+	int64_t split_tasks;       /* stage-2 tasks cut into split-k pieces under the current tuning */
+	int64_t ws_elems;          /* split-k partial workspace (elements per vector) */
 } dmk_info;
+
+/* Launch shape of a handle: output tile (64 or 128 each way) of the stage-1 and stage-2
+ * grouped-GEMM launches, the split-k granularity of stage 2 (target contraction length per
+ * piece, 0 = never split; pieces are summed in a fixed order, so results stay deterministic),
+ * and the workgroup order (0 = by decreasing task cost, 1 = XCD-aware regions). */
+typedef struct dmk_tuning {
+	int32_t stage1_tile_m, stage1_tile_n;
+	int32_t stage2_tile_m, stage2_tile_n;
+	int32_t splitk;
+	int32_t xcd_order;
+} dmk_tuning;
 
 /* Replaces BatchedGemm<double>::BatchedGemm(noperator, npatches, left_patch_size,
  * right_patch_size, Amatrix, ld_Amatrix, Bmatrix, ld_Bmatrix) — dmrg/GPUPlugin/BatchedGemm.h:21-28,
@@ -100,6 +113,18 @@ int dmk_apply_multi_d(dmk_handle* handle, int nvec, const double* vin, size_t ld
 int dmk_set_precision(dmk_handle* handle, int precision);
 
 int dmk_get_info(const dmk_handle* handle, dmk_info* info);
+
+/* Tuning of the launches. No reference counterpart (the Kokkos path hard-codes its shape,
+ * BatchedGemmKokkos.h:110-130). dmk_set_tuning rebuilds the launch tables. */
+int dmk_get_tuning(const dmk_handle* handle, dmk_tuning* tuning);
+int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning);
+
+/* Times the candidate tunings on this handle's problem (iters matvecs each, in the current
+ * precision, on the device vectors vin/vout) and keeps the fastest. The choice is cached in the
+ * process under the block-size histogram of the problem, and handles created later for a
+ * problem with the same histogram start from it. chosen may be NULL. */
+int dmk_autotune_d(dmk_handle* handle, const double* vin, double* vout, int iters,
+                   dmk_tuning* chosen);
 
 /* Waits for all work queued on the handle's stream. */
 int dmk_sync(dmk_handle* handle);

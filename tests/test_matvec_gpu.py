@@ -194,3 +194,73 @@ def test_gpu_multi_vector_and_accumulate():
     # Host round trip gives the same numbers.
     assert np.array_equal(op.apply_host(X[0]), Y[0])
     op.close()
+
+
+TUNINGS = [
+    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=0,
+         xcd_order=0),
+    dict(stage1_tile_m=128, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=128, splitk=16,
+         xcd_order=0),
+    dict(stage1_tile_m=64, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=128, splitk=32,
+         xcd_order=1),
+    dict(stage1_tile_m=128, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=64, splitk=48,
+         xcd_order=1),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tuning", TUNINGS)
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_gpu_tunings_match_oracle(tuning, prec, tol):
+    # Every tile shape, forced split-k (pieces of 16-48 along K, ragged last piece) and both
+    # workgroup orders give the oracle's result; the same tuning twice is bitwise identical.
+    left = {-2: 70, 0: 130, 2: 70}
+    prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(2), seed=21)
+    rand = problems.random_problem([3, 70, 5, 130], [65, 2, 64, 9], 3, density=0.7, seed=2)
+    for p in (prob, rand):
+        x = np.random.default_rng(7).uniform(-1, 1, p.n)
+        want = oracle.plugin_apply(p, x)
+        op = dm.BatchedGemm(p, prec)
+        op.set_tuning(**tuning)
+        assert op.tuning() == tuning
+        if tuning["splitk"]:
+            assert op.info()["split_tasks"] > 0
+        dx, dy = dm.DeviceArray.from_host(x), dm.DeviceArray(p.n)
+        op.apply_Htarget(dx, dy)
+        op.sync()
+        y1 = dy.download()
+        op.apply_Htarget(dx, dy)
+        op.sync()
+        assert np.array_equal(dy.download(), y1)
+        assert rel_err(y1, want) < tol
+        # beta = 1 through the split-k reduction as well.
+        op.apply_multi(dx, dy, 1, beta=1)
+        op.sync()
+        assert rel_err(dy.download(), 2 * want) < tol
+        op.close()
+
+
+@pytest.mark.gpu
+def test_gpu_autotune_keeps_results():
+    left = {-2: 90, 0: 150, 2: 90}
+    prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(3), seed=9)
+    x = np.random.default_rng(3).uniform(-1, 1, prob.n)
+    want = oracle.plugin_apply(prob, x)
+    op = dm.BatchedGemm(prob)
+    dx, dy = dm.DeviceArray.from_host(x), dm.DeviceArray(prob.n)
+    chosen = op.autotune(dx, dy, iters=2)
+    assert chosen == op.tuning()
+    op.apply_Htarget(dx, dy)
+    op.sync()
+    assert rel_err(dy.download(), want) < 1e-12
+    op.close()
+    # A new handle for the same histogram starts from the cached choice.
+    op2 = dm.BatchedGemm(prob)
+    assert op2.tuning() == chosen
+    op2.close()
+
+
+def test_set_tuning_rejects_bad_shapes():
+    t = dm.Tuning(stage1_tile_m=96, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64,
+                  splitk=0, xcd_order=0)
+    assert dm.lib().dmk_set_tuning(None, ctypes.byref(t)) == -1
