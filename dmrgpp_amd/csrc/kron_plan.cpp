@@ -39,6 +39,13 @@ void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchu
                           int64_t wsstride, TO* out, int64_t ystride, int nvec, int beta,
                           hipStream_t stream);
 void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream);
+void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hipStream_t stream);
+void launch_grouped_gemm_bf16x3(int bm, int bn, const Task* tasks, const Seg* segs,
+                                const Tile* tiles, int ntiles, const void* const* bases_hi,
+                                const void* const* bases_lo, int64_t xstride, int64_t bxstride,
+                                double* out_y, int64_t ystride, float* out_f, void* out_h,
+                                void* out_l, int64_t wsstride, bool ws_pair, int nvec, int beta,
+                                hipStream_t stream);
 
 namespace {
 
@@ -474,6 +481,8 @@ Plan::~Plan()
 	(void)hipFree(d_bpack_);
 	(void)hipFree(d_apack32_);
 	(void)hipFree(d_bpack32_);
+	(void)hipFree(d_apack16_);
+	(void)hipFree(d_bpack16_);
 	(void)hipFree(d_bx_);
 	(void)hipFree(d_ws_);
 	(void)hipFree(d_x32_);
@@ -674,14 +683,32 @@ void Plan::ensure_fp32()
 	DMK_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
+// Packs as (hi, lo) bf16 pairs: hi parts of all elements, then the lo parts.
+void Plan::ensure_bf16()
+{
+	if (d_apack16_ || d_bpack16_)
+		return;
+	auto split = [&](const double* src, int64_t n, uint16_t** dst) {
+		if (n <= 0)
+			return;
+		DMK_HIP_CHECK(hipMalloc(dst, size_t(2 * n) * sizeof(uint16_t)));
+		launch_split_f64_bf16x2(src, *dst, *dst + n, n, stream_);
+	};
+	split(d_apack_, stats_.apack_elems, &d_apack16_);
+	split(d_bpack_, stats_.bpack_elems, &d_bpack16_);
+	DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
 void Plan::set_precision(int p)
 {
-	if (p != PREC_FP64 && p != PREC_FP32)
+	if (p != PREC_FP64 && p != PREC_FP32 && p != PREC_BF16X3)
 		throw std::runtime_error("dmk_set_precision: unsupported precision mode "
 		                         + std::to_string(p));
 	precision_ = p;
 	if (p == PREC_FP32)
 		ensure_fp32();
+	if (p == PREC_BF16X3)
+		ensure_bf16();
 	ensure_scratch(1);
 }
 
@@ -706,6 +733,30 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		                                    vout, ldout, ws, wss, nvec, beta, stream_);
 		launch_splitk_reduce<double, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                     vout, ldout, nvec, beta, stream_);
+	} else if (precision_ == PREC_BF16X3) {
+		// Input split into (hi, lo) bf16 pairs in the FP32 input buffer (same byte size).
+		uint16_t*     xh = reinterpret_cast<uint16_t*>(d_x32_);
+		uint16_t*     xl = xh + int64_t(nvec) * stats_.n;
+		uint16_t*     bh = static_cast<uint16_t*>(d_bx_);
+		uint16_t*     bl = bh + int64_t(nvec) * bxs;
+		const int64_t na = stats_.apack_elems, nb = stats_.bpack_elems;
+		for (int v = 0; v < nvec; ++v)
+			launch_split_f64_bf16x2(vin + v * ldin, xh + v * stats_.n, xl + v * stats_.n, stats_.n,
+			                        stream_);
+		const void* hi[BASE_COUNT] = { d_apack16_, d_bpack16_, xh, bh };
+		const void* lo[BASE_COUNT] = { d_apack16_ ? d_apack16_ + na : nullptr,
+			                       d_bpack16_ ? d_bpack16_ + nb : nullptr, xl, bl };
+		launch_grouped_gemm_bf16x3(st1_.bm, st1_.bn, st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
+		                           int(st1_.tiles.size()), hi, lo, stats_.n, bxs, vout, ldout,
+		                           nullptr, bh, bl, bxs, true, nvec, 0, stream_);
+		if (mid)
+			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
+		float* ws = static_cast<float*>(d_ws_);
+		launch_grouped_gemm_bf16x3(st2_.bm, st2_.bn, st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
+		                           int(st2_.tiles.size()), hi, lo, stats_.n, bxs, vout, ldout, ws,
+		                           nullptr, nullptr, wss, false, nvec, beta, stream_);
+		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
+		                                    vout, ldout, nvec, beta, stream_);
 	} else {
 		for (int v = 0; v < nvec; ++v)
 			launch_cast_f64_to_f32(vin + v * ldin, d_x32_ + v * stats_.n, stats_.n, stream_);

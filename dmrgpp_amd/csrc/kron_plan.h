@@ -111,6 +111,7 @@ public:
 private:
 	void ensure_scratch(int nvec);
 	void ensure_fp32();
+	void ensure_bf16();
 	void launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec, int beta,
 	            hipEvent_t mid);
 
@@ -135,6 +136,8 @@ private:
 	double*              d_bpack_   = nullptr;
 	float*               d_apack32_ = nullptr;
 	float*               d_bpack32_ = nullptr;
+	uint16_t*            d_apack16_ = nullptr; // bf16 (hi, lo) pairs: hi parts, then lo parts
+	uint16_t*            d_bpack16_ = nullptr;
 	void*                d_bx_      = nullptr; // stage-1 products, bx_bytes_ bytes
 	void*                d_ws_      = nullptr; // split-k partials, ws_bytes_ bytes
 	float*               d_x32_     = nullptr; // FP32 copy of the input, x32_cap_ vectors

@@ -207,7 +207,7 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 		throw std::invalid_argument("lanczos: null vector");
 	if (p.max_steps < 1 || p.min_steps < 0)
 		throw std::invalid_argument("lanczos: bad step limits");
-	if (p.precision != PREC_FP64 && p.precision != PREC_FP32)
+	if (p.precision != PREC_FP64 && p.precision != PREC_FP32 && p.precision != PREC_BF16X3)
 		throw std::runtime_error("lanczos: unsupported precision mode");
 	const auto    t0 = std::chrono::steady_clock::now();
 	const int64_t n  = plan.n();

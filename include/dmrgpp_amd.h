@@ -45,7 +45,7 @@ enum dmk_precision
 {
 	DMK_PREC_FP64   = 0, /* FP64 MFMA, FP64 accumulate */
 	DMK_PREC_FP32   = 1, /* FP32 MFMA on FP32 copies of factors and input, FP32 accumulate */
-	DMK_PREC_BF16X3 = 2, /* reserved: BF16 hi/lo split, three BF16 MFMAs, FP32 accumulate */
+	DMK_PREC_BF16X3 = 2, /* operands as bf16 (hi, lo) pairs, three BF16 MFMAs, FP32 accumulate */
 };
 
 typedef struct dmk_info {

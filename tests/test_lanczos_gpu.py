@@ -95,3 +95,14 @@ def test_gpu_lanczos_reortho():
     e_orc, _, _, _, _ = oracle.lanczos(prob, init, reortho=True)
     r, _ = gpu_lanczos(prob, init, reortho=1)
     assert abs(r["energy"] - e_orc) < 1e-10 * abs(e_orc)
+
+
+@pytest.mark.gpu
+def test_gpu_lanczos_bf16x3_with_fp64_refinement():
+    prob = _qn(3, {-4: 10, -2: 60, 0: 100, 2: 60, 4: 10})
+    init = np.random.default_rng(1234).uniform(-1, 1, prob.n)
+    r64, _ = gpu_lanczos(prob, init)
+    r16, _ = gpu_lanczos(prob, init, precision="bf16x3", refine=1)
+    assert r16["precision_first"] == dm.PREC_BF16X3
+    assert abs(r16["energy"] - r64["energy"]) < 1e-10 * abs(r64["energy"]), (r16, r64)
+    assert r16["residual"] < 1e-4

@@ -173,6 +173,19 @@ def test_gpu_fp32_mode_error_bound():
 
 
 @pytest.mark.gpu
+def test_gpu_bf16x3_mode_error_bound():
+    # (hi, lo) bf16 pairs carry ~2^-17 relative representation error per operand; with FP32
+    # accumulation the matvec error sits between the FP32 mode's and 1e-4.
+    left = {-2: 90, 0: 150, 2: 90}
+    prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(3), seed=9)
+    x = np.random.default_rng(3).uniform(-1, 1, prob.n)
+    want = oracle.plugin_apply(prob, x)
+    err16 = rel_err(gpu_apply(prob, x, "bf16x3"), want)
+    err32 = rel_err(gpu_apply(prob, x, "fp32"), want)
+    assert err32 < err16 < 1e-4, (err32, err16)
+
+
+@pytest.mark.gpu
 def test_gpu_multi_vector_and_accumulate():
     prob = problems.random_problem([30, 70, 12], [50, 9, 66], 3, density=0.8, seed=4)
     nvec = 4
@@ -210,7 +223,7 @@ TUNINGS = [
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("tuning", TUNINGS)
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-4)])
 def test_gpu_tunings_match_oracle(tuning, prec, tol):
     # Every tile shape, forced split-k (pieces of 16-48 along K, ragged last piece) and both
     # workgroup orders give the oracle's result; the same tuning twice is bitwise identical.
