@@ -62,7 +62,8 @@ class LanczosParams(ctypes.Structure):
     _fields_ = [("max_steps", ctypes.c_int32), ("min_steps", ctypes.c_int32),
                 ("tolerance", ctypes.c_double), ("reortho", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("refine", ctypes.c_int32),
-                ("refine_min_steps", ctypes.c_int32), ("refine_tolerance", ctypes.c_double)]
+                ("refine_min_steps", ctypes.c_int32), ("refine_tolerance", ctypes.c_double),
+                ("graph", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class LanczosResult(ctypes.Structure):

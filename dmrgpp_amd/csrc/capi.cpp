@@ -2,6 +2,7 @@
 #include "dmrgpp_amd.h"
 #include "kron_plan.h"
 #include "lanczos.h"
+#include "pool.h"
 #include "tridiag.h"
 
 #include <hip/hip_runtime.h>
@@ -259,6 +260,7 @@ int dmk_lanczos_default_params(dmk_lanczos_params* params)
 		params->refine           = d.refine;
 		params->refine_min_steps = d.refine_min_steps;
 		params->refine_tolerance = d.refine_tolerance;
+		params->graph            = d.graph;
 		return DMK_OK;
 	});
 }
@@ -279,6 +281,7 @@ int dmk_lanczos_ground_d(dmk_handle* handle, const double* init, double* z,
 		lp.refine           = params->refine;
 		lp.refine_min_steps = params->refine_min_steps;
 		lp.refine_tolerance = params->refine_tolerance;
+		lp.graph            = params->graph;
 		if (!handle->lanczos)
 			handle->lanczos = std::make_unique<dmk::LanczosWorkspace>();
 		dmk::LanczosReport r = dmk::lanczos_ground(p, *handle->lanczos, init, z, lp);
@@ -317,7 +320,7 @@ int dmk_device_alloc(size_t bytes, void** ptr)
 		if (!ptr)
 			throw std::invalid_argument("dmk_device_alloc: null pointer");
 		*ptr = nullptr;
-		DMK_HIP_CHECK(hipMalloc(ptr, bytes ? bytes : 1));
+		*ptr = dmk::pool_device_alloc(bytes ? bytes : 1);
 		return DMK_OK;
 	});
 }
@@ -325,7 +328,13 @@ int dmk_device_alloc(size_t bytes, void** ptr)
 int dmk_device_free(void* ptr)
 {
 	return guarded([&] {
-		DMK_HIP_CHECK(hipFree(ptr));
+		if (dmk::pool_device_owns(ptr)) {
+			// Work queued on any stream may still read it: wait before it can be handed out again.
+			DMK_HIP_CHECK(hipDeviceSynchronize());
+			dmk::pool_device_free(ptr);
+		} else {
+			DMK_HIP_CHECK(hipFree(ptr));
+		}
 		return DMK_OK;
 	});
 }

@@ -17,6 +17,8 @@
 //    apply_Htarget_sparse_not_ready.cpp:99-276).
 #include "kron_plan.h"
 
+#include "pool.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -81,8 +83,7 @@ template <typename X> X* dev_alloc_copy(const std::vector<X>& h)
 {
 	if (h.empty())
 		return nullptr;
-	X* d = nullptr;
-	DMK_HIP_CHECK(hipMalloc(&d, h.size() * sizeof(X)));
+	X* d = static_cast<X*>(pool_device_alloc(h.size() * sizeof(X)));
 	DMK_HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice));
 	return d;
 }
@@ -216,9 +217,9 @@ void StageTables::upload()
 
 void StageTables::release()
 {
-	(void)hipFree(d_tasks);
-	(void)hipFree(d_segs);
-	(void)hipFree(d_tiles);
+	pool_device_free(d_tasks);
+	pool_device_free(d_segs);
+	pool_device_free(d_tiles);
 	d_tasks = nullptr;
 	d_segs  = nullptr;
 	d_tiles = nullptr;
@@ -269,45 +270,74 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	for (int64_t k = 0; k < nC; ++k)
 		for (int64_t J = 0; J < np; ++J)
 			for (int64_t I = 0; I < np; ++I) {
-				const size_t  id = idx(I, J, k);
-				const double* a  = Amatrix[id];
-				const double* b  = Bmatrix[id];
-				if (!a || !b)
-					continue;
-				if (int64_t(ldA[id]) < L[I] || int64_t(ldB[id]) < R[I])
+				const size_t id = idx(I, J, k);
+				if (Amatrix[id] && Bmatrix[id]
+				    && (int64_t(ldA[id]) < L[I] || int64_t(ldB[id]) < R[I]))
 					throw std::runtime_error("dmk_create: leading dimension smaller than rows");
-				const int64_t nnzA = count_nnz(a, L[I], L[J], int64_t(ldA[id]));
-				const int64_t nnzB = count_nnz(b, R[I], R[J], int64_t(ldB[id]));
-				if (nnzA == 0 || nnzB == 0)
-					continue;
-				stats_.nterms++;
-				const double RI = double(R[I]), RJ = double(R[J]), LI = double(L[I]),
-				             LJ = double(L[J]);
-				stats_.flops_plugin += 2.0 * RI * RJ * LJ + 2.0 * RI * LJ * LI;
-				if (is_identity(b, R[I], R[J], int64_t(ldB[id]))) {
-					kind[id] = IDENT_B;
-					stats_.nterms_identity++;
-					stats_.flops_algorithmic += 2.0 * RI * LI * LJ;
-					stats_.bytes_compulsory += 8.0 * LI * LJ;
-				} else if (is_identity(a, L[I], L[J], int64_t(ldA[id]))) {
-					kind[id] = IDENT_A;
-					stats_.nterms_identity++;
-					stats_.flops_algorithmic += 2.0 * RI * RJ * LI;
-					stats_.bytes_compulsory += 8.0 * RI * RJ;
-				} else {
-					kind[id] = GENERAL;
-					const double f1 = 2.0 * RI * RJ * LJ + 2.0 * RI * LJ * LI;
-					const double f2 = 2.0 * RJ * LJ * LI + 2.0 * RI * RJ * LI;
-					stats_.flops_algorithmic += std::min(f1, f2);
-					stats_.bytes_compulsory += 8.0 * (LI * LJ + RI * RJ);
-				}
 			}
+	// Scan every present block once, in parallel (nonzero test and identity test).
+	std::vector<size_t> present;
+	for (size_t id = 0; id < kind.size(); ++id)
+		if (Amatrix[id] && Bmatrix[id])
+			present.push_back(id);
+	std::vector<char> scan(present.size(), NONE);
+	parallel_for(present.size(), [&](size_t t) {
+		const size_t  id = present[t];
+		const int64_t I  = int64_t(id % size_t(np));
+		const int64_t J  = int64_t((id / size_t(np)) % size_t(np));
+		const double* a  = Amatrix[id];
+		const double* b  = Bmatrix[id];
+		if (count_nnz(a, L[I], L[J], int64_t(ldA[id])) == 0
+		    || count_nnz(b, R[I], R[J], int64_t(ldB[id])) == 0)
+			scan[t] = NONE;
+		else if (is_identity(b, R[I], R[J], int64_t(ldB[id])))
+			scan[t] = IDENT_B;
+		else if (is_identity(a, L[I], L[J], int64_t(ldA[id])))
+			scan[t] = IDENT_A;
+		else
+			scan[t] = GENERAL;
+	});
+	for (size_t t = 0; t < present.size(); ++t) {
+		const size_t  id = present[t];
+		const int64_t I  = int64_t(id % size_t(np));
+		const int64_t J  = int64_t((id / size_t(np)) % size_t(np));
+		kind[id]         = scan[t];
+		if (scan[t] == NONE)
+			continue;
+		stats_.nterms++;
+		const double RI = double(R[I]), RJ = double(R[J]), LI = double(L[I]), LJ = double(L[J]);
+		stats_.flops_plugin += 2.0 * RI * RJ * LJ + 2.0 * RI * LJ * LI;
+		if (scan[t] == IDENT_B) {
+			stats_.nterms_identity++;
+			stats_.flops_algorithmic += 2.0 * RI * LI * LJ;
+			stats_.bytes_compulsory += 8.0 * LI * LJ;
+		} else if (scan[t] == IDENT_A) {
+			stats_.nterms_identity++;
+			stats_.flops_algorithmic += 2.0 * RI * RJ * LI;
+			stats_.bytes_compulsory += 8.0 * RI * RJ;
+		} else {
+			const double f1 = 2.0 * RI * RJ * LJ + 2.0 * RI * LJ * LI;
+			const double f2 = 2.0 * RJ * LJ * LI + 2.0 * RI * RJ * LI;
+			stats_.flops_algorithmic += std::min(f1, f2);
+			stats_.bytes_compulsory += 8.0 * (LI * LJ + RI * RJ);
+		}
+	}
 	stats_.bytes_compulsory += 8.0 * 2.0 * double(stats_.n);
 
 	// Packing. A panel of patch I: L_I x K_I, (J outer, k inner) general terms; then the
 	// identity-B terms' A blocks. B stack of (I, J): q*R_I x R_J for its q general terms;
 	// then identity-A terms' B blocks.
-	std::vector<double> apack, bpack;
+	// Packing is planned first (offsets and copy jobs) and done afterwards in parallel into a
+	// pinned staging buffer, then uploaded with one asynchronous copy per pack.
+	struct Copy {
+		double*       dst_is_b; // nullptr: A pack, else B pack (the value is a tag only)
+		int64_t       dst;      // element offset in the pack
+		const double* src;
+		int64_t       rows, cols, ldsrc, lddst;
+	};
+	std::vector<Copy> copies;
+	int64_t           asize = 0, bsize = 0;
+	double* const     BTAG  = reinterpret_cast<double*>(1);
 	std::vector<int64_t> apanel_off(np, 0), K(np, 0), bx_off(np + 1, 0);
 	for (int64_t I = 0; I < np; ++I) {
 		for (int64_t J = 0; J < np; ++J)
@@ -327,8 +357,8 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 
 	for (int64_t I = 0; I < np; ++I) {
 		// A panel for the stage-2 GEMM of patch I.
-		apanel_off[I] = int64_t(apack.size());
-		apack.resize(apack.size() + size_t(L[I] * K[I]));
+		apanel_off[I] = asize;
+		asize += L[I] * K[I];
 		int64_t col = 0;
 		for (int64_t J = 0; J < np; ++J) {
 			// Stage-1 task for (I, J): the row-stacked B blocks of its general terms.
@@ -339,20 +369,15 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			if (ks.empty())
 				continue;
 			const int64_t q       = int64_t(ks.size());
-			const int64_t boff    = int64_t(bpack.size());
+			const int64_t boff    = bsize;
 			const int64_t ldstack = q * R[I];
-			bpack.resize(bpack.size() + size_t(ldstack * R[J]));
+			bsize += ldstack * R[J];
 			for (int64_t t = 0; t < q; ++t) {
-				const size_t  id = idx(I, J, ks[t]);
-				const double* a  = Amatrix[id];
-				const double* b  = Bmatrix[id];
-				double*       ad = apack.data() + apanel_off[I] + (col + t * L[J]) * L[I];
-				for (int64_t c = 0; c < L[J]; ++c)
-					std::memcpy(ad + c * L[I], a + c * int64_t(ldA[id]), size_t(L[I]) * sizeof(double));
-				double* bd = bpack.data() + boff + t * R[I];
-				for (int64_t c = 0; c < R[J]; ++c)
-					std::memcpy(bd + c * ldstack, b + c * int64_t(ldB[id]),
-					            size_t(R[I]) * sizeof(double));
+				const size_t id = idx(I, J, ks[t]);
+				copies.push_back(Copy { nullptr, apanel_off[I] + (col + t * L[J]) * L[I],
+				                        Amatrix[id], L[I], L[J], int64_t(ldA[id]), L[I] });
+				copies.push_back(Copy { BTAG, boff + t * R[I], Bmatrix[id], R[I], R[J],
+				                        int64_t(ldB[id]), ldstack });
 			}
 			Task tk {};
 			tk.coff      = bx_off[I] + col * R[I];
@@ -410,12 +435,10 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 				const size_t id = idx(I, J, k);
 				if (kind[id] == IDENT_B) {
 					// Y_I += X_J * A^T: L = X_J (R_J x L_J), R = A (L_I x L_J) transposed.
-					const int64_t aoff = int64_t(apack.size());
-					apack.resize(apack.size() + size_t(L[I] * L[J]));
-					const double* a = Amatrix[id];
-					for (int64_t c = 0; c < L[J]; ++c)
-						std::memcpy(apack.data() + aoff + c * L[I], a + c * int64_t(ldA[id]),
-						            size_t(L[I]) * sizeof(double));
+					const int64_t aoff = asize;
+					asize += L[I] * L[J];
+					copies.push_back(Copy { nullptr, aoff, Amatrix[id], L[I], L[J],
+					                        int64_t(ldA[id]), L[I] });
 					Seg s {};
 					s.lbase  = BASE_X;
 					s.loff   = off_[J];
@@ -429,12 +452,10 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 					cost += 2.0 * double(R[I]) * double(L[I]) * double(L[J]);
 				} else if (kind[id] == IDENT_A) {
 					// Y_I += B * X_J: L = B (R_I x R_J), R = X_J (R_J x L_J).
-					const int64_t boff = int64_t(bpack.size());
-					bpack.resize(bpack.size() + size_t(R[I] * R[J]));
-					const double* b = Bmatrix[id];
-					for (int64_t c = 0; c < R[J]; ++c)
-						std::memcpy(bpack.data() + boff + c * R[I], b + c * int64_t(ldB[id]),
-						            size_t(R[I]) * sizeof(double));
+					const int64_t boff = bsize;
+					bsize += R[I] * R[J];
+					copies.push_back(Copy { BTAG, boff, Bmatrix[id], R[I], R[J],
+					                        int64_t(ldB[id]), R[I] });
 					Seg s {};
 					s.lbase  = BASE_BPACK;
 					s.loff   = boff;
@@ -457,16 +478,34 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 
 	stats_.stage1_tasks = int64_t(t1_tasks_.size());
 	stats_.stage2_tasks = int64_t(t2_tasks_.size());
-	stats_.apack_elems  = int64_t(apack.size());
-	stats_.bpack_elems  = int64_t(bpack.size());
+	stats_.apack_elems  = asize;
+	stats_.bpack_elems  = bsize;
 	if (stats_.n > (int64_t(1) << 31) * 64)
 		throw std::runtime_error("dmk_create: superblock too large");
 
 	DMK_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 	for (auto& e : ev_)
 		DMK_HIP_CHECK(hipEventCreate(&e));
-	d_apack_ = dev_alloc_copy(apack);
-	d_bpack_ = dev_alloc_copy(bpack);
+	// Pack into pinned staging memory in parallel, then upload both packs asynchronously.
+	const size_t abytes = size_t(std::max<int64_t>(asize, 1)) * sizeof(double);
+	const size_t bbytes = size_t(std::max<int64_t>(bsize, 1)) * sizeof(double);
+	double*      ha     = static_cast<double*>(pool_host_alloc(abytes + bbytes));
+	double*      hb     = ha + std::max<int64_t>(asize, 1);
+	parallel_for(copies.size(), [&](size_t t) {
+		const Copy& c = copies[t];
+		double*     d = (c.dst_is_b ? hb : ha) + c.dst;
+		for (int64_t j = 0; j < c.cols; ++j)
+			std::memcpy(d + j * c.lddst, c.src + j * c.ldsrc, size_t(c.rows) * sizeof(double));
+	});
+	d_apack_ = static_cast<double*>(pool_device_alloc(abytes));
+	d_bpack_ = static_cast<double*>(pool_device_alloc(bbytes));
+	hipError_t e1 = hipMemcpyAsync(d_apack_, ha, abytes, hipMemcpyHostToDevice, stream_);
+	hipError_t e2 = hipMemcpyAsync(d_bpack_, hb, bbytes, hipMemcpyHostToDevice, stream_);
+	hipError_t e3 = hipStreamSynchronize(stream_);
+	pool_host_free(ha);
+	DMK_HIP_CHECK(e1);
+	DMK_HIP_CHECK(e2);
+	DMK_HIP_CHECK(e3);
 	// A tuning the autotuner chose for this histogram earlier in the process, else the default.
 	Tuning t;
 	cached_tuning(histogram_key(), t);
@@ -477,17 +516,17 @@ Plan::~Plan()
 {
 	if (stream_)
 		(void)hipStreamSynchronize(stream_);
-	(void)hipFree(d_apack_);
-	(void)hipFree(d_bpack_);
-	(void)hipFree(d_apack32_);
-	(void)hipFree(d_bpack32_);
-	(void)hipFree(d_apack16_);
-	(void)hipFree(d_bpack16_);
-	(void)hipFree(d_bx_);
-	(void)hipFree(d_ws_);
-	(void)hipFree(d_x32_);
-	(void)hipFree(d_reduce_);
-	(void)hipFree(d_chunks_);
+	pool_device_free(d_apack_);
+	pool_device_free(d_bpack_);
+	pool_device_free(d_apack32_);
+	pool_device_free(d_bpack32_);
+	pool_device_free(d_apack16_);
+	pool_device_free(d_bpack16_);
+	pool_device_free(d_bx_);
+	pool_device_free(d_ws_);
+	pool_device_free(d_x32_);
+	pool_device_free(d_reduce_);
+	pool_device_free(d_chunks_);
 	st1_.release();
 	st2_.release();
 	for (auto& e : ev_)
@@ -622,8 +661,8 @@ void Plan::retune(const Tuning& t)
 	for (size_t r = 0; r < reduce_.size(); ++r)
 		for (int64_t s0 = 0; s0 < reduce_[r].count; s0 += 4096)
 			chunks_.push_back(ReduceChunk { int32_t(r), int32_t(std::min<int64_t>(4096, reduce_[r].count - s0)), s0 });
-	(void)hipFree(d_reduce_);
-	(void)hipFree(d_chunks_);
+	pool_device_free(d_reduce_);
+	pool_device_free(d_chunks_);
 	d_reduce_ = dev_alloc_copy(reduce_);
 	d_chunks_ = dev_alloc_copy(chunks_);
 	st1_.upload();
@@ -631,7 +670,7 @@ void Plan::retune(const Tuning& t)
 	stats_.stage1_tiles = int64_t(st1_.tiles.size());
 	stats_.stage2_tiles = int64_t(st2_.tiles.size());
 	ws_bytes_           = 0; // force the workspace re-check
-	(void)hipFree(d_ws_);
+	pool_device_free(d_ws_);
 	d_ws_ = nullptr;
 	ensure_scratch(1);
 }
@@ -642,28 +681,28 @@ void Plan::ensure_scratch(int nvec)
 	const size_t bytes = size_t(std::max<int64_t>(stats_.bx_elems, 1)) * elem * size_t(nvec);
 	if (bytes > bx_bytes_) {
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
-		(void)hipFree(d_bx_);
+		pool_device_free(d_bx_);
 		d_bx_     = nullptr;
 		bx_bytes_ = 0;
-		DMK_HIP_CHECK(hipMalloc(&d_bx_, bytes));
+		d_bx_     = pool_device_alloc(bytes);
 		bx_bytes_ = bytes;
 	}
 	const size_t wbytes = size_t(std::max<int64_t>(stats_.ws_elems, 1)) * elem * size_t(nvec);
 	if (wbytes > ws_bytes_) {
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
-		(void)hipFree(d_ws_);
+		pool_device_free(d_ws_);
 		d_ws_     = nullptr;
 		ws_bytes_ = 0;
-		DMK_HIP_CHECK(hipMalloc(&d_ws_, wbytes));
+		d_ws_     = pool_device_alloc(wbytes);
 		ws_bytes_ = wbytes;
 	}
 	if (precision_ != PREC_FP64 && nvec > x32_cap_) {
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
-		(void)hipFree(d_x32_);
+		pool_device_free(d_x32_);
 		d_x32_   = nullptr;
 		x32_cap_ = 0;
-		DMK_HIP_CHECK(hipMalloc(&d_x32_, size_t(std::max<int64_t>(stats_.n, 1)) * sizeof(float)
-		                                      * size_t(nvec)));
+		d_x32_   = static_cast<float*>(pool_device_alloc(size_t(std::max<int64_t>(stats_.n, 1))
+		                                                 * sizeof(float) * size_t(nvec)));
 		x32_cap_ = nvec;
 	}
 }
@@ -675,7 +714,7 @@ void Plan::ensure_fp32()
 	auto convert = [&](const double* src, int64_t n, float** dst) {
 		if (n <= 0)
 			return;
-		DMK_HIP_CHECK(hipMalloc(dst, size_t(n) * sizeof(float)));
+		*dst = static_cast<float*>(pool_device_alloc(size_t(n) * sizeof(float)));
 		launch_cast_f64_to_f32(src, *dst, n, stream_);
 	};
 	convert(d_apack_, stats_.apack_elems, &d_apack32_);
@@ -691,7 +730,7 @@ void Plan::ensure_bf16()
 	auto split = [&](const double* src, int64_t n, uint16_t** dst) {
 		if (n <= 0)
 			return;
-		DMK_HIP_CHECK(hipMalloc(dst, size_t(2 * n) * sizeof(uint16_t)));
+		*dst = static_cast<uint16_t*>(pool_device_alloc(size_t(2 * n) * sizeof(uint16_t)));
 		launch_split_f64_bf16x2(src, *dst, *dst + n, n, stream_);
 	};
 	split(d_apack_, stats_.apack_elems, &d_apack16_);

@@ -11,6 +11,7 @@
 // FP64 matvecs because that vector is already close to the FP64 eigenvector.
 #include "lanczos.h"
 
+#include "pool.h"
 #include "tridiag.h"
 
 #include <chrono>
@@ -33,36 +34,54 @@ void launch_combine(double* z, const double* V, int64_t ld, const double* c, int
                     hipStream_t s);
 void launch_residual(const double* hx, const double* x, double e, int64_t n, double* partial,
                      double* out, hipStream_t s);
+void launch_dot_at(const double* x, const double* y, int64_t n, double* partial, double* base,
+                   const int* jdev, int offset, hipStream_t s);
+void launch_lanczos_update_at(double* v2, const double* v1, const double* v0, const double* A,
+                              const double* B, const int* jdev, int64_t n, double* partial,
+                              double* norm_out, hipStream_t s);
+void launch_finish_step(double* v2, double* basis, int64_t n, const double* nrm, const int* jdev,
+                        hipStream_t s);
+void launch_step_scalars(const double* A, double* B, const double* nrm, int* jdev, int maxn,
+                         double* report, hipStream_t s);
+void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
 
 LanczosWorkspace::~LanczosWorkspace()
 {
-	(void)hipFree(basis_);
-	(void)hipFree(partial_);
-	(void)hipFree(scal_);
-	(void)hipFree(tmp_);
-	(void)hipFree(tmp2_);
-	(void)hipHostFree(host_);
+	pool_device_free(basis_);
+	pool_device_free(partial_);
+	pool_device_free(scal_);
+	pool_device_free(tmp_);
+	pool_device_free(tmp2_);
+	pool_device_free(ring_);
+	pool_device_free(jdev_);
+	pool_host_free(host_);
 }
 
 void LanczosWorkspace::reserve(int64_t n, int columns)
 {
 	if (n == n_ && columns <= cols_ && basis_)
 		return;
-	(void)hipFree(basis_);
-	(void)hipFree(partial_);
-	(void)hipFree(scal_);
-	(void)hipFree(tmp_);
-	(void)hipFree(tmp2_);
-	(void)hipHostFree(host_);
-	basis_ = partial_ = scal_ = tmp_ = tmp2_ = host_ = nullptr;
+	pool_device_free(basis_);
+	pool_device_free(partial_);
+	pool_device_free(scal_);
+	pool_device_free(tmp_);
+	pool_device_free(tmp2_);
+	pool_device_free(ring_);
+	pool_device_free(jdev_);
+	pool_host_free(host_);
+	basis_ = partial_ = scal_ = tmp_ = tmp2_ = ring_ = host_ = nullptr;
+	jdev_  = nullptr;
 	n_    = 0;
 	cols_ = 0;
-	DMK_HIP_CHECK(hipMalloc(&basis_, size_t(n) * size_t(columns) * sizeof(double)));
-	DMK_HIP_CHECK(hipMalloc(&partial_, size_t(reduction_blocks()) * sizeof(double)));
-	DMK_HIP_CHECK(hipMalloc(&scal_, size_t(3 * columns + 8) * sizeof(double)));
-	DMK_HIP_CHECK(hipMalloc(&tmp_, size_t(n) * sizeof(double)));
-	DMK_HIP_CHECK(hipMalloc(&tmp2_, size_t(n) * sizeof(double)));
-	DMK_HIP_CHECK(hipHostMalloc(&host_, size_t(3 * columns + 8) * sizeof(double), hipHostMallocDefault));
+	auto dev = [](size_t bytes) { return static_cast<double*>(pool_device_alloc(bytes)); };
+	basis_   = dev(size_t(n) * size_t(columns) * sizeof(double));
+	partial_ = dev(size_t(reduction_blocks()) * sizeof(double));
+	scal_    = dev(size_t(3 * columns + 8) * sizeof(double));
+	tmp_     = dev(size_t(n) * sizeof(double));
+	tmp2_    = dev(size_t(n) * sizeof(double));
+	ring_    = dev(size_t(3) * size_t(n) * sizeof(double));
+	jdev_    = static_cast<int*>(pool_device_alloc(sizeof(int)));
+	host_    = static_cast<double*>(pool_host_alloc(size_t(3 * columns + 8) * sizeof(double)));
 	n_    = n;
 	cols_ = columns;
 }
@@ -78,8 +97,26 @@ struct Decomp {
 
 // One Lanczos decomposition from init (any norm) with the plan's current precision. Leaves the
 // kept vectors in ws.basis(0..m).
+// The three captured phase graphs of one decomposition (ring roles rotate with period 3).
+struct StepGraphs {
+	hipGraph_t     g[3] = { nullptr, nullptr, nullptr };
+	hipGraphExec_t e[3] = { nullptr, nullptr, nullptr };
+	~StepGraphs()
+	{
+		for (int i = 0; i < 3; ++i) {
+			if (e[i])
+				(void)hipGraphExecDestroy(e[i]);
+			if (g[i])
+				(void)hipGraphDestroy(g[i]);
+		}
+	}
+};
+
+// noise: relative matvec noise of a reduced-precision run that an FP64 restart follows (0 for
+// a plain run); see noise_reached below.
 Decomp decompose(Plan& plan, LanczosWorkspace& ws, const double* init, int max_steps, int min_steps,
-                 double tol, int reortho, double& ms_matvec, hipEvent_t ev0, hipEvent_t ev1)
+                 double tol, double noise, int reortho, bool use_graph, double& ms_matvec,
+                 hipEvent_t ev0, hipEvent_t ev1)
 {
 	const int64_t n      = plan.n();
 	hipStream_t   s      = plan.stream();
@@ -135,6 +172,64 @@ Decomp decompose(Plan& plan, LanczosWorkspace& ws, const double* init, int max_s
 	double eold  = 0;
 	int    j     = 1;
 	bool   broke = false;
+	// A reduced-precision run (noise > 0) stops when the residual estimate of its lowest Ritz
+	// pair, |beta_{j+1} s_j| with s the unit eigenvector of T_{j+1}, drops below
+	// noise * ||H|| (||H|| estimated by the extreme Ritz values): past that point its matvec noise
+	// dominates, and the FP64 restart takes over.
+	auto noise_reached = [&](int jj, double e0) {
+		if (noise <= 0 || jj + 1 >= maxn)
+			return false;
+		const double bn = d.b[size_t(jj + 1)];
+		std::vector<double> c = tridiag_eigenvector(d.a.data(), d.b.data(), jj + 1, e0);
+		const double emax  = tridiag_eigenvalue(d.a.data(), d.b.data(), jj + 1, jj);
+		const double hnorm = std::max(std::fabs(e0), std::fabs(emax));
+		return std::fabs(bn * c[size_t(jj)]) < noise * hnorm;
+	};
+	if (use_graph && !reortho && maxn > 1) {
+		// Ring of three fixed vectors: V_{j-1}, V_j, V_{j+1} live in ring[(j-1)%3], ring[j%3],
+		// ring[(j+1)%3]; each captured step also writes V_{j+1} to basis column j+1.
+		double* report = ws.scal() + 3 * cap + 4;
+		launch_copy(ws.ring(0), ws.basis(0), n, s);
+		launch_copy(ws.ring(1), ws.basis(1), n, s);
+		const int one = 1;
+		DMK_HIP_CHECK(hipMemcpyAsync(ws.jdev(), &one, sizeof(int), hipMemcpyHostToDevice, s));
+		DMK_HIP_CHECK(hipStreamSynchronize(s));
+		StepGraphs sg;
+		for (int p = 0; p < 3; ++p) {
+			double* v0 = ws.ring((p + 2) % 3);
+			double* v1 = ws.ring(p);
+			double* v2 = ws.ring((p + 1) % 3);
+			DMK_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+			plan.apply(v1, n, v2, n, 1, 0);
+			launch_dot_at(v1, v2, n, part, A, ws.jdev(), 0, s);
+			launch_lanczos_update_at(v2, v1, v0, A, B, ws.jdev(), n, part, NRM, s);
+			launch_finish_step(v2, ws.basis(0), n, NRM, ws.jdev(), s);
+			launch_step_scalars(A, B, NRM, ws.jdev(), maxn, report, s);
+			(void)hipMemcpyAsync(host, report, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+			hipError_t ec = hipStreamEndCapture(s, &sg.g[p]);
+			DMK_HIP_CHECK(ec);
+			DMK_HIP_CHECK(hipGraphInstantiate(&sg.e[p], sg.g[p], nullptr, nullptr, 0));
+		}
+		const auto t0 = std::chrono::steady_clock::now();
+		for (j = 1; j < maxn; ++j) {
+			DMK_HIP_CHECK(hipGraphLaunch(sg.e[j % 3], s));
+			DMK_HIP_CHECK(hipStreamSynchronize(s));
+			d.matvecs++;
+			d.a[size_t(j)] = host[0];
+			if (j + 1 < maxn)
+				d.b[size_t(j + 1)] = host[1];
+			const double e     = tridiag_eigenvalue(d.a.data(), d.b.data(), j + 1, 0);
+			const double delta = std::fabs(e - eold);
+			eold               = e;
+			bool exitFlag = (delta < tol) || (j == maxn - 1) || noise_reached(j, e);
+			if (exitFlag && (n <= 4 || j >= min_steps)) {
+				broke = true;
+				break;
+			}
+		}
+		// Device time is not split out per matvec inside the graphs; charge the loop.
+		ms_matvec += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+	} else
 	for (j = 1; j < maxn; ++j) {
 		double* v0 = ws.basis(j - 1);
 		double* v1 = ws.basis(j);
@@ -167,7 +262,7 @@ Decomp decompose(Plan& plan, LanczosWorkspace& ws, const double* init, int max_s
 		const double e     = tridiag_eigenvalue(d.a.data(), d.b.data(), j + 1, 0);
 		const double delta = std::fabs(e - eold);
 		eold               = e;
-		bool exitFlag      = (delta < tol) || (j == maxn - 1);
+		bool exitFlag      = (delta < tol) || (j == maxn - 1) || noise_reached(j, e);
 		if (exitFlag && n <= 4) {
 			broke = true;
 			break;
@@ -219,8 +314,12 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 	LanczosReport r;
 	try {
 		plan.set_precision(p.precision);
-		Decomp d = decompose(plan, ws, init, p.max_steps, p.min_steps, p.tolerance, p.reortho,
-		                     r.ms_matvec, ev0, ev1);
+		// Relative matvec noise of the reduced-precision modes (about 16 units of roundoff of the
+		// FP32 accumulation; the (hi, lo) bf16 pair representation for BF16x3).
+		const double noise = (p.precision == PREC_FP32) ? 1e-6 : (p.precision == PREC_BF16X3) ? 1e-4 : 0.0;
+		Decomp d = decompose(plan, ws, init, p.max_steps, p.min_steps, p.tolerance,
+		                     (p.refine ? noise : 0.0), p.reortho, p.graph != 0, r.ms_matvec, ev0,
+		                     ev1);
 		ritz_vector(plan, ws, d, z);
 		r.energy_first    = d.energy;
 		r.steps_first     = d.m;
@@ -235,7 +334,8 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 			DMK_HIP_CHECK(hipMemcpyAsync(ws.tmp2(), z, size_t(n) * sizeof(double),
 			                             hipMemcpyDeviceToDevice, plan.stream()));
 			Decomp d2 = decompose(plan, ws, ws.tmp2(), p.max_steps, p.refine_min_steps,
-			                      p.refine_tolerance, p.reortho, r.ms_matvec, ev0, ev1);
+			                      p.refine_tolerance, 0.0, p.reortho, p.graph != 0, r.ms_matvec,
+			                      ev0, ev1);
 			ritz_vector(plan, ws, d2, z);
 			r.energy = d2.energy;
 			r.steps  = d2.m;

@@ -16,6 +16,7 @@ struct LanczosParams {
 	int    refine           = 1;     // FP64 restart from the mixed-precision Ritz vector
 	int    refine_min_steps = 4;
 	double refine_tolerance = 1e-12;
+	int    graph            = 1;     // capture each step in a HIP graph (not with reortho)
 };
 
 struct LanczosReport {
@@ -41,6 +42,8 @@ public:
 	double* scal() { return scal_; } // device scalars: a[cap], b[cap], norm/inverse pairs, coefs
 	double* tmp() { return tmp_; }
 	double* tmp2() { return tmp2_; }
+	double* ring(int i) { return ring_ + int64_t(i) * n_; } // three work vectors (graph mode)
+	int*    jdev() { return jdev_; }
 	double* host() { return host_; }
 	int     columns() const { return cols_; }
 
@@ -52,6 +55,8 @@ private:
 	double* scal_    = nullptr;
 	double* tmp_     = nullptr;
 	double* tmp2_    = nullptr;
+	double* ring_    = nullptr;
+	int*    jdev_    = nullptr;
 	double* host_    = nullptr; // pinned
 };
 

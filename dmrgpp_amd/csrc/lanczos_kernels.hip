@@ -162,6 +162,73 @@ __global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x
 		y[i] = x[i];
 }
 
+// Step-indexed kernels of the captured Lanczos step: the current step j is read from device
+// memory (jdev), so one captured graph serves every step of its ring phase.
+
+// base[*jdev + offset] = sum(partial[0..np)).
+__global__ __launch_bounds__(kRedThreads) void reduce_final_at_kernel(const double* __restrict__ partial,
+                                                                      int np, double* __restrict__ base,
+                                                                      const int* __restrict__ jdev,
+                                                                      int offset)
+{
+	double s = 0;
+	for (int i = threadIdx.x; i < np; i += blockDim.x)
+		s += partial[i];
+	s = block_sum(s);
+	if (threadIdx.x == 0)
+		base[*jdev + offset] = s;
+}
+
+// v2 = v2 - A[j] v1 - B[j] v0, partial sums of |v2|^2.
+__global__ __launch_bounds__(kRedThreads) void lanczos_update_at_kernel(
+    double* __restrict__ v2, const double* __restrict__ v1, const double* __restrict__ v0,
+    const double* __restrict__ A, const double* __restrict__ B, const int* __restrict__ jdev,
+    int64_t n, double* __restrict__ partial)
+{
+	const int    j      = *jdev;
+	const double a      = A[j];
+	const double b      = B[j];
+	double       s      = 0;
+	int64_t      stride = int64_t(gridDim.x) * blockDim.x;
+	for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const double v = v2[i] - a * v1[i] - b * v0[i];
+		v2[i]          = v;
+		s += v * v;
+	}
+	s = block_sum(s);
+	if (threadIdx.x == 0)
+		partial[blockIdx.x] = s;
+}
+
+// v2 *= nrm[1] and basis column j+1 = v2.
+__global__ void finish_step_kernel(double* __restrict__ v2, double* __restrict__ basis, int64_t n,
+                                   const double* __restrict__ nrm, const int* __restrict__ jdev)
+{
+	const double  inv    = nrm[1];
+	double*       col    = basis + int64_t(*jdev + 1) * n;
+	int64_t       stride = int64_t(gridDim.x) * blockDim.x;
+	for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const double v = v2[i] * inv;
+		v2[i]          = v;
+		col[i]         = v;
+	}
+}
+
+// report = (A[j], |v2|); B[j+1] = |v2| when j+1 < maxn; j += 1.
+__global__ void step_scalars_kernel(const double* __restrict__ A, double* __restrict__ B,
+                                    const double* __restrict__ nrm, int* __restrict__ jdev,
+                                    int maxn, double* __restrict__ report)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		const int j = *jdev;
+		report[0]   = A[j];
+		report[1]   = nrm[0];
+		if (j + 1 < maxn)
+			B[j + 1] = nrm[0];
+		*jdev = j + 1;
+	}
+}
+
 // Host launchers ------------------------------------------------------------------------------
 
 namespace {
@@ -225,6 +292,38 @@ void launch_residual(const double* hx, const double* x, double e, int64_t n, dou
 	                   partial);
 	hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, kRedBlocks,
 	                   out, 1);
+}
+
+void launch_dot_at(const double* x, const double* y, int64_t n, double* partial, double* base,
+                   const int* jdev, int offset, hipStream_t s)
+{
+	hipLaunchKernelGGL(dot_partial_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, x, y, n,
+	                   partial);
+	hipLaunchKernelGGL(reduce_final_at_kernel, dim3(1), dim3(kRedThreads), 0, s, partial,
+	                   kRedBlocks, base, jdev, offset);
+}
+
+void launch_lanczos_update_at(double* v2, const double* v1, const double* v0, const double* A,
+                              const double* B, const int* jdev, int64_t n, double* partial,
+                              double* norm_out, hipStream_t s)
+{
+	hipLaunchKernelGGL(lanczos_update_at_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, v2, v1,
+	                   v0, A, B, jdev, n, partial);
+	hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, kRedBlocks,
+	                   norm_out, 2);
+}
+
+void launch_finish_step(double* v2, double* basis, int64_t n, const double* nrm, const int* jdev,
+                        hipStream_t s)
+{
+	hipLaunchKernelGGL(finish_step_kernel, dim3(grid_for(n)), dim3(256), 0, s, v2, basis, n, nrm,
+	                   jdev);
+}
+
+void launch_step_scalars(const double* A, double* B, const double* nrm, int* jdev, int maxn,
+                         double* report, hipStream_t s)
+{
+	hipLaunchKernelGGL(step_scalars_kernel, dim3(1), dim3(64), 0, s, A, B, nrm, jdev, maxn, report);
 }
 
 void launch_copy(double* y, const double* x, int64_t n, hipStream_t s)

@@ -206,27 +206,31 @@ class GpuSolver:
     """Ground state of a Kron problem on the GPU (C-ABI plugin + device Lanczos)."""
 
     def __init__(self, precision="fp64", refine=1, max_steps=200, min_steps=4, tol=1e-12,
-                 seed=1234):
-        self.precision, self.refine = precision, refine
+                 seed=1234, graph=1):
+        self.precision, self.refine, self.graph = precision, refine, graph
         self.max_steps, self.min_steps, self.tol, self.seed = max_steps, min_steps, tol, seed
 
     def __call__(self, prob: KronProblem):
         from . import BatchedGemm, DeviceArray
 
         init = np.random.default_rng(self.seed).uniform(-1.0, 1.0, prob.n)
+        t0 = time.perf_counter()
         op = BatchedGemm(prob, "fp64")
         info = op.info()
+        t1 = time.perf_counter()
         dinit = DeviceArray.from_host(init)
         dz = DeviceArray(prob.n)
         r = op.lanczos_ground(dinit, dz, max_steps=self.max_steps, min_steps=self.min_steps,
                               tolerance=self.tol, precision=self.precision, refine=self.refine,
-                              refine_tolerance=self.tol)
+                              refine_tolerance=self.tol, graph=self.graph)
         z = dz.download()
+        t2 = time.perf_counter()
         op.close()
         dinit.free()
         dz.free()
         r["plan"] = {k: info[k] for k in ("nterms", "flops_algorithmic", "stage1_tiles",
                                           "stage2_tiles")}
+        r["timing"] = {"plan_ms": 1e3 * (t1 - t0), "lanczos_ms": 1e3 * (t2 - t1)}
         return r["energy"], z, r
 
 
@@ -268,15 +272,18 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
         l = lsize  # the left block's added site
         r = Ls - rsize - 1  # the right block's added site
         # Operators are kept for sites within the bond range of the block's open end.
+        tg = time.perf_counter()
         Lb = grow(S, sites[l], model, lattice, lambda s: l - s < W)
         Rb = grow(sites[r], E, model, lattice, lambda s: s - r < W)
         nsuper = lsize + rsize + 2
+        tk0 = time.perf_counter()
         target = target_of(lsize + rsize + 2)
         prob, patches = kron_problem(Lb, Rb, model, lattice, target,
                                      name=f"{model.name}-{phase}-{l}", offset=Ls - nsuper)
         t0 = time.perf_counter()
         e, z, rep = solver(prob)
         t1 = time.perf_counter()
+        timing = {"grow_ms": 1e3 * (tk0 - tg), "kron_ms": 1e3 * (t0 - tk0)}
         off = prob.offsets
         psi = [z[off[I]:off[I + 1]].reshape((prob.R[I], prob.L[I]), order="F")
                for I in range(prob.npatches)]
@@ -290,13 +297,20 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
             right[rsize + 1], d = truncate(Rb, psi, patches, "right", m, model)
             disc = max(disc, d)
         energy = e
+        timing["truncate_ms"] = 1e3 * (time.perf_counter() - t1)
+        rep = dict(rep)
+        timing.update(rep.get("timing", {}))
+        rep["timing"] = timing
         rec = StepRecord(phase, l, m, prob.n, prob.npatches, len(prob.blocks), e,
                          rep.get("steps", 0), rep.get("matvecs", 0), 1e3 * (t1 - t0),
                          rep.get("ms_matvec", 0.0), disc, rep)
         records.append(rec)
+        tm = rep.get("timing", {})
         log(f"{phase:8s} center={l:3d} m={m:5d} n={prob.n:9d} patches={prob.npatches:4d} "
             f"terms={len(prob.blocks):5d} E={e:.12f} steps={rec.lanczos_steps:3d} "
-            f"lanczos={rec.ms_lanczos:8.1f}ms disc={disc:.2e}")
+            f"solve={rec.ms_lanczos:7.1f}ms (plan {tm.get('plan_ms', 0):6.1f} lanczos "
+            f"{tm.get('lanczos_ms', 0):6.1f}) grow={tm['grow_ms']:6.1f} kron={tm['kron_ms']:6.1f} "
+            f"trunc={tm['truncate_ms']:6.1f} disc={disc:.2e}")
         return rec
 
     # Infinite-size growth: superblock of 2*lsize + 2 sites.

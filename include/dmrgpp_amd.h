@@ -147,6 +147,8 @@ typedef struct dmk_lanczos_params {
 	int32_t refine;           /* 1: after a non-FP64 run, restart in FP64 from its Ritz vector */
 	int32_t refine_min_steps;
 	double  refine_tolerance;
+	int32_t graph;            /* 1: each step is one replay of a captured HIP graph (default) */
+	int32_t pad;
 } dmk_lanczos_params;
 
 typedef struct dmk_lanczos_result {
