@@ -43,10 +43,12 @@ class DmkInfo(ctypes.Structure):
         ("flops_stage2", ctypes.c_double), ("bytes_compulsory", ctypes.c_double),
         ("bytes_bx", ctypes.c_double), ("precision", ctypes.c_int32), ("pad", ctypes.c_int32),
         ("split_tasks", ctypes.c_int64), ("ws_elems", ctypes.c_int64),
+        ("is_complex", ctypes.c_int32), ("pad2", ctypes.c_int32), ("n_complex", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+        return {name: getattr(self, name) for name, _ in self._fields_
+                if name not in ("pad", "pad2")}
 
 
 class Tuning(ctypes.Structure):
@@ -78,6 +80,32 @@ class LanczosResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+KRYLOV_STOP = {"energy": 0, "expm": 1}
+
+
+class KrylovParams(ctypes.Structure):
+    _fields_ = [("max_steps", ctypes.c_int32), ("min_steps", ctypes.c_int32),
+                ("tolerance", ctypes.c_double), ("stop", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("time_direction", ctypes.c_double),
+                ("energy_shift", ctypes.c_double)]
+
+
+class KrylovResult(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int32), ("breakdown", ctypes.c_int32),
+                ("norm", ctypes.c_double), ("error_estimate", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class KrylovStats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_matvec", ctypes.c_double),
+                ("matvec_launches", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 _lib = None
 
 # Every symbol include/dmrgpp_amd.h declares, with its ctypes signature.
@@ -87,7 +115,9 @@ _I = ctypes.c_int
 _D = ctypes.c_double
 SIGNATURES = {
     "dmk_create_d": (_I, [_SZ, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(_VP)]),
+    "dmk_create_z": (_I, [_SZ, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(_VP)]),
     "dmk_destroy": (None, [_VP]),
+    "dmk_apply_z": (_I, [_VP, _VP, _VP]),
     "dmk_apply_d": (_I, [_VP, _VP, _VP]),
     "dmk_apply_host_d": (_I, [_VP, _VP, _VP]),
     "dmk_apply_multi_d": (_I, [_VP, _I, _VP, _SZ, _VP, _SZ, _I]),
@@ -103,6 +133,10 @@ SIGNATURES = {
     "dmk_lanczos_ground_d": (_I, [_VP, _VP, _VP, ctypes.POINTER(LanczosParams),
                                   ctypes.POINTER(LanczosResult)]),
     "dmk_tridiag_lowest": (_I, [_I, _VP, _VP, ctypes.POINTER(_D), _VP]),
+    "dmk_tridiag_eigen": (_I, [_I, _VP, _VP, _VP, _VP]),
+    "dmk_krylov_default_params": (_I, [ctypes.POINTER(KrylovParams)]),
+    "dmk_krylov_evolve_z": (_I, [_VP, _I, _VP, _SZ, _I, _VP, _VP, _SZ,
+                                 ctypes.POINTER(KrylovParams), _VP, ctypes.POINTER(KrylovStats)]),
     "dmk_device_alloc": (_I, [_SZ, ctypes.POINTER(_VP)]),
     "dmk_device_free": (_I, [_VP]),
     "dmk_memcpy_h2d": (_I, [_VP, _VP, _SZ]),
@@ -152,6 +186,32 @@ def tridiag_lowest(a, b):
     _check(lib().dmk_tridiag_lowest(m, a.ctypes.data, b.ctypes.data, ctypes.byref(ev),
                                     vec.ctypes.data), "dmk_tridiag_lowest")
     return ev.value, vec
+
+
+def tridiag_eigen(a, b):
+    """All eigenpairs (ascending) of the Lanczos tridiagonal matrix (diagonal a, off-diagonal
+    b[1:]); eigenvectors are the columns of the returned matrix."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m = a.size
+    assert b.size >= m
+    ev = np.zeros(m)
+    vec = np.zeros(m * m)
+    _check(lib().dmk_tridiag_eigen(m, a.ctypes.data, b.ctypes.data, ev.ctypes.data,
+                                   vec.ctypes.data), "dmk_tridiag_eigen")
+    return ev, vec.reshape((m, m), order="F")
+
+
+def krylov_params(**kw) -> KrylovParams:
+    p = KrylovParams()
+    _check(lib().dmk_krylov_default_params(ctypes.byref(p)), "dmk_krylov_default_params")
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = PRECISIONS[v]
+        if k == "stop" and isinstance(v, str):
+            v = KRYLOV_STOP[v]
+        setattr(p, k, v)
+    return p
 
 
 def lanczos_params(**kw) -> LanczosParams:
@@ -206,17 +266,24 @@ class DeviceArray:
 
 
 class BatchedGemm:
-    """Mirror of ``BatchedGemm<double>`` (dmrg/GPUPlugin/BatchedGemm.h:11-56) over the C ABI."""
+    """Mirror of ``BatchedGemm<T>`` (dmrg/GPUPlugin/BatchedGemm.h:11-56) over the C ABI, for
+    T = double (dmk_create_d) and T = std::complex<double> (dmk_create_z, chosen by the
+    problem's dtype)."""
 
     def __init__(self, prob: KronProblem, precision: str | int = "fp64"):
-        if prob.dtype != np.float64:
-            raise DmkError("complex problems are not supported by the GPU path yet")
+        if prob.dtype == np.float64:
+            create, self.is_complex = lib().dmk_create_d, False
+        elif prob.dtype == np.complex128:
+            create, self.is_complex = lib().dmk_create_z, True
+        else:
+            raise DmkError(f"unsupported scalar type {prob.dtype}")
         args = prob.plugin_args()
         h = ctypes.c_void_p()
-        _check(lib().dmk_create_d(args.noperator, args.npatches, ctypes.addressof(args.L),
-                                  ctypes.addressof(args.R), ctypes.addressof(args.A),
-                                  ctypes.addressof(args.ldA), ctypes.addressof(args.B),
-                                  ctypes.addressof(args.ldB), ctypes.byref(h)), "dmk_create_d")
+        _check(create(args.noperator, args.npatches, ctypes.addressof(args.L),
+                      ctypes.addressof(args.R), ctypes.addressof(args.A),
+                      ctypes.addressof(args.ldA), ctypes.addressof(args.B),
+                      ctypes.addressof(args.ldB), ctypes.byref(h)),
+               "dmk_create_z" if self.is_complex else "dmk_create_d")
         self.handle = h.value
         self.n = prob.n
         self.set_precision(precision)
@@ -229,6 +296,25 @@ class BatchedGemm:
     def apply_Htarget(self, vin: DeviceArray, vout: DeviceArray):
         """vout = H vin on device arrays (BatchedGemm.h:30-31)."""
         _check(lib().dmk_apply_d(self.handle, vin.ptr, vout.ptr), "dmk_apply_d")
+
+    def apply_z(self, vin: DeviceArray, vout: DeviceArray):
+        """vout = H vin on interleaved complex device vectors (2n doubles each)."""
+        _check(lib().dmk_apply_z(self.handle, vin.ptr, vout.ptr), "dmk_apply_z")
+
+    def krylov_evolve(self, phi: DeviceArray, out: DeviceArray, nvec: int, times,
+                      ld: int | None = None, **kw):
+        """out[t, s] = exp(-i (H - E) tau_s d) phi[t] for nvec interleaved complex vectors
+        (TimeVectorsKrylov::calcTimeVectors); returns per-target results and call stats."""
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        ld = self.n if ld is None else int(ld)
+        p = krylov_params(**kw)
+        res = (KrylovResult * nvec)()
+        st = KrylovStats()
+        _check(lib().dmk_krylov_evolve_z(self.handle, nvec, phi.ptr, ld, times.size,
+                                         times.ctypes.data, out.ptr, ld, ctypes.byref(p),
+                                         ctypes.addressof(res), ctypes.byref(st)),
+               "dmk_krylov_evolve_z")
+        return [r.as_dict() for r in res], st.as_dict()
 
     def apply_multi(self, vin: DeviceArray, vout: DeviceArray, nvec: int, beta: int = 0):
         _check(lib().dmk_apply_multi_d(self.handle, nvec, vin.ptr, self.n, vout.ptr, self.n, beta),

@@ -1,6 +1,8 @@
 // extern "C" layer of dmrgpp_amd (declarations and reference citations in include/dmrgpp_amd.h).
+#include "complex_embed.h"
 #include "dmrgpp_amd.h"
 #include "kron_plan.h"
+#include "krylov.h"
 #include "lanczos.h"
 #include "pool.h"
 #include "tridiag.h"
@@ -16,6 +18,16 @@
 struct dmk_handle {
 	std::unique_ptr<dmk::Plan>             plan;
 	std::unique_ptr<dmk::LanczosWorkspace> lanczos;
+	std::unique_ptr<dmk::KrylovWorkspace>  krylov;
+	bool                                   is_complex = false; // plan is a real embedding
+	int64_t                                nc         = 0;     // complex vector length
+	double*                                zbuf       = nullptr; // split in / out of dmk_apply_z
+	~dmk_handle()
+	{
+		if (zbuf && plan)
+			(void)hipStreamSynchronize(plan->stream());
+		dmk::pool_device_free(zbuf);
+	}
 };
 
 namespace dmk {
@@ -79,7 +91,30 @@ int dmk_create_d(size_t noperator, size_t npatches, const size_t* left_patch_siz
 		h->plan = std::make_unique<dmk::Plan>(int64_t(noperator), int64_t(npatches),
 		                                      left_patch_size, right_patch_size, Amatrix,
 		                                      ld_Amatrix, Bmatrix, ld_Bmatrix);
+		h->nc   = h->plan->n();
 		*handle = h.release();
+		return DMK_OK;
+	});
+}
+
+int dmk_create_z(size_t noperator, size_t npatches, const size_t* left_patch_size,
+                 const size_t* right_patch_size, const double* const* Amatrix,
+                 const size_t* ld_Amatrix, const double* const* Bmatrix, const size_t* ld_Bmatrix,
+                 dmk_handle** handle)
+{
+	return guarded([&] {
+		if (!handle)
+			throw std::invalid_argument("dmk_create_z: null handle pointer");
+		*handle = nullptr;
+		dmk::RealEmbedding e = dmk::embed_complex(int64_t(noperator), int64_t(npatches),
+		                                          left_patch_size, right_patch_size, Amatrix,
+		                                          ld_Amatrix, Bmatrix, ld_Bmatrix);
+		auto h  = std::make_unique<dmk_handle>();
+		h->plan = std::make_unique<dmk::Plan>(e.nC, e.np, e.L.data(), e.R.data(), e.A.data(),
+		                                      e.ldA.data(), e.B.data(), e.ldB.data());
+		h->is_complex = true;
+		h->nc         = h->plan->n() / 2;
+		*handle       = h.release();
 		return DMK_OK;
 	});
 }
@@ -90,6 +125,8 @@ int dmk_apply_d(dmk_handle* handle, const double* vin, double* vout)
 {
 	return guarded([&] {
 		dmk::Plan& p = plan_of(handle);
+		if (handle->is_complex)
+			throw std::invalid_argument("dmk_apply_d: complex handle (use dmk_apply_z)");
 		p.apply(vin, p.n(), vout, p.n(), 1, 0);
 		return DMK_OK;
 	});
@@ -121,6 +158,28 @@ int dmk_apply_host_d(dmk_handle* handle, const double* vin, double* vout)
 	});
 }
 
+int dmk_apply_z(dmk_handle* handle, const double* vin, double* vout)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (!vin || !vout)
+			throw std::invalid_argument("dmk_apply_z: null vector");
+		const int64_t nc = handle->nc;
+		if (!handle->zbuf)
+			handle->zbuf = static_cast<double*>(dmk::pool_device_alloc(size_t(4 * nc) * sizeof(double)));
+		double* xin  = handle->zbuf;
+		double* xout = handle->zbuf + 2 * nc;
+		dmk::launch_kry_deinterleave(vin, nc, xin, 2 * nc, nc, 1, p.stream());
+		dmk::KrylovOperator op;
+		op.plan     = &p;
+		op.embedded = handle->is_complex;
+		op.nc       = nc;
+		op.apply(xin, xout, 1);
+		dmk::launch_kry_interleave(xout, 2 * nc, vout, nc, nc, 1, p.stream());
+		return DMK_OK;
+	});
+}
+
 int dmk_apply_multi_d(dmk_handle* handle, int nvec, const double* vin, size_t ldin, double* vout,
                       size_t ldout, int beta)
 {
@@ -128,6 +187,8 @@ int dmk_apply_multi_d(dmk_handle* handle, int nvec, const double* vin, size_t ld
 		dmk::Plan& p = plan_of(handle);
 		if (beta != 0 && beta != 1)
 			throw std::invalid_argument("dmk_apply_multi_d: beta must be 0 or 1");
+		if (handle->is_complex)
+			throw std::invalid_argument("dmk_apply_multi_d: complex handle (use dmk_apply_z)");
 		p.apply(vin, int64_t(ldin), vout, int64_t(ldout), nvec, beta);
 		return DMK_OK;
 	});
@@ -172,6 +233,8 @@ int dmk_get_info(const dmk_handle* handle, dmk_info* info)
 		info->precision         = handle->plan->precision();
 		info->split_tasks       = s.split_tasks;
 		info->ws_elems          = s.ws_elems;
+		info->is_complex        = handle->is_complex ? 1 : 0;
+		info->n_complex         = handle->nc;
 		return DMK_OK;
 	});
 }
@@ -309,6 +372,79 @@ int dmk_tridiag_lowest(int m, const double* a, const double* b, double* eval, do
 		if (evec) {
 			std::vector<double> v = dmk::tridiag_eigenvector(a, b, m, *eval);
 			std::memcpy(evec, v.data(), size_t(m) * sizeof(double));
+		}
+		return DMK_OK;
+	});
+}
+
+int dmk_tridiag_eigen(int m, const double* a, const double* b, double* evals, double* evecs)
+{
+	return guarded([&] {
+		if (m < 1 || !a || !b || !evals || !evecs)
+			throw std::invalid_argument("dmk_tridiag_eigen: bad arguments");
+		dmk::tridiag_eigen_all(a, b, m, evals, evecs);
+		return DMK_OK;
+	});
+}
+
+int dmk_krylov_default_params(dmk_krylov_params* params)
+{
+	return guarded([&] {
+		if (!params)
+			throw std::invalid_argument("dmk_krylov_default_params: null pointer");
+		dmk::KrylovParams d;
+		std::memset(params, 0, sizeof(*params));
+		params->max_steps      = d.max_steps;
+		params->min_steps      = d.min_steps;
+		params->tolerance      = d.tolerance;
+		params->stop           = d.stop;
+		params->precision      = d.precision;
+		params->time_direction = d.time_direction;
+		params->energy_shift   = d.energy_shift;
+		return DMK_OK;
+	});
+}
+
+int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t ldphi, int ntimes,
+                        const double* times, double* out, size_t ldout,
+                        const dmk_krylov_params* params, dmk_krylov_result* results,
+                        dmk_krylov_stats* stats)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (!params)
+			throw std::invalid_argument("dmk_krylov_evolve_z: null params");
+		dmk::KrylovParams kp;
+		kp.max_steps      = params->max_steps;
+		kp.min_steps      = params->min_steps;
+		kp.tolerance      = params->tolerance;
+		kp.stop           = params->stop;
+		kp.precision      = params->precision;
+		kp.time_direction = params->time_direction;
+		kp.energy_shift   = params->energy_shift;
+		dmk::KrylovOperator op;
+		op.plan     = &p;
+		op.embedded = handle->is_complex;
+		op.nc       = handle->nc;
+		if (!handle->krylov)
+			handle->krylov = std::make_unique<dmk::KrylovWorkspace>();
+		double ms_total = 0, ms_mv = 0;
+		int    launches = 0;
+		std::vector<dmk::KrylovReport> reps = dmk::krylov_evolve(
+		    op, *handle->krylov, nvec, phi, int64_t(ldphi), ntimes, times, out, int64_t(ldout), kp,
+		    &ms_total, &ms_mv, &launches);
+		if (results)
+			for (int t = 0; t < nvec; ++t) {
+				results[t].steps          = reps[size_t(t)].steps;
+				results[t].breakdown      = reps[size_t(t)].breakdown;
+				results[t].norm           = reps[size_t(t)].norm;
+				results[t].error_estimate = reps[size_t(t)].error_estimate;
+			}
+		if (stats) {
+			std::memset(stats, 0, sizeof(*stats));
+			stats->ms_total        = ms_total;
+			stats->ms_matvec       = ms_mv;
+			stats->matvec_launches = launches;
 		}
 		return DMK_OK;
 	});

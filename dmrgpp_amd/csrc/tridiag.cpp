@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <stdexcept>
 
 namespace dmk {
 
@@ -136,6 +137,86 @@ std::vector<double> tridiag_eigenvector(const double* a, const double* b, int m,
 		rhs = x;
 	}
 	return x;
+}
+
+void tridiag_eigen_all(const double* a, const double* b, int m, double* evals, double* evecs)
+{
+	if (m < 1)
+		return;
+	std::vector<double> d(a, a + m);
+	// e[i] couples rows i and i+1; e[m-1] = 0.
+	std::vector<double> e(static_cast<size_t>(m), 0.0);
+	for (int i = 0; i + 1 < m; ++i)
+		e[static_cast<size_t>(i)] = b[i + 1];
+	std::vector<double> z(static_cast<size_t>(m) * static_cast<size_t>(m), 0.0);
+	for (int i = 0; i < m; ++i)
+		z[static_cast<size_t>(i) * static_cast<size_t>(m + 1)] = 1.0;
+	auto Z = [&](int row, int col) -> double& {
+		return z[static_cast<size_t>(row) + static_cast<size_t>(col) * static_cast<size_t>(m)];
+	};
+	for (int l = 0; l < m; ++l) {
+		for (int iter = 0;; ++iter) {
+			// Smallest mm >= l with a negligible coupling below it.
+			int mm = l;
+			for (; mm + 1 < m; ++mm) {
+				const double dd = std::fabs(d[size_t(mm)]) + std::fabs(d[size_t(mm + 1)]);
+				if (std::fabs(e[size_t(mm)]) <= kEps * dd)
+					break;
+			}
+			if (mm == l)
+				break;
+			if (iter >= 100)
+				throw std::runtime_error("tridiag_eigen_all: no convergence");
+			// Wilkinson shift from the leading 2x2 block of the unreduced part.
+			double g = (d[size_t(l + 1)] - d[size_t(l)]) / (2.0 * e[size_t(l)]);
+			double r = std::hypot(g, 1.0);
+			g        = d[size_t(mm)] - d[size_t(l)] + e[size_t(l)] / (g + std::copysign(r, g));
+			double s = 1.0, c = 1.0, p = 0.0;
+			int    i        = mm - 1;
+			bool   deflated = false;
+			for (; i >= l; --i) {
+				const double f  = s * e[size_t(i)];
+				const double bb = c * e[size_t(i)];
+				r               = std::hypot(f, g);
+				e[size_t(i + 1)] = r;
+				if (r == 0.0) {
+					// Underflow: the bulge vanished, split here.
+					d[size_t(i + 1)] -= p;
+					e[size_t(mm)] = 0.0;
+					deflated      = true;
+					break;
+				}
+				s = f / r;
+				c = g / r;
+				g = d[size_t(i + 1)] - p;
+				r = (d[size_t(i)] - g) * s + 2.0 * c * bb;
+				p = s * r;
+				d[size_t(i + 1)] = g + p;
+				g                = c * r - bb;
+				for (int k = 0; k < m; ++k) {
+					const double zf = Z(k, i + 1);
+					Z(k, i + 1)     = s * Z(k, i) + c * zf;
+					Z(k, i)         = c * Z(k, i) - s * zf;
+				}
+			}
+			if (deflated)
+				continue;
+			d[size_t(l)] -= p;
+			e[size_t(l)]  = g;
+			e[size_t(mm)] = 0.0;
+		}
+	}
+	// Ascending order.
+	std::vector<int> order(static_cast<size_t>(m));
+	for (int i = 0; i < m; ++i)
+		order[size_t(i)] = i;
+	std::sort(order.begin(), order.end(), [&](int x, int y) { return d[size_t(x)] < d[size_t(y)]; });
+	for (int k = 0; k < m; ++k) {
+		const int src = order[size_t(k)];
+		evals[k]      = d[size_t(src)];
+		for (int row = 0; row < m; ++row)
+			evecs[size_t(row) + size_t(k) * size_t(m)] = Z(row, src);
+	}
 }
 
 } // namespace dmk

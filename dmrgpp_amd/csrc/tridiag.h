@@ -19,4 +19,10 @@ double tridiag_eigenvalue(const double* a, const double* b, int m, int k);
 // partially pivoted LU of T - lambda I.
 std::vector<double> tridiag_eigenvector(const double* a, const double* b, int m, double lambda);
 
+// All eigenpairs of T, ascending: evals[k], and evecs column-major m x m (column k is the unit
+// eigenvector of evals[k]). Implicit QL iteration with Wilkinson shifts (the EISPACK tql2
+// method), rotations accumulated into the eigenvector matrix; the Krylov propagator needs the
+// whole spectrum of T (PsimagLite::diag(T, eigs, 'V') in TimeVectorsKrylov.h:224-225).
+void tridiag_eigen_all(const double* a, const double* b, int m, double* evals, double* evecs);
+
 } // namespace dmk

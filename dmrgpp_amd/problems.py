@@ -165,3 +165,16 @@ def random_problem(L, R, nC: int, density: float = 0.5, seed: int = 0,
                         b = b + 1j * rng.uniform(-1, 1, size=b.shape)
                     blocks[(I, J, k)] = (a, b)
     return KronProblem(L=L, R=R, nC=nC, blocks=blocks, dtype=dtype, name="random")
+
+
+def hermitian_problem(L, R, nC: int, density: float = 0.5, seed: int = 0,
+                      dtype=np.complex128) -> KronProblem:
+    """Random Hermitian (real symmetric for a real dtype) problem: every random term
+    A_k(I,J) (x) B_k(I,J) comes with its adjoint A^H (x) B^H at (J, I, k + nC), since the (J, I)
+    block of H^H is the adjoint of the (I, J) block."""
+    base = random_problem(L, R, nC, density=density, seed=seed, dtype=dtype)
+    blocks = dict(base.blocks)
+    for (I, J, k), (a, b) in base.blocks.items():
+        blocks[(J, I, k + nC)] = (np.conj(a.T), np.conj(b.T))
+    return KronProblem(L=base.L, R=base.R, nC=2 * nC, blocks=blocks, dtype=dtype,
+                       name="hermitian")

@@ -68,6 +68,9 @@ typedef struct dmk_info {
 	int32_t pad;
 	int64_t split_tasks;       /* stage-2 tasks cut into split-k pieces under the current tuning */
 	int64_t ws_elems;          /* split-k partial workspace (elements per vector) */
+	int32_t is_complex;        /* 1: created by dmk_create_z (n above is 2 n_complex) */
+	int32_t pad2;
+	int64_t n_complex;         /* complex vector length (equals n for a real handle) */
 } dmk_info;
 
 /* Launch shape of a handle: output tile (64 or 128 each way) of the stage-1 and stage-2
@@ -90,6 +93,16 @@ int dmk_create_d(size_t noperator, size_t npatches, const size_t* left_patch_siz
                  const size_t* ld_Amatrix, const double* const* Bmatrix, const size_t* ld_Bmatrix,
                  dmk_handle** handle);
 
+/* Replaces BatchedGemm<std::complex<double>>::BatchedGemm (the same constructor,
+ * BatchedGemm.h:21-28, instantiated at BatchedGemm.cpp:119). Blocks are std::complex<double>
+ * column-major host arrays passed as interleaved (re, im) doubles, leading dimensions in complex
+ * elements. The handle holds the real embedding of H on 2n reals (real parts of all patches,
+ * then imaginary parts); blocks with zero imaginary parts cost nothing extra. */
+int dmk_create_z(size_t noperator, size_t npatches, const size_t* left_patch_size,
+                 const size_t* right_patch_size, const double* const* Amatrix,
+                 const size_t* ld_Amatrix, const double* const* Bmatrix, const size_t* ld_Bmatrix,
+                 dmk_handle** handle);
+
 /* Replaces BatchedGemm<T>::~BatchedGemm — BatchedGemm.h:33, BatchedGemm.cpp:72-79. */
 void dmk_destroy(dmk_handle* handle);
 
@@ -101,6 +114,11 @@ int dmk_apply_d(dmk_handle* handle, const double* vin, double* vout);
 /* Same on host vectors (the BatchedGemmPluginSc::matrixVector round trip,
  * BatchedGemmPluginSc.h:139-161): copies vin in, applies, copies vout out, synchronous. */
 int dmk_apply_host_d(dmk_handle* handle, const double* vin, double* vout);
+
+/* Replaces BatchedGemm<std::complex<double>>::apply_Htarget (BatchedGemm.h:30-31): vout = H vin
+ * on interleaved complex device vectors of n_complex elements. Also valid on a real handle
+ * (real H times a complex vector: real and imaginary parts go through one two-column launch). */
+int dmk_apply_z(dmk_handle* handle, const double* vin, double* vout);
 
 /* nvec right-hand sides in one launch pair: vout[v*ldout..] = H vin[v*ldin..] (beta = 0) or
  * += (beta = 1). No reference counterpart (ParallelTriDiag.h:132-161 tridiagonalizes one vector
@@ -180,6 +198,58 @@ int dmk_lanczos_ground_d(dmk_handle* handle, const double* init, double* z,
  * lays it out (TridiagonalMatrix.h:149-163); replaces its LAPACK dsyev call (Matrix.cpp:15-45)
  * for the ground state. evec may be NULL. Host arrays. */
 int dmk_tridiag_lowest(int m, const double* a, const double* b, double* eval, double* evec);
+
+/* All eigenpairs of the same tridiagonal matrix, ascending: evals[0..m), evecs column major
+ * m x m (column k belongs to evals[k]). Replaces PsimagLite::diag(T, eigs, 'V') of the Krylov
+ * step (TimeVectorsKrylov.h:224-225). Host arrays. */
+int dmk_tridiag_eigen(int m, const double* a, const double* b, double* evals, double* evecs);
+
+/* Krylov time step. Defaults (dmk_krylov_default_params) are the "Tridiag" solver defaults of
+ * ParametersForSolver (ParametersForSolver.h:97-128) used by ParallelTriDiag.h:146-147. */
+enum dmk_krylov_stop
+{
+	DMK_KRYLOV_STOP_ENERGY = 0, /* lowest Ritz value moved less than tolerance (the reference) */
+	DMK_KRYLOV_STOP_EXPM   = 1, /* beta_m |[exp(-i T tau) e_0]_{m-1}| < tolerance (relative) */
+};
+
+typedef struct dmk_krylov_params {
+	int32_t max_steps;
+	int32_t min_steps;
+	double  tolerance;
+	int32_t stop;            /* enum dmk_krylov_stop */
+	int32_t precision;       /* matvec precision (enum dmk_precision); recurrence in FP64 */
+	double  time_direction;  /* d in exp(-i (H - E) tau d), TimeVectorsKrylov.h:287 */
+	double  energy_shift;    /* E (the reference passes E_g), TimeVectorsKrylov.h:291 */
+} dmk_krylov_params;
+
+typedef struct dmk_krylov_result {
+	int32_t steps;           /* Krylov dimension of this target */
+	int32_t breakdown;       /* 1: an invariant subspace was reached */
+	double  norm;            /* |phi| */
+	double  error_estimate;  /* beta_m |[exp(-i T tau_max) e_0]_{m-1}| |phi| */
+} dmk_krylov_result;
+
+typedef struct dmk_krylov_stats {
+	double  ms_total;        /* wall time of the call */
+	double  ms_matvec;       /* device time inside the batched matvec launches */
+	int32_t matvec_launches; /* batched matvecs (one per Krylov step, all targets together) */
+	int32_t pad;
+} dmk_krylov_stats;
+
+int dmk_krylov_default_params(dmk_krylov_params* params);
+
+/* out_{t,s} = exp(-i (H - E) tau_s d) phi_t for nvec target vectors phi_t and ntimes (<= 8)
+ * times tau_s: the Krylov time step of TimeVectorsKrylov::calcTimeVectors
+ * (TimeVectorsKrylov.h:165-322) over ParallelTriDiag::triDiag (ParallelTriDiag.h:132-161) and
+ * KrylovHelper::calcR (KrylovHelper.h, default abridgement). The reference tridiagonalizes one
+ * vector per call; here all nvec targets share one batched matvec launch per Krylov step, each
+ * keeping the Krylov dimension its own stop test gives. phi: interleaved complex device vectors
+ * at ldphi complex elements; out: target t, time s at out + 2*ldout*(t*ntimes + s); times on the
+ * host. results (nvec entries) and stats may be NULL. Works on real and complex handles. */
+int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t ldphi, int ntimes,
+                        const double* times, double* out, size_t ldout,
+                        const dmk_krylov_params* params, dmk_krylov_result* results,
+                        dmk_krylov_stats* stats);
 
 /* Device memory helpers for callers without their own allocator. */
 int dmk_device_alloc(size_t bytes, void** ptr);

@@ -17,6 +17,9 @@
 //                               LanczosVectors::oneStepDecomposition / reorthoIfNecessary / excitedVector
 //                                                                  PsimagLite/LanczosVectors.h:192-271
 //                               LanczosSolver::computeAllStatesBelow PsimagLite/LanczosSolver.h:68-88
+//   orc_krylov_z                TimeVectorsKrylov::calcTimeVectors Engine/TimeVectorsKrylov.h:165-322
+//                               ParallelTriDiag::triDiag           Engine/ParallelTriDiag.h:132-161
+//                               KrylovHelper::calcR (abridged)     Engine/KrylovHelper.h
 //   orc_tridiag_eigen           eigen-decomposition of the Lanczos tridiagonal matrix
 //                               (TridiagonalMatrix.h:149-200 builds it; the reference hands it to LAPACK
 //                               dsyev, Matrix.cpp:15-45). Implemented here as implicit-shift QL.
@@ -24,7 +27,9 @@
 // Pinning (tests/test_oracle.py): the KronConnections fixture values of test_KronConnections.cpp,
 // the mt19937_64 fixture of test_BatchedGemm.cpp, the dumped Kron problem kron_data31_0_ref.txt
 // checked against brute-force assembly (orc_kron_form) and Hermiticity, closed-form spectra
-// (free fermions, ED) for the Lanczos restatement, numpy.linalg.eigh for the tridiagonal solver.
+// (free fermions, ED) for the Lanczos restatement, numpy.linalg.eigh for the tridiagonal solver,
+// the exact exponential of the assembled H (full Krylov space), unitarity and time reversal for
+// the Krylov step.
 //
 // Storage conventions follow the reference: every matrix is column major (PsimagLite::Matrix,
 // Matrix.h:250-262); the vector segment of patch I is an R_I x L_I column-major matrix
@@ -445,7 +450,7 @@ template <typename T> double dotre(size_t n, const T* x, const T* y)
 template <typename T>
 int lanczos(size_t n, const std::function<void(T*, const T*)>& matvec, const T* init, int maxSteps,
             int minSteps, double tol, int reortho, double* energy, T* z, double* alpha,
-            double* beta, int* stepsOut)
+            double* beta, int* stepsOut, std::vector<T>* keepV = nullptr)
 {
 	size_t max_nstep = size_t(maxSteps);
 	if (max_nstep > n)
@@ -568,6 +573,54 @@ int lanczos(size_t n, const std::function<void(T*, const T*)>& matvec, const T* 
 			beta[i] = b[i];
 	}
 	*stepsOut = int(nsteps);
+	if (keepV) // lanczosVectorsSwap (ParallelTriDiag.h:158): V is n x nsteps, column major
+		keepV->assign(data.begin(), data.begin() + long(nsteps * n));
+	return 0;
+}
+
+// TimeVectorsKrylov::calcTargetVector (TimeVectorsKrylov.h:263-322) for one sector: T = S eps S^T
+// from the decomposition of phi, r[k] = conj(S(0,k)) <V_0|phi> exp(-i (eps_k - Eg) t d)
+// (KrylovHelper::calcR with abridgement: only kprime = 0), tmp = S r, out = V tmp.
+int krylov_times(size_t n, const std::function<void(cplx*, const cplx*)>& matvec, const cplx* phi,
+                 int ntimes, const double* times, double Eg, double timeDirection, int maxSteps,
+                 int minSteps, double tol, cplx* out, int* stepsOut)
+{
+	std::vector<cplx>   z(n), V;
+	double              energy = 0;
+	int                 steps  = 0;
+	std::vector<double> a(size_t(std::max(1, maxSteps))), b(size_t(std::max(1, maxSteps)));
+	int rc = lanczos<cplx>(n, matvec, phi, maxSteps, minSteps, tol, 0, &energy, z.data(), a.data(),
+	                       b.data(), &steps, &V);
+	if (rc != 0)
+		return rc;
+	const size_t        n2 = size_t(steps);
+	std::vector<double> eps(a.begin(), a.begin() + long(n2)), ee(n2, 0.0), S(n2 * n2);
+	for (size_t i = 0; i + 1 < n2; ++i)
+		ee[i] = b[i + 1];
+	if (tridiag_ql(int(n2), eps.data(), ee.data(), S.data()) != 0)
+		return -3;
+	// calcVTimesPhi(0): sum_j conj(V(j,0)) phi_j.
+	cplx vphi = 0;
+	for (size_t j = 0; j < n; ++j)
+		vphi += std::conj(V[j]) * phi[j];
+	for (int q = 0; q < ntimes; ++q) {
+		std::vector<cplx> r(n2), tmp(n2, cplx(0));
+		for (size_t k = 0; k < n2; ++k) {
+			const double arg = (eps[k] - Eg) * times[q] * timeDirection;
+			r[k]             = S[k * n2] * vphi * std::exp(cplx(0, -arg)); // conj(T(0,k)), real S
+		}
+		for (size_t i = 0; i < n2; ++i)
+			for (size_t k = 0; k < n2; ++k)
+				tmp[i] += S[i + k * n2] * r[k];
+		cplx* o = out + size_t(q) * n;
+		for (size_t i = 0; i < n; ++i) {
+			cplx acc = 0;
+			for (size_t k = 0; k < n2; ++k)
+				acc += V[i + k * n] * tmp[k];
+			o[i] = acc;
+		}
+	}
+	*stepsOut = steps;
 	return 0;
 }
 
@@ -715,6 +768,27 @@ int orc_lanczos_z(int64_t nC, int64_t np, const size_t* L, const size_t* R, cons
 	};
 	return lanczos<cplx>(n, mv, reinterpret_cast<const cplx*>(init), maxSteps, minSteps, tol, reortho,
 	                     energy, reinterpret_cast<cplx*>(z), alpha, beta, steps);
+}
+
+// Krylov time step of one vector phi (interleaved complex, n) of H in the plugin layout:
+// out[q*n ..] = exp(-i (H - Eg) times[q] d) phi restated from the reference.
+int orc_krylov_z(int64_t nC, int64_t np, const size_t* L, const size_t* R, const double* const* A,
+                 const size_t* ldA, const double* const* B, const size_t* ldB, const double* phi,
+                 int ntimes, const double* times, double Eg, double timeDirection, int maxSteps,
+                 int minSteps, double tol, double* out, int* steps)
+{
+	PluginInput<cplx> in = make_input<cplx>(nC, np, L, R, reinterpret_cast<const cplx* const*>(A),
+	                                        ldA, reinterpret_cast<const cplx* const*>(B), ldB);
+	std::vector<size_t> off = patch_offsets(in);
+	const size_t        n   = off[size_t(np)];
+	std::vector<cplx>   tmp(n);
+	auto                mv = [&](cplx* x, const cplx* y) {
+                plugin_apply<cplx>(in, y, tmp.data());
+                for (size_t i = 0; i < n; ++i)
+                        x[i] += tmp[i];
+	};
+	return krylov_times(n, mv, reinterpret_cast<const cplx*>(phi), ntimes, times, Eg, timeDirection,
+	                    maxSteps, minSteps, tol, reinterpret_cast<cplx*>(out), steps);
 }
 
 } // extern "C"

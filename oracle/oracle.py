@@ -50,6 +50,10 @@ def lib():
                           ctypes.c_int64, _VP, ctypes.c_int64, ctypes.c_int64, _VP, _VP]
         L.orc_estimate_kron_method.restype = ctypes.c_int
         L.orc_estimate_kron_method.argtypes = [ctypes.c_int64] * 6 + [ctypes.c_double]
+        L.orc_krylov_z.restype = ctypes.c_int
+        L.orc_krylov_z.argtypes = _ARGS + [_VP, ctypes.c_int, _VP, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, _VP, _VP]
         L.orc_tridiag_eigen.restype = ctypes.c_int
         L.orc_tridiag_eigen.argtypes = [ctypes.c_int, _VP, _VP, _VP]
         _lib = L
@@ -149,3 +153,26 @@ def lanczos(prob, init: np.ndarray, max_steps=200, min_steps=4, tol=1e-12, reort
         raise RuntimeError(f"oracle lanczos failed ({rc})")
     s = steps.value
     return energy.value, z, s, alpha[:s], beta[:s]
+
+
+def krylov(prob, phi: np.ndarray, times, Eg=0.0, time_direction=1.0, max_steps=200, min_steps=4,
+           tol=1e-12):
+    """Krylov time step of one vector (TimeVectorsKrylov::calcTimeVectors restated): returns
+    (out[ntimes, n] complex, steps). A real problem is promoted to complex blocks."""
+    if not np.issubdtype(prob.dtype, np.complexfloating):
+        from dmrgpp_amd.problem import KronProblem
+        prob = KronProblem(L=prob.L, R=prob.R, nC=prob.nC,
+                           blocks={k: (a.astype(np.complex128), b.astype(np.complex128))
+                                   for k, (a, b) in prob.blocks.items()},
+                           dtype=np.complex128, name=prob.name)
+    args = prob.plugin_args()
+    phi = np.ascontiguousarray(phi, dtype=np.complex128)
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    out = np.zeros((times.size, prob.n), dtype=np.complex128)
+    steps = ctypes.c_int()
+    rc = lib().orc_krylov_z(*_pargs(args), phi.ctypes.data, int(times.size), times.ctypes.data,
+                            float(Eg), float(time_direction), int(max_steps), int(min_steps),
+                            float(tol), out.ctypes.data, ctypes.addressof(steps))
+    if rc != 0:
+        raise RuntimeError(f"oracle krylov failed ({rc})")
+    return out, steps.value

@@ -5,6 +5,7 @@
 #include "dmrgpp_amd_batched_gemm.hpp"
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <random>
 #include <vector>
@@ -52,6 +53,28 @@ int main()
 		}
 	} catch (const std::exception& e) {
 		std::printf("exception: %s\n", e.what());
+		return 2;
+	}
+	// Complex: test_KronConnections.cpp:132-178, two 1x1 patches, one connection;
+	// H = [[3+i, 5i], [5-5i, -4]] and H (1+i, 2-i) = (7+14i, 2+4i).
+	try {
+		using cplx = std::complex<double>;
+		std::vector<cplx> a { cplx(1, 1), cplx(1, -2), cplx(2, 1), cplx(-1, 1) };
+		std::vector<cplx> b { cplx(2, -1), cplx(3, 1), cplx(1, 2), cplx(2, 2) };
+		std::vector<cplx*> Ap { &a[0], &a[1], &a[2], &a[3] }, Bp { &b[0], &b[1], &b[2], &b[3] };
+		std::vector<std::size_t> one2 { 1, 1 }, one4 { 1, 1, 1, 1 };
+		dmrgpp_amd::BatchedGemm<cplx> bz(1, 2, one2, one2, Ap, one4, Bp, one4);
+		std::vector<cplx> zin { cplx(1, 1), cplx(2, -1) }, zout(2);
+		bz.apply_Htarget_host(zin.data(), zout.data());
+		const cplx want[2] = { cplx(7, 14), cplx(2, 4) };
+		for (int i = 0; i < 2; i++)
+			if (std::abs(zout[i] - want[i]) > 1e-13) {
+				std::printf("complex mismatch at %d: got (%.17g, %.17g)\n", i, zout[i].real(),
+				            zout[i].imag());
+				++failures;
+			}
+	} catch (const std::exception& e) {
+		std::printf("exception (complex): %s\n", e.what());
 		return 2;
 	}
 	std::printf("test_batched_gemm_shim: %s\n", failures ? "FAILED" : "passed");

@@ -162,3 +162,45 @@ def test_plugin_matches_connections_on_random_problems():
         np.testing.assert_allclose(oracle.plugin_apply(prob, x), want, atol=1e-12)
         np.testing.assert_allclose(oracle.kron_connections(prob, x), want, atol=1e-12)
         np.testing.assert_allclose(prob.apply_numpy(x), want, atol=1e-12)
+
+
+def _expm_apply(H, phi, tau, Eg=0.0, d=1.0):
+    w, U = np.linalg.eigh(H)
+    return U @ (np.exp(-1j * (w - Eg) * tau * d) * (U.conj().T @ phi))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_krylov_oracle_full_space_is_exact(cplx):
+    # With tolerance 0 the recurrence runs to the full space (n steps) and the Krylov step is
+    # the exact exponential of the assembled H.
+    from dmrgpp_amd import problems
+    prob = problems.hermitian_problem([2, 3, 2], [3, 2, 2], nC=2, density=0.7, seed=5,
+                                      dtype=np.complex128 if cplx else np.float64)
+    H = oracle.kron_form(prob)
+    assert np.abs(H - H.conj().T).max() < 1e-12
+    rng = np.random.default_rng(9)
+    phi = rng.uniform(-1, 1, prob.n) + 1j * rng.uniform(-1, 1, prob.n)
+    times = [0.0, 0.05, 0.2, -0.3]
+    out, steps = oracle.krylov(prob, phi, times, Eg=0.7, time_direction=1.0, max_steps=200,
+                               min_steps=4, tol=0.0)
+    assert steps == prob.n
+    for q, t in enumerate(times):
+        want = _expm_apply(H, phi, t, Eg=0.7)
+        assert np.linalg.norm(out[q] - want) < 1e-10 * np.linalg.norm(phi)
+
+
+def test_krylov_oracle_unitary_and_time_reversal():
+    full, _, _, _ = fixtures.kron31_problem()
+    H = oracle.kron_form(full)
+    H = 0.5 * (H + H.T)
+    rng = np.random.default_rng(2)
+    phi = rng.uniform(-1, 1, full.n) + 1j * rng.uniform(-1, 1, full.n)
+    out, steps = oracle.krylov(full, phi, [0.1], Eg=-3.0)
+    assert 4 <= steps <= full.n
+    # The reference stop test (lowest Ritz value settled to 1e-12) leaves a Krylov space large
+    # enough that the short-time propagator is exact to roundoff here.
+    want = _expm_apply(H, phi, 0.1, Eg=-3.0)
+    assert np.linalg.norm(out[0] - want) < 1e-10 * np.linalg.norm(phi)
+    assert abs(np.linalg.norm(out[0]) - np.linalg.norm(phi)) < 1e-10 * np.linalg.norm(phi)
+    back, _ = oracle.krylov(full, out[0], [0.1], Eg=-3.0, time_direction=-1.0)
+    assert np.linalg.norm(back[0] - phi) < 1e-9 * np.linalg.norm(phi)
