@@ -134,6 +134,10 @@ SIGNATURES = {
                                   ctypes.POINTER(LanczosResult)]),
     "dmk_tridiag_lowest": (_I, [_I, _VP, _VP, ctypes.POINTER(_D), _VP]),
     "dmk_tridiag_eigen": (_I, [_I, _VP, _VP, _VP, _VP]),
+    "dmk_gemm_vbatch_d": (_I, [ctypes.c_char, ctypes.c_char, _VP, _VP, _VP, _D, _VP, _VP, _VP,
+                               _VP, _D, _VP, _VP, _SZ, _VP]),
+    "dmk_gemm_vbatch_z": (_I, [ctypes.c_char, ctypes.c_char, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
+                               _VP, _VP, _VP, _VP, _SZ, _VP]),
     "dmk_krylov_default_params": (_I, [ctypes.POINTER(KrylovParams)]),
     "dmk_krylov_evolve_z": (_I, [_VP, _I, _VP, _SZ, _I, _VP, _VP, _SZ,
                                  ctypes.POINTER(KrylovParams), _VP, ctypes.POINTER(KrylovStats)]),
@@ -200,6 +204,42 @@ def tridiag_eigen(a, b):
     _check(lib().dmk_tridiag_eigen(m, a.ctypes.data, b.ctypes.data, ev.ctypes.data,
                                    vec.ctypes.data), "dmk_tridiag_eigen")
     return ev, vec.reshape((m, m), order="F")
+
+
+def gemm_vbatch(transa: str, transb: str, groups, alpha=1.0, beta=0.0, cplx: bool = False):
+    """Variable-size batched GEMM on device matrices (dmrg_Xgemm_vbatch semantics):
+    groups is a list of (m, n, k, lda, ldb, ldc, [(a_ptr, b_ptr, c_ptr), ...]); pointers are
+    device addresses (ints). Complex matrices are interleaved; alpha/beta may be complex."""
+    G = len(groups)
+    ia = lambda vals: (ctypes.c_int * max(G, 1))(*vals)
+    m = ia([g[0] for g in groups])
+    n = ia([g[1] for g in groups])
+    k = ia([g[2] for g in groups])
+    lda = ia([g[3] for g in groups])
+    ldb = ia([g[4] for g in groups])
+    ldc = ia([g[5] for g in groups])
+    gs = ia([len(g[6]) for g in groups])
+    ents = [e for g in groups for e in g[6]]
+    nE = max(len(ents), 1)
+    A = (ctypes.c_void_p * nE)(*[e[0] for e in ents])
+    B = (ctypes.c_void_p * nE)(*[e[1] for e in ents])
+    C = (ctypes.c_void_p * nE)(*[e[2] for e in ents])
+    args_tail = [ctypes.addressof(ldc), G, ctypes.addressof(gs)]
+    if not cplx:
+        _check(lib().dmk_gemm_vbatch_d(transa.encode(), transb.encode(), ctypes.addressof(m),
+                                       ctypes.addressof(n), ctypes.addressof(k), float(alpha),
+                                       ctypes.addressof(A), ctypes.addressof(lda),
+                                       ctypes.addressof(B), ctypes.addressof(ldb), float(beta),
+                                       ctypes.addressof(C), *args_tail), "dmk_gemm_vbatch_d")
+    else:
+        al = (ctypes.c_double * 2)(complex(alpha).real, complex(alpha).imag)
+        be = (ctypes.c_double * 2)(complex(beta).real, complex(beta).imag)
+        _check(lib().dmk_gemm_vbatch_z(transa.encode(), transb.encode(), ctypes.addressof(m),
+                                       ctypes.addressof(n), ctypes.addressof(k),
+                                       ctypes.addressof(al), ctypes.addressof(A),
+                                       ctypes.addressof(lda), ctypes.addressof(B),
+                                       ctypes.addressof(ldb), ctypes.addressof(be),
+                                       ctypes.addressof(C), *args_tail), "dmk_gemm_vbatch_z")
 
 
 def krylov_params(**kw) -> KrylovParams:

@@ -1,7 +1,7 @@
 """The benchmark workloads as sweep-driver settings, and a command-line runner.
 
-  python -m dmrgpp_amd.configs --config heisenberg-chain-32 [--precision fp64|fp32] [--sweeps N]
-                               [--stop-after K] [--json out.json]
+  python -m dmrgpp_amd.configs --config heisenberg-chain-32 [--precision fp64|fp32|bf16x3]
+                               [--sweeps N] [--stop-after K] [--wft] [--json out.json]
 
 Each run prints one line per DMRG step and, with --json, writes the step records and the total
 sweep wall time.
@@ -40,14 +40,14 @@ CONFIGS = {
 
 
 def run_config(name: str, precision="fp64", m=None, sweeps=None, stop_after=None, log=print,
-               solver=None):
+               solver=None, wft=None):
     c = CONFIGS[name]
     m = m or c["m"]
     nsw = c["sweeps"] if sweeps is None else sweeps
     solver = solver or sweep.GpuSolver(precision=precision)
     t0 = time.perf_counter()
     e, recs = sweep.run(c["model"](), c["lattice"](), c["target"], m, [m] * nsw, solver, log=log,
-                        stop_after=stop_after)
+                        stop_after=stop_after, wft=wft)
     wall = time.perf_counter() - t0
     return e, recs, wall
 
@@ -59,10 +59,12 @@ def main(argv=None):
     ap.add_argument("--m", type=int, default=None)
     ap.add_argument("--sweeps", type=int, default=None)
     ap.add_argument("--stop-after", type=int, default=None)
+    ap.add_argument("--wft", action="store_true",
+                    help="start finite-sweep Lanczos runs from the transformed previous state")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     e, recs, wall = run_config(a.config, a.precision, a.m, a.sweeps, a.stop_after,
-                               log=lambda s: print(s, flush=True))
+                               log=lambda s: print(s, flush=True), wft=True if a.wft else None)
     sweep_steps = [r for r in recs if r.phase != "infinite"]
     out = {"config": a.config, "precision": a.precision, "energy": e, "wall_s": wall,
            "steps": len(recs), "matvecs": sum(r.matvecs for r in recs),

@@ -6,6 +6,7 @@
 #include "lanczos.h"
 #include "pool.h"
 #include "tridiag.h"
+#include "vbatch.h"
 
 #include <hip/hip_runtime.h>
 
@@ -446,6 +447,36 @@ int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t 
 			stats->ms_matvec       = ms_mv;
 			stats->matvec_launches = launches;
 		}
+		return DMK_OK;
+	});
+}
+
+int dmk_gemm_vbatch_d(char transa, char transb, const int* m_array, const int* n_array,
+                      const int* k_array, double alpha, const double* const* a_array,
+                      const int* lda_array, const double* const* b_array, const int* ldb_array,
+                      double beta, double* const* c_array, const int* ldc_array,
+                      size_t group_count, const int* group_size)
+{
+	return guarded([&] {
+		dmk::gemm_vbatch(false, transa, transb, m_array, n_array, k_array, &alpha, a_array,
+		                 lda_array, b_array, ldb_array, &beta, c_array, ldc_array, group_count,
+		                 group_size);
+		return DMK_OK;
+	});
+}
+
+int dmk_gemm_vbatch_z(char transa, char transb, const int* m_array, const int* n_array,
+                      const int* k_array, const double* alpha, const double* const* a_array,
+                      const int* lda_array, const double* const* b_array, const int* ldb_array,
+                      const double* beta, double* const* c_array, const int* ldc_array,
+                      size_t group_count, const int* group_size)
+{
+	return guarded([&] {
+		if (!alpha || !beta)
+			throw std::invalid_argument("dmk_gemm_vbatch_z: null alpha or beta");
+		dmk::gemm_vbatch(true, transa, transb, m_array, n_array, k_array, alpha, a_array,
+		                 lda_array, b_array, ldb_array, beta, c_array, ldc_array, group_count,
+		                 group_size);
 		return DMK_OK;
 	});
 }

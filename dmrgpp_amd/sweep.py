@@ -32,6 +32,8 @@ class Block:
     H: BSOp
     ops: dict       # (site, name) -> BSOp
     parity: np.ndarray
+    product: object = None  # grown blocks: the Product of the two blocks they were made of
+    wft: dict = None        # truncated blocks: {grown group: (V, (new group, kept))}
 
     @property
     def dim(self):
@@ -86,7 +88,7 @@ def grow(A: Block, B: Block, model: SiteModel, lattice: Lattice, keep) -> Block:
             ops[(s, name)] = P.kron(pA if op.fermionic else None, op)
             ops[(s, name)].fermionic = op.fermionic
     parity = np.array([_parity_of(model, q) for q in P.basis.qns])
-    return Block(sorted(A.sites + B.sites), P.basis, H, ops, parity)
+    return Block(sorted(A.sites + B.sites), P.basis, H, ops, parity, product=P)
 
 
 def kron_problem(Lb: Block, Rb: Block, model: SiteModel, lattice: Lattice, target: tuple,
@@ -182,7 +184,7 @@ def truncate(block: Block, psi_blocks: list, patches: list, side: str, m: int,
     for key, op in block.ops.items():
         ops[key] = op.transform(W, nb, W, nb)
     parity = np.array([_parity_of(model, q) for q in nb.qns])
-    return Block(block.sites, nb, H, ops, parity), discarded
+    return Block(block.sites, nb, H, ops, parity, wft=W), discarded
 
 
 @dataclass
@@ -210,15 +212,20 @@ class GpuSolver:
         self.precision, self.refine, self.graph = precision, refine, graph
         self.max_steps, self.min_steps, self.tol, self.seed = max_steps, min_steps, tol, seed
 
-    def __call__(self, prob: KronProblem):
+    def __call__(self, prob: KronProblem, init=None):
+        """init: None (uniform random from the seed), a host vector, or a DeviceArray (the
+        transformed wave function of the previous step). The returned report carries the
+        device ground vector as r["z_dev"] (the caller frees it)."""
         from . import BatchedGemm, DeviceArray
 
-        init = np.random.default_rng(self.seed).uniform(-1.0, 1.0, prob.n)
+        own_init = not isinstance(init, DeviceArray)
+        if init is None:
+            init = np.random.default_rng(self.seed).uniform(-1.0, 1.0, prob.n)
         t0 = time.perf_counter()
         op = BatchedGemm(prob, "fp64")
         info = op.info()
         t1 = time.perf_counter()
-        dinit = DeviceArray.from_host(init)
+        dinit = DeviceArray.from_host(init) if own_init else init
         dz = DeviceArray(prob.n)
         r = op.lanczos_ground(dinit, dz, max_steps=self.max_steps, min_steps=self.min_steps,
                               tolerance=self.tol, precision=self.precision, refine=self.refine,
@@ -226,8 +233,9 @@ class GpuSolver:
         z = dz.download()
         t2 = time.perf_counter()
         op.close()
-        dinit.free()
-        dz.free()
+        if own_init:
+            dinit.free()
+        r["z_dev"] = dz
         r["plan"] = {k: info[k] for k in ("nterms", "flops_algorithmic", "stage1_tiles",
                                           "stage2_tiles")}
         r["timing"] = {"plan_ms": 1e3 * (t1 - t0), "lanczos_ms": 1e3 * (t2 - t1)}
@@ -241,7 +249,8 @@ def clone_state(state: dict) -> dict:
 
 
 def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: list,
-        solver, log=print, stop_after=None, state: dict | None = None, resume: bool = False):
+        solver, log=print, stop_after=None, state: dict | None = None, resume: bool = False,
+        wft=None):
     """Infinite growth to the full lattice, then finite sweeps.
 
     target_of(nsites) gives the target quantum number of a superblock of nsites sites;
@@ -264,6 +273,9 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
     energy = None
 
     W = lattice.max_range
+    prev = {}
+    if wft is True:
+        wft = GpuWft()
 
     def step(phase, lsize, rsize, m, trunc):
         nonlocal energy
@@ -280,10 +292,28 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
         target = target_of(lsize + rsize + 2)
         prob, patches = kron_problem(Lb, Rb, model, lattice, target,
                                      name=f"{model.name}-{phase}-{l}", offset=Ls - nsuper)
+        tw = time.perf_counter()
+        init = _wft_init(wft, prev, phase, l, Lb, Rb, prob, patches, left, right, lsize,
+                         rsize) if wft else None
         t0 = time.perf_counter()
-        e, z, rep = solver(prob)
+        e, z, rep = solver(prob, init=init) if wft else solver(prob)
         t1 = time.perf_counter()
-        timing = {"grow_ms": 1e3 * (tk0 - tg), "kron_ms": 1e3 * (t0 - tk0)}
+        rep = dict(rep)
+        if init is not None:
+            wft.free(init)
+        if wft:
+            old = prev.pop("z_dev", None)
+            if old is not None:
+                wft.free(old)
+            prev.update(phase=phase, l=l, Lb=Lb, Rb=Rb, prob=prob, patches=patches,
+                        z_dev=rep.pop("z_dev", None))
+            rep["wft_init"] = init is not None
+        elif "z_dev" in rep:
+            zd = rep.pop("z_dev")
+            if hasattr(zd, "free"):
+                zd.free()
+        timing = {"grow_ms": 1e3 * (tk0 - tg), "kron_ms": 1e3 * (tw - tk0),
+                  "wft_ms": 1e3 * (t0 - tw)}
         off = prob.offsets
         psi = [z[off[I]:off[I + 1]].reshape((prob.R[I], prob.L[I]), order="F")
                for I in range(prob.npatches)]
@@ -298,7 +328,6 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
             disc = max(disc, d)
         energy = e
         timing["truncate_ms"] = 1e3 * (time.perf_counter() - t1)
-        rep = dict(rep)
         timing.update(rep.get("timing", {}))
         rep["timing"] = timing
         rec = StepRecord(phase, l, m, prob.n, prob.npatches, len(prob.blocks), e,
@@ -310,7 +339,7 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
             f"terms={len(prob.blocks):5d} E={e:.12f} steps={rec.lanczos_steps:3d} "
             f"solve={rec.ms_lanczos:7.1f}ms (plan {tm.get('plan_ms', 0):6.1f} lanczos "
             f"{tm.get('lanczos_ms', 0):6.1f}) grow={tm['grow_ms']:6.1f} kron={tm['kron_ms']:6.1f} "
-            f"trunc={tm['truncate_ms']:6.1f} disc={disc:.2e}")
+            f"wft={tm['wft_ms']:6.1f} trunc={tm['truncate_ms']:6.1f} disc={disc:.2e}")
         return rec
 
     # Infinite-size growth: superblock of 2*lsize + 2 sites.
@@ -331,4 +360,54 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
         step(phase, l, Ls - l - 2, m, trunc)
         if stop_after and len(records) >= stop_after:
             break
+    if wft and prev.get("z_dev") is not None:
+        wft.free(prev["z_dev"])
     return energy, records
+
+
+class GpuWft:
+    """Executes the wave-function transformation on the device (DeviceArray vectors)."""
+
+    def copy(self, z, n):
+        from . import DeviceArray, _check, lib
+        y = DeviceArray(n)
+        _check(lib().dmk_device_copy_d(y.ptr, z.ptr, n), "dmk_device_copy_d")
+        return y
+
+    def apply(self, chains, z, old_off, old_R, n, new_off, new_R):
+        from . import wft as W
+        return W.apply_gpu(chains, z, old_off, old_R, n, new_off, new_R)
+
+    def free(self, z):
+        z.free()
+
+
+def _wft_init(ex, prev, phase, l, Lb, Rb, prob, patches, left, right, lsize, rsize):
+    """The previous ground state carried to this superblock, or None (infinite growth, first
+    step). A step at the same center (end of the infinite growth) reuses it unchanged."""
+    from . import wft as W
+
+    if not prev or phase == "infinite" or prev.get("z_dev") is None:
+        return None
+    pprob, pl = prev["prob"], prev["l"]
+    old = (prev["patches"], pprob.L, pprob.R)
+    new = (patches, prob.L, prob.R)
+    if l == pl:
+        if prob.n != pprob.n or list(patches) != list(prev["patches"]):
+            return None
+        return ex.copy(prev["z_dev"], prob.n)
+    if l == pl + 1:
+        # S' = left[lsize] was truncated from the old left block; E = right[rsize + 1] expands
+        # into s_{l+1}.right[rsize] = this step's right block.
+        Sp, E = left[lsize], right[rsize + 1]
+        if Sp.wft is None or E.wft is None:
+            return None
+        chains = W.chains_move_right(old, new, Sp.wft, E.wft, prev["Rb"].product, Lb.product)
+    elif l == pl - 1:
+        Epp, S = right[rsize], left[lsize + 1]
+        if Epp.wft is None or S.wft is None:
+            return None
+        chains = W.chains_move_left(old, new, Epp.wft, S.wft, prev["Lb"].product, Rb.product)
+    else:
+        return None
+    return ex.apply(chains, prev["z_dev"], pprob.offsets, pprob.R, prob.n, prob.offsets, prob.R)

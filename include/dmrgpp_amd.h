@@ -251,6 +251,29 @@ int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t 
                         const dmk_krylov_params* params, dmk_krylov_result* results,
                         dmk_krylov_stats* stats);
 
+/* Variable-size batched GEMM: replaces dmrg_Xgemm_vbatch<T> (GPUPlugin/dmrg_vbatch.h:118-133,
+ * dmrg_vbatch.cpp:129-652), the free function the reference plugin and its test_vbatch driver
+ * call. For every entry e of every group g: C_e = alpha op(A_e) op(B_e) + beta C_e, column major,
+ * op from transa/transb ('N', 'T', or 'C' = conjugate transpose as in BLAS). m/n/k/lda/ldb/ldc
+ * and group_size have group_count entries; a/b/c hold one device pointer per entry, in group
+ * order (host arrays). C is not read when beta == 0. Runs on the MFMA grouped kernel (FP64);
+ * synchronous. Used by the sweep driver for the wave-function transformation GEMMs
+ * (WftAccelBlocks.h:162-322). */
+int dmk_gemm_vbatch_d(char transa, char transb, const int* m_array, const int* n_array,
+                      const int* k_array, double alpha, const double* const* a_array,
+                      const int* lda_array, const double* const* b_array, const int* ldb_array,
+                      double beta, double* const* c_array, const int* ldc_array,
+                      size_t group_count, const int* group_size);
+
+/* Complex instantiation (dmrg_vbatch.cpp template for std::complex<double>): matrices are
+ * interleaved (re, im), leading dimensions in complex elements; alpha and beta point to
+ * (re, im) pairs. */
+int dmk_gemm_vbatch_z(char transa, char transb, const int* m_array, const int* n_array,
+                      const int* k_array, const double* alpha, const double* const* a_array,
+                      const int* lda_array, const double* const* b_array, const int* ldb_array,
+                      const double* beta, double* const* c_array, const int* ldc_array,
+                      size_t group_count, const int* group_size);
+
 /* Device memory helpers for callers without their own allocator. */
 int dmk_device_alloc(size_t bytes, void** ptr);
 int dmk_device_free(void* ptr);

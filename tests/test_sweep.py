@@ -117,3 +117,80 @@ def test_gpu_sweep_heisenberg_chain32_reference_pin():
     e32, _, _ = configs.run_config("heisenberg-chain-32", precision="fp32", sweeps=1,
                                    log=lambda s: None)
     assert abs(e32 - e) < 1e-10 * abs(e)
+
+
+class HostWft:
+    """Host executor of the wave-function transformation chains (test reference)."""
+
+    def copy(self, z, n):
+        return np.array(z)
+
+    def apply(self, chains, z, old_off, old_R, n, new_off, new_R):
+        from dmrgpp_amd import wft
+        return wft.apply_numpy(chains, z, old_off, old_R, n, new_off, new_R)
+
+    def free(self, z):
+        pass
+
+
+class OracleSolverInit:
+    """Oracle Lanczos that starts from a given vector and reports the overlap of that start with
+    the ground state it finds."""
+
+    def __init__(self):
+        self.overlaps = []
+
+    def __call__(self, prob, init=None):
+        x0 = np.random.default_rng(1234).uniform(-1, 1, prob.n) if init is None else init
+        e, z, s, _, _ = oracle.lanczos(prob, x0)
+        zn = z / np.linalg.norm(z)
+        if init is not None:
+            self.overlaps.append((s, abs(zn @ init) / np.linalg.norm(init)))
+        return e, z, {"steps": s, "matvecs": s, "z_dev": zn}
+
+
+@pytest.mark.parametrize("name,model,lat,target,m,tol", [
+    # No truncation (m covers the blocks): the transformation is exact.
+    ("heisenberg-chain-10", models.heisenberg(), models.chain(10), lambda N: (0,), 64, 1e-12),
+    ("heisenberg-ladder-4x2", models.heisenberg(), models.ladder(4, 2), lambda N: (0,), 16, 1e-12),
+    # Truncated fermions (Jordan-Wigner signs in the transformed state): overlap near 1.
+    ("hubbard-chain-8", models.hubbard(4.0), models.chain(8), lambda N: (N // 2, N // 2), 64, 1e-7),
+])
+def test_wft_carries_ground_state_between_steps(name, model, lat, target, m, tol):
+    e_ref, recs_ref = sweep.run(model, lat, target, m, [m, m], OracleSolver(), log=lambda s: None)
+    sol = OracleSolverInit()
+    e, recs = sweep.run(model, lat, target, m, [m, m], sol, log=lambda s: None, wft=HostWft())
+    assert abs(e - e_ref) < 1e-10
+    # Every finite step after the first starts from the transformed previous ground state.
+    nfinite = sum(r.phase != "infinite" for r in recs)
+    assert len(sol.overlaps) == nfinite
+    assert min(ov for _, ov in sol.overlaps) > 1 - tol
+    steps_wft = sum(r.lanczos_steps for r in recs if r.phase != "infinite")
+    steps_ref = sum(r.lanczos_steps for r in recs_ref if r.phase != "infinite")
+    assert steps_wft < 0.6 * steps_ref, (steps_wft, steps_ref)
+
+
+@pytest.mark.gpu
+def test_gpu_wft_sweep_matches_host_chains():
+    # The same sweep with the device executor (batched GEMMs) and the host one: equal energies,
+    # and the GPU-transformed start vectors agree with the host evaluation of the same chains.
+    from dmrgpp_amd import DeviceArray, wft
+
+    model, lat, target, m = models.heisenberg(), models.ladder(6, 2), (lambda N: (0,)), 64
+    e_host, _ = sweep.run(model, lat, target, m, [m], OracleSolverInit(), log=lambda s: None,
+                          wft=HostWft())
+    checked = []
+
+    class CheckingGpuWft(sweep.GpuWft):
+        def apply(self, chains, z, old_off, old_R, n, new_off, new_R):
+            y = super().apply(chains, z, old_off, old_R, n, new_off, new_R)
+            want = wft.apply_numpy(chains, z.download(), old_off, old_R, n, new_off, new_R)
+            got = y.download()
+            checked.append(np.abs(got - want).max() / np.abs(want).max())
+            return y
+
+    e_gpu, recs = sweep.run(model, lat, target, m, [m], sweep.GpuSolver(), log=lambda s: None,
+                            wft=CheckingGpuWft())
+    assert abs(e_gpu - e_host) < 1e-9
+    assert checked and max(checked) < 1e-13
+    assert all(r.info.get("wft_init") for r in recs if r.phase != "infinite")
