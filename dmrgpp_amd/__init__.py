@@ -65,7 +65,8 @@ class LanczosParams(ctypes.Structure):
                 ("tolerance", ctypes.c_double), ("reortho", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("refine", ctypes.c_int32),
                 ("refine_min_steps", ctypes.c_int32), ("refine_tolerance", ctypes.c_double),
-                ("graph", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("graph", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("stop_residual", ctypes.c_double)]
 
 
 class LanczosResult(ctypes.Structure):
@@ -106,6 +107,29 @@ class KrylovStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+PRECONDS = {"none": 0, "diag": 1, "mlp": 2}
+MLP_PARAMS = 4545
+MLP_FEATURES = 4
+
+
+class DavidsonParams(ctypes.Structure):
+    _fields_ = [("max_matvecs", ctypes.c_int32), ("max_basis", ctypes.c_int32),
+                ("tolerance", ctypes.c_double), ("precond", ctypes.c_int32),
+                ("harvest", ctypes.c_int32), ("harvest_samples", ctypes.c_int32),
+                ("harvest_iters", ctypes.c_int32)]
+
+
+class DavidsonResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("residual", ctypes.c_double),
+                ("iterations", ctypes.c_int32), ("matvecs", ctypes.c_int32),
+                ("restarts", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("samples", ctypes.c_int64), ("ms_total", ctypes.c_double),
+                ("ms_matvec", ctypes.c_double), ("ms_precond", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 # Every symbol include/dmrgpp_amd.h declares, with its ctypes signature.
@@ -141,6 +165,21 @@ SIGNATURES = {
     "dmk_krylov_default_params": (_I, [ctypes.POINTER(KrylovParams)]),
     "dmk_krylov_evolve_z": (_I, [_VP, _I, _VP, _SZ, _I, _VP, _VP, _SZ,
                                  ctypes.POINTER(KrylovParams), _VP, ctypes.POINTER(KrylovStats)]),
+    "dmk_diagonal_d": (_I, [_VP, _VP]),
+    "dmk_mlp_create": (_I, [ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_VP)]),
+    "dmk_mlp_destroy": (None, [_VP]),
+    "dmk_mlp_get_params": (_I, [_VP, _VP]),
+    "dmk_mlp_set_params": (_I, [_VP, _VP]),
+    "dmk_mlp_forward": (_I, [_VP, _VP, ctypes.c_int64, _VP]),
+    "dmk_mlp_gradient": (_I, [_VP, _VP, _VP, ctypes.c_int64, _VP, ctypes.POINTER(_D)]),
+    "dmk_mlp_add_samples": (_I, [_VP, _VP, _VP, ctypes.c_int64]),
+    "dmk_mlp_samples": (_I, [_VP, ctypes.POINTER(ctypes.c_int64)]),
+    "dmk_mlp_clear_samples": (_I, [_VP]),
+    "dmk_mlp_train": (_I, [_VP, _I, ctypes.c_int64, ctypes.c_float, ctypes.POINTER(_D),
+                           ctypes.POINTER(_D)]),
+    "dmk_davidson_default_params": (_I, [ctypes.POINTER(DavidsonParams)]),
+    "dmk_davidson_ground_d": (_I, [_VP, _VP, _VP, ctypes.POINTER(DavidsonParams), _VP,
+                                   ctypes.POINTER(DavidsonResult)]),
     "dmk_device_alloc": (_I, [_SZ, ctypes.POINTER(_VP)]),
     "dmk_device_free": (_I, [_VP]),
     "dmk_memcpy_h2d": (_I, [_VP, _VP, _SZ]),
@@ -252,6 +291,85 @@ def krylov_params(**kw) -> KrylovParams:
             v = KRYLOV_STOP[v]
         setattr(p, k, v)
     return p
+
+
+def davidson_params(**kw) -> DavidsonParams:
+    p = DavidsonParams()
+    _check(lib().dmk_davidson_default_params(ctypes.byref(p)), "dmk_davidson_default_params")
+    for k, v in kw.items():
+        if k == "precond" and isinstance(v, str):
+            v = PRECONDS[v]
+        setattr(p, k, v)
+    return p
+
+
+class Mlp:
+    """The Davidson MLP preconditioner (dmk_mlp_*): parameters, training set, training."""
+
+    def __init__(self, seed: int = 7, capacity: int = 1 << 20):
+        h = ctypes.c_void_p()
+        _check(lib().dmk_mlp_create(seed, capacity, ctypes.byref(h)), "dmk_mlp_create")
+        self.handle = h.value
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        _check(lib().dmk_mlp_get_params(self.handle, p.ctypes.data), "dmk_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == MLP_PARAMS
+        _check(lib().dmk_mlp_set_params(self.handle, p.ctypes.data), "dmk_mlp_set_params")
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        cnt = x.shape[0]
+        dx = DeviceArray.from_host(x.ravel())
+        dy = DeviceArray(max(cnt, 1), np.float32)
+        _check(lib().dmk_mlp_forward(self.handle, dx.ptr, cnt, dy.ptr), "dmk_mlp_forward")
+        return dy.download(cnt)
+
+    def gradient(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        dx, dt = DeviceArray.from_host(x.ravel()), DeviceArray.from_host(t)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().dmk_mlp_gradient(self.handle, dx.ptr, dt.ptr, t.size, g.ctypes.data,
+                                      ctypes.byref(loss)), "dmk_mlp_gradient")
+        return g, loss.value
+
+    def add_samples(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        dx, dt = DeviceArray.from_host(x.ravel()), DeviceArray.from_host(t)
+        _check(lib().dmk_mlp_add_samples(self.handle, dx.ptr, dt.ptr, t.size),
+               "dmk_mlp_add_samples")
+
+    def samples(self) -> int:
+        c = ctypes.c_int64()
+        _check(lib().dmk_mlp_samples(self.handle, ctypes.byref(c)), "dmk_mlp_samples")
+        return c.value
+
+    def clear_samples(self):
+        _check(lib().dmk_mlp_clear_samples(self.handle), "dmk_mlp_clear_samples")
+
+    def train(self, iters: int, batch: int = 4096, lr: float = 1e-3):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dmk_mlp_train(self.handle, iters, batch, lr, ctypes.byref(a), ctypes.byref(b)),
+               "dmk_mlp_train")
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().dmk_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def lanczos_params(**kw) -> LanczosParams:
@@ -400,6 +518,20 @@ class BatchedGemm:
         r = LanczosResult()
         _check(lib().dmk_lanczos_ground_d(self.handle, init.ptr, z.ptr, ctypes.byref(p),
                                           ctypes.byref(r)), "dmk_lanczos_ground_d")
+        return r.as_dict()
+
+    def diagonal(self, out: DeviceArray):
+        """diag(H) into a device array of n elements."""
+        _check(lib().dmk_diagonal_d(self.handle, out.ptr), "dmk_diagonal_d")
+
+    def davidson_ground(self, init: DeviceArray, z: DeviceArray, mlp: "Mlp | None" = None,
+                        **kw) -> dict:
+        """Davidson lowest state with the diagonal or MLP correction (dmk_davidson_ground_d)."""
+        p = davidson_params(**kw)
+        r = DavidsonResult()
+        _check(lib().dmk_davidson_ground_d(self.handle, init.ptr, z.ptr, ctypes.byref(p),
+                                           mlp.handle if mlp is not None else None,
+                                           ctypes.byref(r)), "dmk_davidson_ground_d")
         return r.as_dict()
 
     def profile(self, vin: DeviceArray, vout: DeviceArray, iters: int) -> tuple:

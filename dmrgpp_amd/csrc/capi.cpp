@@ -1,5 +1,6 @@
 // extern "C" layer of dmrgpp_amd (declarations and reference citations in include/dmrgpp_amd.h).
 #include "complex_embed.h"
+#include "davidson.h"
 #include "dmrgpp_amd.h"
 #include "kron_plan.h"
 #include "krylov.h"
@@ -15,6 +16,10 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+struct dmk_mlp {
+	std::unique_ptr<dmk::Mlp> net;
+};
 
 struct dmk_handle {
 	std::unique_ptr<dmk::Plan>             plan;
@@ -66,6 +71,13 @@ template <typename F> int guarded(F&& f)
 		dmk::set_last_error(e.what());
 		return DMK_ERR_DEVICE;
 	}
+}
+
+dmk::Mlp& net_of(const dmk_mlp* m)
+{
+	if (!m || !m->net)
+		throw std::invalid_argument("null dmk_mlp");
+	return *m->net;
 }
 
 dmk::Plan& plan_of(dmk_handle* h)
@@ -325,6 +337,7 @@ int dmk_lanczos_default_params(dmk_lanczos_params* params)
 		params->refine_min_steps = d.refine_min_steps;
 		params->refine_tolerance = d.refine_tolerance;
 		params->graph            = d.graph;
+		params->stop_residual    = d.stop_residual;
 		return DMK_OK;
 	});
 }
@@ -346,6 +359,7 @@ int dmk_lanczos_ground_d(dmk_handle* handle, const double* init, double* z,
 		lp.refine_min_steps = params->refine_min_steps;
 		lp.refine_tolerance = params->refine_tolerance;
 		lp.graph            = params->graph;
+		lp.stop_residual    = params->stop_residual;
 		if (!handle->lanczos)
 			handle->lanczos = std::make_unique<dmk::LanczosWorkspace>();
 		dmk::LanczosReport r = dmk::lanczos_ground(p, *handle->lanczos, init, z, lp);
@@ -477,6 +491,175 @@ int dmk_gemm_vbatch_z(char transa, char transb, const int* m_array, const int* n
 		dmk::gemm_vbatch(true, transa, transb, m_array, n_array, k_array, alpha, a_array,
 		                 lda_array, b_array, ldb_array, beta, c_array, ldc_array, group_count,
 		                 group_size);
+		return DMK_OK;
+	});
+}
+
+int dmk_diagonal_d(dmk_handle* handle, double* diag)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		p.diagonal(diag);
+		DMK_HIP_CHECK(hipStreamSynchronize(p.stream()));
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_create(uint64_t seed, int64_t sample_capacity, dmk_mlp** mlp)
+{
+	return guarded([&] {
+		if (!mlp)
+			throw std::invalid_argument("dmk_mlp_create: null pointer");
+		*mlp   = nullptr;
+		auto m = std::make_unique<dmk_mlp>();
+		m->net = std::make_unique<dmk::Mlp>(seed, sample_capacity);
+		*mlp   = m.release();
+		return DMK_OK;
+	});
+}
+
+void dmk_mlp_destroy(dmk_mlp* mlp) { delete mlp; }
+
+int dmk_mlp_get_params(const dmk_mlp* mlp, float* params)
+{
+	return guarded([&] {
+		if (!params)
+			throw std::invalid_argument("dmk_mlp_get_params: null pointer");
+		net_of(mlp).get_params(params);
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_set_params(dmk_mlp* mlp, const float* params)
+{
+	return guarded([&] {
+		if (!params)
+			throw std::invalid_argument("dmk_mlp_set_params: null pointer");
+		net_of(mlp).set_params(params);
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_forward(dmk_mlp* mlp, const float* x, int64_t count, float* y)
+{
+	return guarded([&] {
+		dmk::Mlp& m = net_of(mlp);
+		if (count < 0 || (count > 0 && (!x || !y)))
+			throw std::invalid_argument("dmk_mlp_forward: bad arguments");
+		dmk::launch_mlp_forward(m.params(), x, count, y, m.stream());
+		DMK_HIP_CHECK(hipStreamSynchronize(m.stream()));
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_gradient(dmk_mlp* mlp, const float* x, const float* t, int64_t count, float* grad,
+                     double* loss)
+{
+	return guarded([&] {
+		double l = net_of(mlp).gradient(x, t, count, grad);
+		if (loss)
+			*loss = l;
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_add_samples(dmk_mlp* mlp, const float* x, const float* t, int64_t count)
+{
+	return guarded([&] {
+		dmk::Mlp& m = net_of(mlp);
+		if (count < 0 || (count > 0 && (!x || !t)))
+			throw std::invalid_argument("dmk_mlp_add_samples: bad arguments");
+		const int64_t first = m.append(count);
+		const int64_t got   = m.samples() - first;
+		if (got > 0) {
+			DMK_HIP_CHECK(hipMemcpyAsync(m.samples_x() + first * dmk::kMlpIn, x,
+			                             size_t(got) * dmk::kMlpIn * sizeof(float),
+			                             hipMemcpyDeviceToDevice, m.stream()));
+			DMK_HIP_CHECK(hipMemcpyAsync(m.samples_t() + first, t, size_t(got) * sizeof(float),
+			                             hipMemcpyDeviceToDevice, m.stream()));
+			DMK_HIP_CHECK(hipStreamSynchronize(m.stream()));
+		}
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_samples(const dmk_mlp* mlp, int64_t* count)
+{
+	return guarded([&] {
+		if (!count)
+			throw std::invalid_argument("dmk_mlp_samples: null pointer");
+		*count = net_of(mlp).samples();
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_clear_samples(dmk_mlp* mlp)
+{
+	return guarded([&] {
+		net_of(mlp).clear_samples();
+		return DMK_OK;
+	});
+}
+
+int dmk_mlp_train(dmk_mlp* mlp, int iters, int64_t batch, float lr, double* loss_last,
+                  double* loss_all)
+{
+	return guarded([&] {
+		if (iters < 0 || !(lr > 0))
+			throw std::invalid_argument("dmk_mlp_train: bad arguments");
+		net_of(mlp).train(iters, batch, lr, loss_last, loss_all);
+		return DMK_OK;
+	});
+}
+
+int dmk_davidson_default_params(dmk_davidson_params* params)
+{
+	return guarded([&] {
+		if (!params)
+			throw std::invalid_argument("dmk_davidson_default_params: null pointer");
+		dmk::DavidsonParams d;
+		std::memset(params, 0, sizeof(*params));
+		params->max_matvecs     = d.max_matvecs;
+		params->max_basis       = d.max_basis;
+		params->tolerance       = d.tolerance;
+		params->precond         = d.precond;
+		params->harvest         = d.harvest;
+		params->harvest_samples = d.harvest_samples;
+		params->harvest_iters   = d.harvest_iters;
+		return DMK_OK;
+	});
+}
+
+int dmk_davidson_ground_d(dmk_handle* handle, const double* init, double* z,
+                          const dmk_davidson_params* params, dmk_mlp* mlp,
+                          dmk_davidson_result* result)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (handle->is_complex)
+			throw std::invalid_argument("dmk_davidson_ground_d: complex handle");
+		if (!params || !result)
+			throw std::invalid_argument("dmk_davidson_ground_d: null argument");
+		dmk::DavidsonParams dp;
+		dp.max_matvecs     = params->max_matvecs;
+		dp.max_basis       = params->max_basis;
+		dp.tolerance       = params->tolerance;
+		dp.precond         = params->precond;
+		dp.harvest         = params->harvest;
+		dp.harvest_samples = params->harvest_samples;
+		dp.harvest_iters   = params->harvest_iters;
+		dmk::DavidsonReport r = dmk::davidson_ground(p, init, z, dp, mlp ? mlp->net.get() : nullptr);
+		std::memset(result, 0, sizeof(*result));
+		result->energy     = r.energy;
+		result->residual   = r.residual;
+		result->iterations = r.iterations;
+		result->matvecs    = r.matvecs;
+		result->restarts   = r.restarts;
+		result->converged  = r.converged;
+		result->samples    = r.samples;
+		result->ms_total   = r.ms_total;
+		result->ms_matvec  = r.ms_matvec;
+		result->ms_precond = r.ms_precond;
 		return DMK_OK;
 	});
 }

@@ -324,6 +324,27 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	}
 	stats_.bytes_compulsory += 8.0 * 2.0 * double(stats_.n);
 
+	// Diagonals of the (I, I, k) terms, for diag(H).
+	pL_.resize(size_t(np));
+	pR_.resize(size_t(np));
+	dg_begin_.assign(size_t(np + 1), 0);
+	for (int64_t I = 0; I < np; ++I) {
+		pL_[size_t(I)]      = int32_t(L[I]);
+		pR_[size_t(I)]      = int32_t(R[I]);
+		dg_begin_[size_t(I)] = int32_t(dg_off_.size());
+		for (int64_t k = 0; k < nC; ++k) {
+			const size_t id = idx(I, I, k);
+			if (kind[id] == NONE)
+				continue;
+			dg_off_.push_back(int64_t(dg_vals_.size()));
+			for (int64_t l = 0; l < L[I]; ++l)
+				dg_vals_.push_back(Amatrix[id][l + l * int64_t(ldA[id])]);
+			for (int64_t r = 0; r < R[I]; ++r)
+				dg_vals_.push_back(Bmatrix[id][r + r * int64_t(ldB[id])]);
+		}
+	}
+	dg_begin_[size_t(np)] = int32_t(dg_off_.size());
+
 	// Packing. A panel of patch I: L_I x K_I, (J outer, k inner) general terms; then the
 	// identity-B terms' A blocks. B stack of (I, J): q*R_I x R_J for its q general terms;
 	// then identity-A terms' B blocks.
@@ -527,6 +548,7 @@ Plan::~Plan()
 	pool_device_free(d_x32_);
 	pool_device_free(d_reduce_);
 	pool_device_free(d_chunks_);
+	pool_device_free(d_dg_);
 	st1_.release();
 	st2_.release();
 	for (auto& e : ev_)
@@ -534,6 +556,43 @@ Plan::~Plan()
 			(void)hipEventDestroy(e);
 	if (stream_)
 		(void)hipStreamDestroy(stream_);
+}
+
+void launch_plan_diagonal(const void* tables, int np, int64_t nterms, int64_t nvals,
+                          const int64_t* poff, double* out, int64_t n, hipStream_t s);
+
+void Plan::diagonal(double* out)
+{
+	if (!out)
+		throw std::invalid_argument("diagonal: null output");
+	const int64_t np = int64_t(pL_.size());
+	// Tables: patch offsets (np + 1) int64, term offsets int64, then L, R, begin (int32), then
+	// the values (doubles), all 8-byte aligned.
+	const size_t nterms = dg_off_.size();
+	const size_t bytes  = 8 * (size_t(np + 1) + nterms) + 4 * (3 * size_t(np) + 1 + 1)
+	                     + 8 * dg_vals_.size() + 8;
+	if (!d_dg_) {
+		std::vector<char> h(bytes, 0);
+		char*             p = h.data();
+		std::memcpy(p, off_.data(), size_t(np + 1) * 8);
+		p += size_t(np + 1) * 8;
+		std::memcpy(p, dg_off_.data(), nterms * 8);
+		p += nterms * 8;
+		std::memcpy(p, pL_.data(), size_t(np) * 4);
+		p += size_t(np) * 4;
+		std::memcpy(p, pR_.data(), size_t(np) * 4);
+		p += size_t(np) * 4;
+		std::memcpy(p, dg_begin_.data(), size_t(np + 1) * 4);
+		p += size_t(np + 1) * 4;
+		if ((p - h.data()) % 8)
+			p += 8 - (p - h.data()) % 8;
+		std::memcpy(p, dg_vals_.data(), dg_vals_.size() * 8);
+		d_dg_ = pool_device_alloc(bytes);
+		DMK_HIP_CHECK(hipMemcpyAsync(d_dg_, h.data(), bytes, hipMemcpyHostToDevice, stream_));
+		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+	}
+	launch_plan_diagonal(d_dg_, int(np), int64_t(nterms), int64_t(dg_vals_.size()), nullptr, out,
+	                     stats_.n, stream_);
 }
 
 std::string Plan::histogram_key() const

@@ -103,6 +103,10 @@ public:
 	hipStream_t      stream() const { return stream_; }
 	int64_t          n() const { return stats_.n; }
 
+	// diag(H) into out (device, n): sum over the diagonal-patch terms (I, I, k) of
+	// diag(A_k(I,I)) (x) diag(B_k(I,I)). Enqueued on the plan's stream.
+	void diagonal(double* out);
+
 	// Average device time (ms) of each stage's kernel and of the whole matvec over iters calls,
 	// from HIP events recorded on the plan's stream around every launch.
 	void profile(const double* vin, double* vout, int iters, double* ms_stage1, double* ms_stage2,
@@ -146,6 +150,14 @@ private:
 	size_t               ws_bytes_  = 0;
 
 	hipEvent_t ev_[3] = { nullptr, nullptr, nullptr };
+
+	// Diagonals of the (I, I, k) terms: per term diag(A) (L_I values) then diag(B) (R_I values)
+	// at dg_off_[t]; terms of patch I are [dg_begin_[I], dg_begin_[I + 1]).
+	std::vector<double>  dg_vals_;
+	std::vector<int64_t> dg_off_;
+	std::vector<int32_t> dg_begin_;
+	std::vector<int32_t> pL_, pR_;
+	void*                d_dg_ = nullptr; // uploaded tables (one allocation)
 };
 
 } // namespace dmk

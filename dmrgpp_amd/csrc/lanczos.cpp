@@ -115,8 +115,8 @@ struct StepGraphs {
 // noise: relative matvec noise of a reduced-precision run that an FP64 restart follows (0 for
 // a plain run); see noise_reached below.
 Decomp decompose(Plan& plan, LanczosWorkspace& ws, const double* init, int max_steps, int min_steps,
-                 double tol, double noise, int reortho, bool use_graph, double& ms_matvec,
-                 hipEvent_t ev0, hipEvent_t ev1)
+                 double tol, double noise, double stop_res, int reortho, bool use_graph,
+                 double& ms_matvec, hipEvent_t ev0, hipEvent_t ev1)
 {
 	const int64_t n      = plan.n();
 	hipStream_t   s      = plan.stream();
@@ -176,14 +176,20 @@ Decomp decompose(Plan& plan, LanczosWorkspace& ws, const double* init, int max_s
 	// pair, |beta_{j+1} s_j| with s the unit eigenvector of T_{j+1}, drops below
 	// noise * ||H|| (||H|| estimated by the extreme Ritz values): past that point its matvec noise
 	// dominates, and the FP64 restart takes over.
+	// With stop_res > 0 the same estimate is also compared with that absolute threshold.
 	auto noise_reached = [&](int jj, double e0) {
-		if (noise <= 0 || jj + 1 >= maxn)
+		if ((noise <= 0 && stop_res <= 0) || jj + 1 >= maxn)
 			return false;
 		const double bn = d.b[size_t(jj + 1)];
 		std::vector<double> c = tridiag_eigenvector(d.a.data(), d.b.data(), jj + 1, e0);
+		const double est = std::fabs(bn * c[size_t(jj)]);
+		if (stop_res > 0 && est < stop_res)
+			return true;
+		if (noise <= 0)
+			return false;
 		const double emax  = tridiag_eigenvalue(d.a.data(), d.b.data(), jj + 1, jj);
 		const double hnorm = std::max(std::fabs(e0), std::fabs(emax));
-		return std::fabs(bn * c[size_t(jj)]) < noise * hnorm;
+		return est < noise * hnorm;
 	};
 	if (use_graph && !reortho && maxn > 1) {
 		// Ring of three fixed vectors: V_{j-1}, V_j, V_{j+1} live in ring[(j-1)%3], ring[j%3],
@@ -317,9 +323,10 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 		// Relative matvec noise of the reduced-precision modes (about 16 units of roundoff of the
 		// FP32 accumulation; the (hi, lo) bf16 pair representation for BF16x3).
 		const double noise = (p.precision == PREC_FP32) ? 1e-6 : (p.precision == PREC_BF16X3) ? 1e-4 : 0.0;
+		const bool mixed_first = (p.precision != PREC_FP64 && p.refine);
 		Decomp d = decompose(plan, ws, init, p.max_steps, p.min_steps, p.tolerance,
-		                     (p.refine ? noise : 0.0), p.reortho, p.graph != 0, r.ms_matvec, ev0,
-		                     ev1);
+		                     (p.refine ? noise : 0.0), mixed_first ? 0.0 : p.stop_residual,
+		                     p.reortho, p.graph != 0, r.ms_matvec, ev0, ev1);
 		ritz_vector(plan, ws, d, z);
 		r.energy_first    = d.energy;
 		r.steps_first     = d.m;
@@ -334,8 +341,8 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 			DMK_HIP_CHECK(hipMemcpyAsync(ws.tmp2(), z, size_t(n) * sizeof(double),
 			                             hipMemcpyDeviceToDevice, plan.stream()));
 			Decomp d2 = decompose(plan, ws, ws.tmp2(), p.max_steps, p.refine_min_steps,
-			                      p.refine_tolerance, 0.0, p.reortho, p.graph != 0, r.ms_matvec,
-			                      ev0, ev1);
+			                      p.refine_tolerance, 0.0, p.stop_residual, p.reortho,
+			                      p.graph != 0, r.ms_matvec, ev0, ev1);
 			ritz_vector(plan, ws, d2, z);
 			r.energy = d2.energy;
 			r.steps  = d2.m;

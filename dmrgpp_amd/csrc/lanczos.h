@@ -17,6 +17,7 @@ struct LanczosParams {
 	int    refine_min_steps = 4;
 	double refine_tolerance = 1e-12;
 	int    graph            = 1;     // capture each step in a HIP graph (not with reortho)
+	double stop_residual    = 0;     // > 0: stop once |beta_{j+1} s_j| < stop_residual
 };
 
 struct LanczosReport {

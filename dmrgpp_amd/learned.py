@@ -1,0 +1,155 @@
+"""The learned-preconditioner workload: Davidson with an MLP correction trained on pairs harvested
+from the solver's own converged runs, compared with plain Lanczos and with Davidson's diagonal
+correction at the same FP64 residual.
+
+  python -m dmrgpp_amd.learned --config hubbard-ladder-8x2 [--m M] [--train-sweeps 1]
+
+Phase 1 runs the first sweeps with Davidson (diagonal correction) and harvests training pairs
+from every converged step; the MLP is then trained on the GPU. Phase 2 runs the remaining
+sweeps three times from the same block stores (Lanczos, Davidson + diagonal, Davidson + MLP),
+each step started from the same transformed wave function, and counts matvecs and time.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import models, sweep
+
+
+class DavidsonSolver:
+    """sweep.run solver: Davidson with the given correction; harvests into mlp when asked."""
+
+    def __init__(self, precond="diag", mlp=None, harvest=False, tol=1e-9, max_basis=24,
+                 max_matvecs=600, seed=1234):
+        self.precond, self.mlp, self.harvest = precond, mlp, harvest
+        self.tol, self.max_basis, self.max_matvecs, self.seed = tol, max_basis, max_matvecs, seed
+
+    def __call__(self, prob, init=None):
+        from . import BatchedGemm, DeviceArray
+
+        own = not isinstance(init, DeviceArray)
+        if init is None:
+            init = np.random.default_rng(self.seed).uniform(-1.0, 1.0, prob.n)
+        t0 = time.perf_counter()
+        op = BatchedGemm(prob, "fp64")
+        t1 = time.perf_counter()
+        dinit = DeviceArray.from_host(init) if own else init
+        dz = DeviceArray(prob.n)
+        r = op.davidson_ground(dinit, dz, mlp=self.mlp, precond=self.precond, tolerance=self.tol,
+                               max_basis=self.max_basis, max_matvecs=self.max_matvecs,
+                               harvest=1 if self.harvest else 0)
+        z = dz.download()
+        t2 = time.perf_counter()
+        op.close()
+        if own:
+            dinit.free()
+        r["steps"] = r["iterations"]
+        r["z_dev"] = dz
+        r["timing"] = {"plan_ms": 1e3 * (t1 - t0), "lanczos_ms": 1e3 * (t2 - t1)}
+        return r["energy"], z, r
+
+
+class LanczosResidualSolver(sweep.GpuSolver):
+    """Plain Lanczos stopped at the same FP64 residual as the Davidson runs."""
+
+    def __init__(self, tol=1e-9, max_steps=600):
+        super().__init__(max_steps=max_steps, tol=0.0)
+        self.stop_residual = tol
+
+    def __call__(self, prob, init=None):
+        from . import BatchedGemm, DeviceArray
+
+        own = not isinstance(init, DeviceArray)
+        if init is None:
+            init = np.random.default_rng(self.seed).uniform(-1.0, 1.0, prob.n)
+        t0 = time.perf_counter()
+        op = BatchedGemm(prob, "fp64")
+        t1 = time.perf_counter()
+        dinit = DeviceArray.from_host(init) if own else init
+        dz = DeviceArray(prob.n)
+        r = op.lanczos_ground(dinit, dz, max_steps=self.max_steps, min_steps=self.min_steps,
+                              tolerance=0.0, stop_residual=self.stop_residual, graph=1)
+        z = dz.download()
+        t2 = time.perf_counter()
+        op.close()
+        if own:
+            dinit.free()
+        r["z_dev"] = dz
+        r["timing"] = {"plan_ms": 1e3 * (t1 - t0), "lanczos_ms": 1e3 * (t2 - t1)}
+        return r["energy"], z, r
+
+
+CONFIGS = {
+    # Hubbard 4-leg cylinder 16x4, U=8, t=1, half filling, m=4096 (the learned-preconditioner
+    # workload); smaller stand-ins of the same model for tests.
+    "hubbard-cylinder-16x4": dict(model=lambda: models.hubbard(8.0),
+                                  lattice=lambda: models.ladder(16, 4, periodic_y=True),
+                                  target=lambda N: (N // 2, N // 2), m=4096),
+    "hubbard-ladder-8x2": dict(model=lambda: models.hubbard(8.0), lattice=lambda: models.ladder(8, 2),
+                               target=lambda N: (N // 2, N // 2), m=256),
+    "hubbard-chain-12": dict(model=lambda: models.hubbard(8.0), lattice=lambda: models.chain(12),
+                             target=lambda N: (N // 2, N // 2), m=128),
+}
+
+
+def compare(name: str, m=None, train_sweeps=1, test_sweeps=1, tol=1e-9, train_iters=3000,
+            log=print, mlp_seed=11):
+    """Returns a dict with per-method totals over the test sweeps."""
+    from . import Mlp
+
+    c = CONFIGS[name]
+    model, lat, target = c["model"](), c["lattice"](), c["target"]
+    m = m or c["m"]
+    mlp = Mlp(seed=mlp_seed, capacity=1 << 22)
+    state = {}
+    t0 = time.perf_counter()
+    harvest = DavidsonSolver("diag", mlp=mlp, harvest=True, tol=tol)
+    sweep.run(model, lat, target, m, [m] * train_sweeps, harvest, log=log, state=state, wft=True)
+    t1 = time.perf_counter()
+    nsamp = mlp.samples()
+    loss_last, loss_all = mlp.train(train_iters, batch=8192, lr=2e-3)
+    t2 = time.perf_counter()
+    log(f"harvested {nsamp} pairs in {t1 - t0:.1f} s; trained {train_iters} Adam steps in "
+        f"{t2 - t1:.2f} s, loss {loss_all:.4g}")
+    out = {"config": name, "m": m, "samples": nsamp, "train_s": t2 - t1, "loss": loss_all}
+    solvers = {"lanczos": LanczosResidualSolver(tol=tol),
+               "davidson-diag": DavidsonSolver("diag", tol=tol),
+               "davidson-mlp": DavidsonSolver("mlp", mlp=mlp, tol=tol)}
+    for key, solver in solvers.items():
+        st = sweep.clone_state(state)
+        ts = time.perf_counter()
+        e, recs = sweep.run(model, lat, target, m, [m] * test_sweeps, solver, log=lambda s: None,
+                            state=st, resume=True, wft=True)
+        te = time.perf_counter()
+        mv = sum(r.matvecs for r in recs)
+        res = max(r.info.get("residual", 0.0) for r in recs)
+        out[key] = {"energy": e, "matvecs": mv, "wall_s": te - ts, "max_residual": res,
+                    "steps": len(recs)}
+        log(f"{key:14s} E={e:.12f} matvecs={mv:6d} wall={te - ts:7.2f}s max_res={res:.2e}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="hubbard-ladder-8x2", choices=sorted(CONFIGS))
+    ap.add_argument("--m", type=int, default=None)
+    ap.add_argument("--train-sweeps", type=int, default=1)
+    ap.add_argument("--test-sweeps", type=int, default=1)
+    ap.add_argument("--train-iters", type=int, default=3000)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    out = compare(a.config, a.m, a.train_sweeps, a.test_sweeps, train_iters=a.train_iters,
+                  log=lambda s: print(s, flush=True))
+    print(json.dumps(out), flush=True)
+    if a.json:
+        with open(a.json, "w") as fh:
+            json.dump(out, fh)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

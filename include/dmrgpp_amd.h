@@ -167,6 +167,8 @@ typedef struct dmk_lanczos_params {
 	double  refine_tolerance;
 	int32_t graph;            /* 1: each step is one replay of a captured HIP graph (default) */
 	int32_t pad;
+	double  stop_residual;    /* > 0: also stop once the Ritz residual estimate |beta_{j+1} s_j|
+	                             of the unit lowest Ritz vector drops below it (0 = off) */
 } dmk_lanczos_params;
 
 typedef struct dmk_lanczos_result {
@@ -273,6 +275,77 @@ int dmk_gemm_vbatch_z(char transa, char transb, const int* m_array, const int* n
                       const int* lda_array, const double* const* b_array, const int* ldb_array,
                       const double* beta, double* const* c_array, const int* ldc_array,
                       size_t group_count, const int* group_size);
+
+/* diag(H) into diag (device, n elements): sum over the diagonal-patch terms (I, I, k) of
+ * diag(A_k(I,I)) (x) diag(B_k(I,I)). No reference counterpart (used by the Davidson
+ * corrections below). Synchronous. */
+int dmk_diagonal_d(dmk_handle* handle, double* diag);
+
+/* Learned Davidson preconditioner: a per-element MLP (4 features -> 64 -> 64 -> 1, tanh, FP32)
+ * with its training set on the device. No reference counterpart (the reference has no
+ * Davidson, ParametersDmrgSolver.h:154-158). Parameters are DMK_MLP_PARAMS floats: W1 (64x4
+ * row major), b1 (64), W2 (64x64 row major), b2 (64), W3 (64), b3 (1). */
+#define DMK_MLP_PARAMS 4545
+#define DMK_MLP_FEATURES 4
+typedef struct dmk_mlp dmk_mlp;
+int  dmk_mlp_create(uint64_t seed, int64_t sample_capacity, dmk_mlp** mlp);
+void dmk_mlp_destroy(dmk_mlp* mlp);
+int  dmk_mlp_get_params(const dmk_mlp* mlp, float* params);
+int  dmk_mlp_set_params(dmk_mlp* mlp, const float* params);
+/* y[s] = MLP(x[s]) for count samples; x (count x 4, row major) and y are device arrays. */
+int  dmk_mlp_forward(dmk_mlp* mlp, const float* x, int64_t count, float* y);
+/* Gradient (host, DMK_MLP_PARAMS floats) of the mean squared error (loss, host) over the
+ * device batch (x, t). */
+int  dmk_mlp_gradient(dmk_mlp* mlp, const float* x, const float* t, int64_t count, float* grad,
+                      double* loss);
+/* Appends count device samples (x, t) to the training set (truncated at its capacity). */
+int  dmk_mlp_add_samples(dmk_mlp* mlp, const float* x, const float* t, int64_t count);
+int  dmk_mlp_samples(const dmk_mlp* mlp, int64_t* count);
+int  dmk_mlp_clear_samples(dmk_mlp* mlp);
+/* iters Adam steps (lr) over minibatches of batch consecutive training samples; loss_last is
+ * the mean squared error of the last minibatch, loss_all of the whole set after training (either
+ * may be NULL). */
+int  dmk_mlp_train(dmk_mlp* mlp, int iters, int64_t batch, float lr, double* loss_last,
+                   double* loss_all);
+
+enum dmk_precond
+{
+	DMK_PRECOND_NONE = 0, /* t = r */
+	DMK_PRECOND_DIAG = 1, /* t = r / (theta - diag H) */
+	DMK_PRECOND_MLP  = 2, /* t = MLP(features) in units of rms(r) / rms(diag H - theta) */
+};
+
+typedef struct dmk_davidson_params {
+	int32_t max_matvecs;
+	int32_t max_basis;       /* restart size (3..128) */
+	double  tolerance;       /* stop when ||H u - theta u|| < tolerance for the unit Ritz vector */
+	int32_t precond;         /* enum dmk_precond */
+	int32_t harvest;         /* 1: append training pairs from this run to the MLP's set */
+	int32_t harvest_samples; /* sampled elements per recorded iterate */
+	int32_t harvest_iters;   /* recorded iterates at most */
+} dmk_davidson_params;
+
+typedef struct dmk_davidson_result {
+	double  energy;
+	double  residual;        /* ||H z - E z|| with a fresh FP64 matvec (z unit) */
+	int32_t iterations;
+	int32_t matvecs;         /* including the residual check */
+	int32_t restarts;
+	int32_t converged;
+	int64_t samples;         /* training pairs appended */
+	double  ms_total;
+	double  ms_matvec;
+	double  ms_precond;
+} dmk_davidson_result;
+
+int dmk_davidson_default_params(dmk_davidson_params* params);
+
+/* Lowest eigenpair by Davidson with the chosen correction (FP64 matvecs, device-resident basis);
+ * mlp is needed for DMK_PRECOND_MLP and for harvest, else may be NULL. z receives the unit
+ * Ritz vector. */
+int dmk_davidson_ground_d(dmk_handle* handle, const double* init, double* z,
+                          const dmk_davidson_params* params, dmk_mlp* mlp,
+                          dmk_davidson_result* result);
 
 /* Device memory helpers for callers without their own allocator. */
 int dmk_device_alloc(size_t bytes, void** ptr);
