@@ -5,14 +5,18 @@ states. The sweep driver grows the system by the infinite-size algorithm with th
 the GPU; the superblock of its last step (the middle of the 64-site ladder: left block S+s and
 right block e+E of 2m states each) is the matvec workload. Untimed setup.
 
-A step is one matvec y = H x with x and y resident in HBM; value = FP64 matvecs per second.
-Beside it: the FP32-MFMA matvec rate and its error against FP64; the ground state of the same
-superblock by the device Lanczos in FP64 and in FP32 + FP64 refinement (energy difference, the
-|dE| < 1e-10 gate); a bounded finite-sweep segment in both modes with the estimated full-sweep
-time; the roofline of the dominant grouped-GEMM launch from HIP events on its stream; and the
-CPU baseline (KronConnections restatement with numpy BLAS) on a bounded number of matvecs.
+A step is one matvec y = H x with x and y resident in HBM. value = matvecs per second of the
+FP32-MFMA matvec (FP32 accumulation), the mode the mixed-precision eigensolver uses, provided
+the ground state of the same superblock by FP32 Lanczos + FP64 refinement matches the FP64
+Lanczos energy to |dE| < 1e-10 relative (the gate is reported; without it the line says so).
+Beside it: the FP64 and BF16x3 matvec rates with their errors against FP64; the ground state in
+all three modes; a bounded finite-sweep segment (steps started from the transformed previous
+wave function) in each mode with the estimated full-sweep time; the roofline of the dominant
+grouped-GEMM launch of each mode from HIP events on its stream; and the CPU baseline
+(KronConnections restatement with numpy BLAS) on a bounded number of matvecs.
 
 Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--sweep-steps 6] [--no-cpu]
+                       [--save-problem F.npz | --load-problem F.npz]
 """
 import argparse
 import json
@@ -25,9 +29,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 157.3}  # bf16x3: FP32-equivalent work
 M = 2048
 LX = 32
+# HBM bytes per launch of the dominant kernel (stage 2) from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this workload, FETCH scaled by the copy-kernel calibration factor 2.0 (the
+# profiles directory holds the summaries); None where not measured.
+TRAFFIC = {}
 
 
 def log(*a):
@@ -59,6 +67,28 @@ def cpu_baseline(prob, x, budget_s=15.0, max_calls=20):
                       f"restatement, numpy BLAS GEMMs, {threads} BLAS threads), {dt:.1f} s"}
 
 
+def save_problem(prob, path):
+    keys = sorted(prob.blocks)
+    arrays = {"L": prob.L, "R": prob.R, "nC": np.array([prob.nC]),
+              "keys": np.array(keys, dtype=np.int64).reshape(-1, 3)}
+    for i, k in enumerate(keys):
+        a, b = prob.blocks[k]
+        arrays[f"a{i}"] = a
+        arrays[f"b{i}"] = b
+    np.savez(path, **arrays)
+
+
+def load_problem(path):
+    from dmrgpp_amd.problem import KronProblem
+
+    d = np.load(path, allow_pickle=False)
+    keys = [tuple(int(v) for v in k) for k in d["keys"]]
+    blocks = {k: (np.asfortranarray(d[f"a{i}"]), np.asfortranarray(d[f"b{i}"]))
+              for i, k in enumerate(keys)}
+    return KronProblem(L=d["L"], R=d["R"], nC=int(d["nC"][0]), blocks=blocks,
+                       name="heisenberg-ladder-32x2-m2048")
+
+
 class Recorder:
     """GPU solver that keeps the last superblock it solved."""
 
@@ -82,6 +112,9 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=6,
                     help="finite-sweep DMRG steps timed per precision mode (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--save-problem", default=None, help="write the superblock to this .npz")
+    ap.add_argument("--load-problem", default=None,
+                    help="read the superblock from this .npz instead of growing it (no sweep)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel on N doubles 5 times (HBM counter "
                          "calibration for rocprofv3 --pmc) and exit")
@@ -111,10 +144,17 @@ def main():
     # ---- setup: infinite-size growth of the 32x2 ladder at m=2048 (untimed)
     t0 = time.perf_counter()
     model, lattice = models.heisenberg(1.0), models.ladder(LX, 2)
-    rec = Recorder(sweep.GpuSolver(precision="fp64"))
     state = {}
-    e_inf, recs_inf = sweep.run(model, lattice, lambda N: (0,), M, [], rec, log=log, state=state)
-    prob = rec.prob
+    if args.load_problem:
+        prob = load_problem(args.load_problem)
+        recs_inf = []
+    else:
+        rec = Recorder(sweep.GpuSolver(precision="fp64"))
+        e_inf, recs_inf = sweep.run(model, lattice, lambda N: (0,), M, [], rec, log=log,
+                                    state=state)
+        prob = rec.prob
+        if args.save_problem:
+            save_problem(prob, args.save_problem)
     t_setup = time.perf_counter() - t0
     log(f"setup: infinite growth {len(recs_inf)} steps in {t_setup:.1f} s; superblock "
         f"{prob.histogram()}")
@@ -131,7 +171,7 @@ def main():
 
     # ---- timed matvecs
     results = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16x3"):
         op.set_precision(prec)
         for _ in range(args.warmup):
             op.apply_Htarget(dx, dy)
@@ -148,59 +188,79 @@ def main():
         log(f"{prec}: {1e3 * dt / args.steps:.3f} ms/matvec (events: stage1 {ms1:.3f} ms, "
             f"stage2 {ms2:.3f} ms, total {mst:.3f} ms)")
     y64 = results["fp64"]["y"]
-    fp32_err = float(np.linalg.norm(results["fp32"]["y"] - y64) / np.linalg.norm(y64))
+    for prec in ("fp32", "bf16x3"):
+        results[prec]["rel_err"] = float(np.linalg.norm(results[prec]["y"] - y64)
+                                         / np.linalg.norm(y64))
 
     # ---- ground state of the same superblock: FP64 vs FP32 + FP64 refinement
     dz = dm.DeviceArray(prob.n)
     op.set_precision("fp64")
     lz = {}
-    for mode, kw in (("fp64", {}), ("fp32+refine", {"precision": "fp32", "refine": 1})):
+    modes = (("fp64", {}), ("fp32+refine", {"precision": "fp32", "refine": 1}),
+             ("bf16x3+refine", {"precision": "bf16x3", "refine": 1}))
+    for mode, kw in modes:
         r = op.lanczos_ground(dx, dz, **kw)
         lz[mode] = r
         log(f"lanczos {mode}: E={r['energy']:.14f} steps={r['steps']} matvecs={r['matvecs']} "
-            f"residual={r['residual']:.2e} {r['ms_total']:.1f} ms")
-    dE = abs(lz["fp32+refine"]["energy"] - lz["fp64"]["energy"])
+            f"first={r['matvecs_first']} residual={r['residual']:.2e} {r['ms_total']:.1f} ms")
+    e64 = lz["fp64"]["energy"]
+    dE = {m: abs(lz[m]["energy"] - e64) for m in lz}
     op.close()
 
     # ---- bounded finite-sweep segment in both modes, continuing from the grown blocks
     sweep_out = None
-    if args.sweep_steps > 0:
+    if args.sweep_steps > 0 and state:
         sweep_out = {}
         steps_per_sweep = 2 * (lattice.nsites - 3)
         for mode, solver in (("fp64", sweep.GpuSolver(precision="fp64")),
-                             ("fp32+refine", sweep.GpuSolver(precision="fp32", refine=1))):
+                             ("fp32+refine", sweep.GpuSolver(precision="fp32", refine=1)),
+                             ("bf16x3+refine", sweep.GpuSolver(precision="bf16x3", refine=1))):
             st = sweep.clone_state(state)
             t = time.perf_counter()
             e, recs = sweep.run(model, lattice, lambda N: (0,), M, [M], solver, log=log,
-                                state=st, resume=True, stop_after=args.sweep_steps)
+                                state=st, resume=True, stop_after=args.sweep_steps, wft=True)
             dt = time.perf_counter() - t
             per = dt / max(1, len(recs))
+            lz_ms = sum(r.ms_lanczos for r in recs) / max(1, len(recs))
             sweep_out[mode] = {"steps": len(recs), "s_per_step": round(per, 4),
+                               "eigensolve_ms_per_step": round(lz_ms, 1),
                                "sweep_s_est": round(per * steps_per_sweep, 2),
                                "lanczos_matvecs": int(sum(r.matvecs for r in recs)),
                                "energy_last": e}
-            log(f"sweep segment {mode}: {len(recs)} steps, {per:.3f} s/step, "
-                f"full sweep ~{per * steps_per_sweep:.1f} s")
+            log(f"sweep segment {mode}: {len(recs)} steps, {per:.3f} s/step "
+                f"(eigensolve {lz_ms:.1f} ms/step), full sweep ~{per * steps_per_sweep:.1f} s")
+        for mode in ("fp32+refine", "bf16x3+refine"):
+            sweep_out[mode]["eigensolve_speedup_vs_fp64"] = round(
+                sweep_out["fp64"]["eigensolve_ms_per_step"]
+                / max(1e-9, sweep_out[mode]["eigensolve_ms_per_step"]), 3)
 
-    r = results["fp64"]
+    def roofline_of(prec):
+        r = results[prec]
+        if r["ms_stage2"] >= r["ms_stage1"]:
+            kname, kms, kflops = "stage2", r["ms_stage2"], info["flops_stage2"]
+        else:
+            kname, kms, kflops = "stage1", r["ms_stage1"], info["flops_stage1"]
+        achieved = kflops / (kms * 1e-3) / 1e12
+        kern = {"fp64": "grouped_gemm_kernel<double,double>",
+                "fp32": "grouped_gemm_kernel<float,float|double>",
+                "bf16x3": "grouped_gemm_bf16x3_kernel"}[prec]
+        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec],
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[prec], 4),
+                "traffic": TRAFFIC.get(prec), "kernel": f"{kern} ({kname})",
+                "ms_per_launch": round(kms, 4), "flops_per_launch": kflops}
+
+    r = results["fp32"]
     value = 1e3 / r["ms_per_step"]
-    if r["ms_stage2"] >= r["ms_stage1"]:
-        kname, kms, kflops = "stage2", r["ms_stage2"], info["flops_stage2"]
-    else:
-        kname, kms, kflops = "stage1", r["ms_stage1"], info["flops_stage1"]
-    achieved = kflops / (kms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS["fp64"],
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["fp64"], 4),
-                "traffic": None, "kernel": f"grouped_gemm_kernel<double,double> ({kname})"}
+    roofline = roofline_of("fp32")
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prob, x)
         log("cpu baseline:", cpu)
 
-    r32 = results["fp32"]
+    e_rel = {m: dE[m] / abs(e64) for m in dE}
     line = {
-        "metric": "superblock matvecs/s (m=2048 Heisenberg 32x2 ladder superblock)",
+        "metric": "superblock matvecs/s + sweep time at m=2048, mixed vs FP64 GPU at |dE|<1e-10",
         "value": round(value, 3),
         "unit": "matvecs/s",
         "n_gpus": 1,
@@ -210,31 +270,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic model run: the superblock of the last infinite-growth step of the "
                 "Heisenberg 32x2 ladder at m=2048 (GPU Lanczos, seed-1234 U[-1,1) start vectors)",
-        "config": {"workload": "heisenberg-ladder-32x2-m2048-superblock-matvec", "n": prob.n,
-                   "npatches": prob.npatches, "nC": prob.nC, "nterms": len(prob.blocks),
-                   "flops_per_matvec": info["flops_algorithmic"]},
+        "config": {"workload": "heisenberg-ladder-32x2-m2048-superblock-matvec",
+                   "precision": "fp32-mfma matvec, fp64 recurrence + fp64 refinement",
+                   "n": prob.n, "npatches": prob.npatches, "nC": prob.nC,
+                   "nterms": len(prob.blocks), "flops_per_matvec": info["flops_algorithmic"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "fp32_mfma": {"matvecs_per_s": round(1e3 / r32["ms_per_step"], 3),
-                      "ms_per_step": round(r32["ms_per_step"], 4),
-                      "rel_err_vs_fp64": fp32_err,
-                      "tflops": round(info["flops_algorithmic"] / (r32["ms_total_events"] * 1e-3)
-                                      / 1e12, 3)},
-        "fp64_tflops_per_matvec": round(info["flops_algorithmic"] / (r["ms_total_events"] * 1e-3)
-                                        / 1e12, 3),
-        "ground_state": {"energy_fp64": lz["fp64"]["energy"],
-                         "energy_mixed": lz["fp32+refine"]["energy"],
-                         "abs_dE": dE, "rel_dE": dE / abs(lz["fp64"]["energy"]),
-                         "gate_1e-10": bool(dE < 1e-10 * abs(lz["fp64"]["energy"])),
-                         "ms_fp64": round(lz["fp64"]["ms_total"], 2),
-                         "ms_mixed": round(lz["fp32+refine"]["ms_total"], 2),
-                         "matvecs_fp64": lz["fp64"]["matvecs"],
-                         "matvecs_mixed": lz["fp32+refine"]["matvecs"],
-                         "residual_fp64": lz["fp64"]["residual"],
-                         "residual_mixed": lz["fp32+refine"]["residual"]},
+        "gate": {"mode": "fp32+refine", "rel_dE": e_rel["fp32+refine"],
+                 "passed": bool(e_rel["fp32+refine"] < 1e-10)},
+        "modes": {prec: {"matvecs_per_s": round(1e3 / results[prec]["ms_per_step"], 3),
+                         "ms_per_step": round(results[prec]["ms_per_step"], 4),
+                         "rel_err_vs_fp64": results[prec].get("rel_err", 0.0),
+                         "tflops_per_matvec": round(info["flops_algorithmic"]
+                                                    / (results[prec]["ms_total_events"] * 1e-3)
+                                                    / 1e12, 3),
+                         "roofline": roofline_of(prec)}
+                  for prec in ("fp64", "fp32", "bf16x3")},
+        "ground_state": {m: {"energy": lz[m]["energy"], "rel_dE_vs_fp64": e_rel[m],
+                             "gate_1e-10": bool(e_rel[m] < 1e-10),
+                             "ms": round(lz[m]["ms_total"], 2), "matvecs": lz[m]["matvecs"],
+                             "matvecs_first": lz[m]["matvecs_first"],
+                             "residual": lz[m]["residual"]} for m in lz},
         "sweep": sweep_out,
         "setup_s": round(t_setup, 1),
     }

@@ -145,6 +145,8 @@ SIGNATURES = {
     "dmk_apply_d": (_I, [_VP, _VP, _VP]),
     "dmk_apply_host_d": (_I, [_VP, _VP, _VP]),
     "dmk_apply_multi_d": (_I, [_VP, _I, _VP, _SZ, _VP, _SZ, _I]),
+    "dmk_set_sector_map": (_I, [_VP, _VP]),
+    "dmk_matvec_sector_d": (_I, [_VP, _VP, _VP]),
     "dmk_set_precision": (_I, [_VP, _I]),
     "dmk_get_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
     "dmk_set_tuning": (_I, [_VP, ctypes.POINTER(Tuning)]),
@@ -454,6 +456,16 @@ class BatchedGemm:
     def apply_Htarget(self, vin: DeviceArray, vout: DeviceArray):
         """vout = H vin on device arrays (BatchedGemm.h:30-31)."""
         _check(lib().dmk_apply_d(self.handle, vin.ptr, vout.ptr), "dmk_apply_d")
+
+    def set_sector_map(self, perm):
+        """perm[ip] = sector index of patch-order element ip (InitKronHamiltonian::copyIn)."""
+        perm = np.ascontiguousarray(perm, dtype=np.int64)
+        assert perm.size == self.n
+        _check(lib().dmk_set_sector_map(self.handle, perm.ctypes.data), "dmk_set_sector_map")
+
+    def matvec_sector(self, vin: DeviceArray, vout: DeviceArray):
+        """vout += H vin in sector order (KronMatrix::matrixVectorProductImpl)."""
+        _check(lib().dmk_matvec_sector_d(self.handle, vin.ptr, vout.ptr), "dmk_matvec_sector_d")
 
     def apply_z(self, vin: DeviceArray, vout: DeviceArray):
         """vout = H vin on interleaved complex device vectors (2n doubles each)."""

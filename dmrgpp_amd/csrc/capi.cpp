@@ -28,17 +28,24 @@ struct dmk_handle {
 	bool                                   is_complex = false; // plan is a real embedding
 	int64_t                                nc         = 0;     // complex vector length
 	double*                                zbuf       = nullptr; // split in / out of dmk_apply_z
+	int64_t*                               smap       = nullptr; // sector map (device)
+	double*                                sbuf       = nullptr; // gathered input, product
 	~dmk_handle()
 	{
-		if (zbuf && plan)
+		if ((zbuf || smap || sbuf) && plan)
 			(void)hipStreamSynchronize(plan->stream());
 		dmk::pool_device_free(zbuf);
+		dmk::pool_device_free(smap);
+		dmk::pool_device_free(sbuf);
 	}
 };
 
 namespace dmk {
 
 void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
+void launch_sector_gather(const double* v, const int64_t* map, int64_t n, double* y, hipStream_t s);
+void launch_sector_scatter_add(const double* x, const int64_t* map, int64_t n, double* v,
+                               hipStream_t s);
 
 namespace {
 thread_local std::string g_last_error;
@@ -189,6 +196,49 @@ int dmk_apply_z(dmk_handle* handle, const double* vin, double* vout)
 		op.nc       = nc;
 		op.apply(xin, xout, 1);
 		dmk::launch_kry_interleave(xout, 2 * nc, vout, nc, nc, 1, p.stream());
+		return DMK_OK;
+	});
+}
+
+int dmk_set_sector_map(dmk_handle* handle, const int64_t* map)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (!map)
+			throw std::invalid_argument("dmk_set_sector_map: null map");
+		if (handle->is_complex)
+			throw std::invalid_argument("dmk_set_sector_map: complex handle");
+		const int64_t     n = p.n();
+		std::vector<char> seen(size_t(n), 0);
+		for (int64_t i = 0; i < n; ++i) {
+			if (map[i] < 0 || map[i] >= n || seen[size_t(map[i])])
+				throw std::invalid_argument("dmk_set_sector_map: not a permutation of 0..n-1 (entry "
+				                            + std::to_string(i) + ")");
+			seen[size_t(map[i])] = 1;
+		}
+		if (!handle->smap)
+			handle->smap = static_cast<int64_t*>(dmk::pool_device_alloc(size_t(n) * sizeof(int64_t)));
+		DMK_HIP_CHECK(hipMemcpyAsync(handle->smap, map, size_t(n) * sizeof(int64_t),
+		                             hipMemcpyHostToDevice, p.stream()));
+		DMK_HIP_CHECK(hipStreamSynchronize(p.stream()));
+		return DMK_OK;
+	});
+}
+
+int dmk_matvec_sector_d(dmk_handle* handle, const double* vin, double* vout)
+{
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (!handle->smap)
+			throw std::invalid_argument("dmk_matvec_sector_d: no sector map (dmk_set_sector_map)");
+		if (!vin || !vout)
+			throw std::invalid_argument("dmk_matvec_sector_d: null vector");
+		const int64_t n = p.n();
+		if (!handle->sbuf)
+			handle->sbuf = static_cast<double*>(dmk::pool_device_alloc(size_t(2 * n) * sizeof(double)));
+		dmk::launch_sector_gather(vin, handle->smap, n, handle->sbuf, p.stream());
+		p.apply(handle->sbuf, n, handle->sbuf + n, n, 1, 0);
+		dmk::launch_sector_scatter_add(handle->sbuf + n, handle->smap, n, vout, p.stream());
 		return DMK_OK;
 	});
 }

@@ -1,0 +1,156 @@
+"""Reader and writer of DMRG++'s Kron-problem dump format (KronLogger,
+dmrg/Engine/MatrixVectorKron/KronLogger.hh:112-174), so that real superblock problems logged by
+any DMRG++ installation can be replayed through the GPU plugin, and problems built here can be
+written for the reference's tools.
+
+The format: a "Vector" line, the vector length, one value per line (the y of one H*y); then per
+visited (outPatch, inPatch, ic) a header comment and the line
+    nc np outPatch inPatch ic applyAdjoint performTranspose offsetX offsetY
+followed by "## A: dense|sparse" and "## B: dense|sparse" blocks. Dense blocks are "rows cols"
+and then one row per line (PsimagLite's Matrix operator<<); sparse blocks are MatrixMarket
+coordinate (1-based "row col value" lines after a "rows cols nnz" line). Complex values are
+written "(re,im)". Entries with performTranspose = 1 hold the stored lower-part blocks
+xc(inPatch, outPatch), yc(inPatch, outPatch) that KronConnections applies transposed.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .problem import KronProblem
+
+
+def _num(tok: str):
+    tok = tok.strip()
+    if tok.startswith("("):
+        re_, im_ = tok.strip("()").split(",")
+        return complex(float(re_), float(im_))
+    return float(tok)
+
+
+def _split(line: str):
+    # Complex values "(a,b)" contain no spaces, so whitespace splitting keeps them whole.
+    return line.split()
+
+
+def read(path: str):
+    """Returns (y, entries); entries are dicts with nc, np, out, in, ic, applyAdjoint,
+    performTranspose, offsetX, offsetY and the A / B blocks as numpy arrays."""
+    with open(path) as fh:
+        lines = [ln.rstrip("\n") for ln in fh]
+    i, y, entries = 0, None, []
+
+    def block(i):
+        head = lines[i]
+        if not (head.startswith("## A:") or head.startswith("## B:")):
+            raise ValueError(f"{path}:{i + 1}: expected a block header, got {head!r}")
+        kind = head.split(":", 1)[1].strip()
+        i += 1
+        if kind == "dense":
+            rows, cols = (int(t) for t in _split(lines[i]))
+            i += 1
+            vals = []
+            for _ in range(rows):
+                row = [_num(t) for t in _split(lines[i])]
+                if len(row) != cols:
+                    raise ValueError(f"{path}:{i + 1}: row of {len(row)} values, want {cols}")
+                vals.append(row)
+                i += 1
+            mat = np.array(vals).reshape(rows, cols) if rows else np.zeros((0, cols))
+        elif kind == "sparse":
+            if not lines[i].startswith("%%MatrixMarket"):
+                raise ValueError(f"{path}:{i + 1}: expected a MatrixMarket header")
+            i += 1
+            while lines[i].startswith("%"):
+                i += 1
+            rows, cols, nnz = (int(t) for t in _split(lines[i]))
+            i += 1
+            trip = []
+            for _ in range(nnz):
+                r, c, v = _split(lines[i])
+                trip.append((int(r) - 1, int(c) - 1, _num(v)))
+                i += 1
+            cplx = any(isinstance(v, complex) for _, _, v in trip)
+            mat = np.zeros((rows, cols), dtype=np.complex128 if cplx else np.float64)
+            for r, c, v in trip:
+                mat[r, c] = v
+        else:
+            raise ValueError(f"{path}:{i}: unknown block format {kind!r}")
+        return i, mat
+
+    while i < len(lines):
+        ln = lines[i].strip()
+        if ln == "Vector":
+            n = int(lines[i + 1])
+            y = np.array([_num(lines[i + 2 + k]) for k in range(n)])
+            i += 2 + n
+            continue
+        if ln.startswith("# nc, total"):
+            f = [int(t) for t in _split(lines[i + 1])]
+            i += 2
+            i, a = block(i)
+            i, b = block(i)
+            entries.append({"nc": f[0], "np": f[1], "out": f[2], "in": f[3], "ic": f[4],
+                            "applyAdjoint": f[5], "performTranspose": f[6], "offsetX": f[7],
+                            "offsetY": f[8], "A": a, "B": b})
+            continue
+        i += 1
+    return y, entries
+
+
+def to_problem(entries: list, name: str = "kronlog") -> KronProblem:
+    """The full plugin-layout problem of a dump: every logged (out, in, ic) block, with the
+    transposed-stored lower-part entries turned back into the (out, in) blocks that
+    KronConnections applies (op = 't')."""
+    if not entries:
+        raise ValueError("no Kron entries")
+    npch, nC = entries[0]["np"], entries[0]["nc"]
+    L, R = [0] * npch, [0] * npch
+    blocks = {}
+    cplx = False
+    for e in entries:
+        A, B = np.asarray(e["A"]), np.asarray(e["B"])
+        cplx |= np.iscomplexobj(A) or np.iscomplexobj(B)
+        o, i, k = e["out"], e["in"], e["ic"]
+        if e["applyAdjoint"]:
+            raise ValueError("adjoint entries are not a plain H*y problem")
+        if e["performTranspose"]:
+            A, B = A.T, B.T
+        blocks[(o, i, k)] = (np.asfortranarray(A), np.asfortranarray(B))
+        L[o], R[o] = A.shape[0], B.shape[0]
+        L[i], R[i] = A.shape[1], B.shape[1]
+    if min(L) < 1 or min(R) < 1:
+        raise ValueError("a patch has no logged block; its sizes are unknown")
+    dt = np.complex128 if cplx else np.float64
+    return KronProblem(L=L, R=R, nC=nC, blocks={k: (a.astype(dt), b.astype(dt))
+                                                for k, (a, b) in blocks.items()},
+                       dtype=dt, name=name)
+
+
+def _fmt(v) -> str:
+    if isinstance(v, complex) or np.iscomplexobj(v):
+        v = complex(v)
+        return f"({v.real!r},{v.imag!r})"
+    return repr(float(v))
+
+
+def write(path: str, prob: KronProblem, y: np.ndarray | None = None):
+    """Writes the problem (every block dense, no transposed storage) and y in the dump
+    format."""
+    off = prob.offsets
+    with open(path, "w") as fh:
+        fh.write("## KronLogger-format dump written by dmrgpp_amd.kronlog\n")
+        if y is not None:
+            fh.write("Vector\n")
+            fh.write(f"{len(y)}\n")
+            for v in y:
+                fh.write(_fmt(v) + "\n")
+        for (o, i, k) in sorted(prob.blocks, key=lambda t: (t[0], t[1], t[2])):
+            A, B = prob.blocks[(o, i, k)]
+            fh.write("\n# nc, total, outPatch, inPatch, ic, applyAdjoint, performTranspose, "
+                     "offsetX, offsetY\n")
+            fh.write(f"{prob.nC} {prob.npatches} {o} {i} {k} 0 0 {off[o]} {off[i]}\n")
+            for tag, M in (("A", A), ("B", B)):
+                fh.write(f"## {tag}: dense\n")
+                fh.write(f"{M.shape[0]} {M.shape[1]}\n")
+                for r in range(M.shape[0]):
+                    fh.write(" ".join(_fmt(v) for v in M[r]) + "\n")

@@ -120,6 +120,17 @@ int dmk_apply_host_d(dmk_handle* handle, const double* vin, double* vout);
  * (real H times a complex vector: real and imaginary parts go through one two-column launch). */
 int dmk_apply_z(dmk_handle* handle, const double* vin, double* vout);
 
+/* Sector ordering of the caller's vectors: map[ip] (host, n entries, a permutation of 0..n-1)
+ * is the sector index of patch-order element ip, i.e. permutationInverse[i + j*nl] - offset for
+ * the ip = vstart[patch] + iright + ileft*sizeRight of InitKronHamiltonian::copyIn
+ * (InitKronHamiltonian.h:158-221). Uploaded once per handle. */
+int dmk_set_sector_map(dmk_handle* handle, const int64_t* map);
+
+/* vout += H vin on sector-ordered device vectors (KronMatrix::matrixVectorProductImpl with the
+ * plugin enabled, KronMatrix.h:137-151: copyIn, apply, xout += tmp, copyOut), with the gather
+ * and scatter on the device instead of the host. Needs dmk_set_sector_map. Asynchronous. */
+int dmk_matvec_sector_d(dmk_handle* handle, const double* vin, double* vout);
+
 /* nvec right-hand sides in one launch pair: vout[v*ldout..] = H vin[v*ldin..] (beta = 0) or
  * += (beta = 1). No reference counterpart (ParallelTriDiag.h:132-161 tridiagonalizes one vector
  * at a time); used by the batched Krylov time evolution. */

@@ -277,3 +277,29 @@ def test_set_tuning_rejects_bad_shapes():
     t = dm.Tuning(stage1_tile_m=96, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64,
                   splitk=0, xcd_order=0)
     assert dm.lib().dmk_set_tuning(None, ctypes.byref(t)) == -1
+
+
+def test_sector_map_rejects_non_permutations():
+    assert dm.lib().dmk_set_sector_map(None, None) == -1
+
+
+@pytest.mark.gpu
+def test_gpu_sector_ordered_matvec_accumulates():
+    # KronMatrix semantics: vout += P^T H P vin with P the copyIn gather.
+    prob = problems.random_problem([5, 9, 2, 13], [7, 3, 11, 4], nC=3, density=0.7, seed=12)
+    rng = np.random.default_rng(12)
+    perm = rng.permutation(prob.n)
+    vin = rng.uniform(-1, 1, prob.n)
+    vout0 = rng.uniform(-1, 1, prob.n)
+    want = vout0.copy()
+    want[perm] += oracle.plugin_apply(prob, vin[perm])
+    op = dm.BatchedGemm(prob)
+    op.set_sector_map(perm)
+    din, dout = dm.DeviceArray.from_host(vin), dm.DeviceArray.from_host(vout0)
+    op.matvec_sector(din, dout)
+    op.sync()
+    assert rel_err(dout.download(), want) < 1e-13
+    bad = perm.copy()
+    bad[0] = bad[1]
+    with pytest.raises(dm.DmkError):
+        op.set_sector_map(bad)
