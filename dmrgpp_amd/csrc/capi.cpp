@@ -311,6 +311,8 @@ void to_c(const dmk::Tuning& t, dmk_tuning* c)
 	c->stage2_tile_n = t.bn2;
 	c->splitk        = t.splitk;
 	c->xcd_order     = t.xcd;
+	c->mfma32        = t.mfma32;
+	c->pad           = 0;
 }
 } // namespace
 
@@ -336,6 +338,7 @@ int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning)
 		t.bn2    = tuning->stage2_tile_n;
 		t.splitk = tuning->splitk;
 		t.xcd    = tuning->xcd_order;
+		t.mfma32 = tuning->mfma32;
 		plan_of(handle).retune(t);
 		return DMK_OK;
 	});

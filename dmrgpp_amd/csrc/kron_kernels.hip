@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdexcept>
+#include <type_traits>
 
 namespace dmk {
 
@@ -32,7 +33,13 @@ template <typename T> struct MfmaOp;
 // Operand layouts of the 16x16x4 MFMAs (both types): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]. Result layouts differ (checked on gfx950): f64 holds D[l/16 + 4r][l%16] in
 // register r, f32 holds D[4(l/16) + r][l%16].
+// kM x kM output block and kK-deep contraction per instruction: lane l supplies operand
+// A[l % kM][l / kM] and B[l / kM][l % kM]; register r of the result holds row row(l, r),
+// column l % kM.
 template <> struct MfmaOp<double> {
+	static constexpr int kM    = 16;
+	static constexpr int kK    = 4;
+	static constexpr int kRegs = 4;
 	using acc_t = double __attribute__((ext_vector_type(4)));
 	__device__ static inline acc_t mma(double a, double b, acc_t c)
 	{
@@ -42,12 +49,32 @@ template <> struct MfmaOp<double> {
 };
 
 template <> struct MfmaOp<float> {
+	static constexpr int kM    = 16;
+	static constexpr int kK    = 4;
+	static constexpr int kRegs = 4;
 	using acc_t = float __attribute__((ext_vector_type(4)));
 	__device__ static inline acc_t mma(float a, float b, acc_t c)
 	{
 		return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 	}
 	__device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// FP32 with 32x32x2 blocks: twice the flops per LDS operand read of the 16x16x4 form.
+// Result register r of lane l holds row 8(r/4) + 4(l/32) + r%4 (checked on gfx950).
+struct MfmaF32x32 {
+	static constexpr int kM    = 32;
+	static constexpr int kK    = 2;
+	static constexpr int kRegs = 16;
+	using acc_t = float __attribute__((ext_vector_type(16)));
+	__device__ static inline acc_t mma(float a, float b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r)
+	{
+		return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+	}
 };
 
 template <typename T> struct Bases {
@@ -132,16 +159,15 @@ __device__ inline bool next_chunk(const Seg* __restrict__ segs, int seg_end, int
 // T: operand/accumulate type; TO: output element type (the stage-2 output of the FP32 mode is
 // written in FP64 for the FP64 Lanczos recurrence). BM x BN output tile per workgroup, each of
 // the 4 waves owning a (BM/2) x (BN/2) sub-tile of (BM/32) x (BN/32) MFMA 16x16 blocks.
-template <typename T, typename TO, int BM, int BN>
+template <typename T, typename TO, int BM, int BN, typename Op = MfmaOp<T>>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
     const Task* __restrict__ tasks, const Seg* __restrict__ segs, const Tile* __restrict__ tiles,
     Bases<T> bases, int64_t xstride, int64_t bxstride, TO* __restrict__ out_y, int64_t ystride,
     T* __restrict__ out_ws, int64_t wsstride, int beta)
 {
-	using Op    = MfmaOp<T>;
 	using acc_t = typename Op::acc_t;
 	constexpr int WM = BM / 2, WN = BN / 2;
-	constexpr int FI = WM / 16, FJ = WN / 16;
+	constexpr int FI = WM / Op::kM, FJ = WN / Op::kM;
 	static_assert(BM * kBK % kThreads == 0 && BN * kBK % kThreads == 0, "tile / thread mismatch");
 
 	__shared__ T As[2][kBK][BM + kPad];
@@ -176,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 	for (int i = 0; i < FI; ++i)
 #pragma unroll
 		for (int j = 0; j < FJ; ++j)
-			acc[i][j] = acc_t { 0, 0, 0, 0 };
+			acc[i][j] = acc_t {};
 
 	int seg = tk.seg_begin;
 	int k0  = 0;
@@ -202,15 +228,15 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 				                      tl.m0, tl.n0, tk.m, tk.n, nk0, tid);
 			}
 #pragma unroll
-			for (int kk = 0; kk < kBK; kk += 4) {
-				const int kr = kk + (lane >> 4);
+			for (int kk = 0; kk < kBK; kk += Op::kK) {
+				const int kr = kk + lane / Op::kM;
 				T         a[FI], b[FJ];
 #pragma unroll
 				for (int i = 0; i < FI; ++i)
-					a[i] = As[buf][kr][wm + 16 * i + (lane & 15)];
+					a[i] = As[buf][kr][wm + Op::kM * i + lane % Op::kM];
 #pragma unroll
 				for (int j = 0; j < FJ; ++j)
-					b[j] = Bs[buf][kr][wn + 16 * j + (lane & 15)];
+					b[j] = Bs[buf][kr][wn + Op::kM * j + lane % Op::kM];
 #pragma unroll
 				for (int i = 0; i < FI; ++i)
 #pragma unroll
@@ -227,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 		}
 	}
 
-	// Epilogue: register r of lane l holds row Op::row(l, r), column l%16 of each 16x16 block.
+	// Epilogue: register r of lane l holds row Op::row(l, r), column l % kM of each block.
 	// Tasks writing the caller's vector convert to TO (and accumulate when beta); workspace
 	// tasks (stage-1 panels, split-k partials) store the compute type and always overwrite.
 	const bool to_y = tk.out_is_y != 0;
@@ -236,9 +262,9 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 #pragma unroll
 		for (int j = 0; j < FJ; ++j)
 #pragma unroll
-			for (int r = 0; r < 4; ++r) {
-				const int row = tl.m0 + wm + 16 * i + Op::row(lane, r);
-				const int col = tl.n0 + wn + 16 * j + (lane & 15);
+			for (int r = 0; r < Op::kRegs; ++r) {
+				const int row = tl.m0 + wm + Op::kM * i + Op::row(lane, r);
+				const int col = tl.n0 + wn + Op::kM * j + lane % Op::kM;
 				if (row < tk.m && col < tk.n) {
 					const int     rr   = row % tk.rsplit;
 					const int     blk  = row / tk.rsplit;
@@ -287,21 +313,17 @@ __global__ void cast_f64_to_f32_kernel(const double* __restrict__ x, float* __re
 
 // Host-side launchers ------------------------------------------------------------------------
 
-template <typename T, typename TO>
-void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
-                         int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
-                         TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
-                         int beta, hipStream_t stream)
+namespace {
+template <typename T, typename TO, typename Op>
+void launch_grouped_gemm_op(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
+                            int ntiles, const Bases<T>& b, int64_t xstride, int64_t bxstride,
+                            TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
+                            int beta, hipStream_t stream)
 {
-	if (ntiles <= 0 || nvec <= 0)
-		return;
-	Bases<T> b;
-	for (int i = 0; i < BASE_COUNT; ++i)
-		b.p[i] = bases[i];
 	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
 #define DMK_LAUNCH(BM, BN)                                                                          \
-	hipLaunchKernelGGL((grouped_gemm_kernel<T, TO, BM, BN>), grid, dim3(kThreads), 0, stream,  \
-	                   tasks, segs, tiles, b, xstride, bxstride, out_y, ystride, out_ws,          \
+	hipLaunchKernelGGL((grouped_gemm_kernel<T, TO, BM, BN, Op>), grid, dim3(kThreads), 0,      \
+	                   stream, tasks, segs, tiles, b, xstride, bxstride, out_y, ystride, out_ws,  \
 	                   wsstride, beta)
 	if (bm == 64 && bn == 64)
 		DMK_LAUNCH(64, 64);
@@ -314,6 +336,32 @@ void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, con
 	else
 		throw std::runtime_error("grouped gemm: unsupported tile shape");
 #undef DMK_LAUNCH
+}
+} // namespace
+
+// variant 1 selects the 32x32x2 MFMA form (FP32 only).
+template <typename T, typename TO>
+void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
+                         int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
+                         TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
+                         int beta, hipStream_t stream, int variant)
+{
+	if (ntiles <= 0 || nvec <= 0)
+		return;
+	Bases<T> b;
+	for (int i = 0; i < BASE_COUNT; ++i)
+		b.p[i] = bases[i];
+	if constexpr (std::is_same<T, float>::value) {
+		if (variant == 1) {
+			launch_grouped_gemm_op<T, TO, MfmaF32x32>(bm, bn, tasks, segs, tiles, ntiles, b,
+			                                          xstride, bxstride, out_y, ystride, out_ws,
+			                                          wsstride, nvec, beta, stream);
+			return;
+		}
+	}
+	launch_grouped_gemm_op<T, TO, MfmaOp<T>>(bm, bn, tasks, segs, tiles, ntiles, b, xstride,
+	                                         bxstride, out_y, ystride, out_ws, wsstride, nvec,
+	                                         beta, stream);
 }
 
 template <typename T, typename TO>
@@ -331,7 +379,7 @@ void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchu
 #define DMK_INST(T, TO)                                                                            \
 	template void launch_grouped_gemm<T, TO>(int, int, const Task*, const Seg*, const Tile*, int, \
 	                                         const T* const*, int64_t, int64_t, TO*, int64_t, T*,  \
-	                                         int64_t, int, int, hipStream_t);                     \
+	                                         int64_t, int, int, hipStream_t, int);                \
 	template void launch_splitk_reduce<T, TO>(const Reduce*, const ReduceChunk*, int, const T*,   \
 	                                          int64_t, TO*, int64_t, int, int, hipStream_t);
 DMK_INST(double, double)

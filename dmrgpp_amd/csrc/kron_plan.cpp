@@ -35,7 +35,7 @@ template <typename T, typename TO>
 void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
                          int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
                          TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
-                         int beta, hipStream_t stream);
+                         int beta, hipStream_t stream, int variant = 0);
 template <typename T, typename TO>
 void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchunks, const T* ws,
                           int64_t wsstride, TO* out, int64_t ystride, int nvec, int beta,
@@ -631,6 +631,8 @@ void Plan::retune(const Tuning& t)
 	};
 	if (!valid(t.bm1, t.bn1) || !valid(t.bm2, t.bn2) || t.splitk < 0)
 		throw std::invalid_argument("dmk tuning: tile shapes must be 64 or 128, splitk >= 0");
+	if (t.mfma32 != 0 && t.mfma32 != 1)
+		throw std::invalid_argument("dmk tuning: mfma32 must be 0 or 1");
 	if (stream_)
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
 	tuning_ = t;
@@ -863,12 +865,13 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		const float* bases[BASE_COUNT] = { d_apack32_, d_bpack32_, d_x32_, bx };
 		launch_grouped_gemm<float, float>(st1_.bm, st1_.bn, st1_.d_tasks, st1_.d_segs,
 		                                  st1_.d_tiles, int(st1_.tiles.size()), bases, stats_.n,
-		                                  bxs, bx, bxs, bx, bxs, nvec, 0, stream_);
+		                                  bxs, bx, bxs, bx, bxs, nvec, 0, stream_, tuning_.mfma32);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
 		launch_grouped_gemm<float, double>(st2_.bm, st2_.bn, st2_.d_tasks, st2_.d_segs,
 		                                   st2_.d_tiles, int(st2_.tiles.size()), bases, stats_.n,
-		                                   bxs, vout, ldout, ws, wss, nvec, beta, stream_);
+		                                   bxs, vout, ldout, ws, wss, nvec, beta, stream_,
+		                                   tuning_.mfma32);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                    vout, ldout, nvec, beta, stream_);
 	}
@@ -921,17 +924,20 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		throw std::invalid_argument("dmk_autotune: iters must be >= 1");
 	const Tuning start = tuning_;
 	std::vector<Tuning> cands;
-	for (int bm1 : { 64, 128 })
-		for (int bm2 : { 64, 128 })
-			for (int sk : { 0, 512, 1024, 2048 }) {
-				Tuning t;
-				t.bm1    = bm1;
-				t.bn1    = 64;
-				t.bm2    = bm2;
-				t.bn2    = 64;
-				t.splitk = sk;
-				cands.push_back(t);
-			}
+	const int nvar = (precision_ == PREC_FP32) ? 2 : 1; // FP32: 16x16x4 and 32x32x2 MFMA
+	for (int var = 0; var < nvar; ++var)
+		for (int bm1 : { 64, 128 })
+			for (int bm2 : { 64, 128 })
+				for (int sk : { 0, 512, 1024, 2048 }) {
+					Tuning t;
+					t.bm1    = bm1;
+					t.bn1    = 64;
+					t.bm2    = bm2;
+					t.bn2    = 64;
+					t.splitk = sk;
+					t.mfma32 = var;
+					cands.push_back(t);
+				}
 	Tuning best     = start;
 	double best_ms  = 1e300;
 	for (const Tuning& t : cands) {

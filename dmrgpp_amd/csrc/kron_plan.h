@@ -52,6 +52,7 @@ struct Tuning {
 	int bn2    = 64;
 	int splitk = 1024;
 	int xcd    = 0; // 1: XCD-aware tile order
+	int mfma32 = 0; // 1: FP32 mode on 32x32x2 MFMA blocks
 };
 
 struct StageTables {

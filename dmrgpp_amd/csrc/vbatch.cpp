@@ -27,7 +27,7 @@ template <typename T, typename TO>
 void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
                          int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
                          TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
-                         int beta, hipStream_t stream);
+                         int beta, hipStream_t stream, int variant = 0);
 
 namespace {
 

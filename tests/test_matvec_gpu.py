@@ -211,13 +211,15 @@ def test_gpu_multi_vector_and_accumulate():
 
 TUNINGS = [
     dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=0,
-         xcd_order=0),
+         xcd_order=0, mfma32=0),
     dict(stage1_tile_m=128, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=128, splitk=16,
-         xcd_order=0),
+         xcd_order=0, mfma32=1),
     dict(stage1_tile_m=64, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=128, splitk=32,
-         xcd_order=1),
+         xcd_order=1, mfma32=1),
     dict(stage1_tile_m=128, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=64, splitk=48,
-         xcd_order=1),
+         xcd_order=1, mfma32=0),
+    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=1024,
+         xcd_order=0, mfma32=1),
 ]
 
 
@@ -225,8 +227,9 @@ TUNINGS = [
 @pytest.mark.parametrize("tuning", TUNINGS)
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-4)])
 def test_gpu_tunings_match_oracle(tuning, prec, tol):
-    # Every tile shape, forced split-k (pieces of 16-48 along K, ragged last piece) and both
-    # workgroup orders give the oracle's result; the same tuning twice is bitwise identical.
+    # Every tile shape, forced split-k (pieces of 16-48 along K, ragged last piece), both
+    # workgroup orders and both FP32 MFMA forms give the oracle's result; the same tuning twice
+    # is bitwise identical.
     left = {-2: 70, 0: 130, 2: 70}
     prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(2), seed=21)
     rand = problems.random_problem([3, 70, 5, 130], [65, 2, 64, 9], 3, density=0.7, seed=2)
