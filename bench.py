@@ -68,25 +68,15 @@ def cpu_baseline(prob, x, budget_s=15.0, max_calls=20):
 
 
 def save_problem(prob, path):
-    keys = sorted(prob.blocks)
-    arrays = {"L": prob.L, "R": prob.R, "nC": np.array([prob.nC]),
-              "keys": np.array(keys, dtype=np.int64).reshape(-1, 3)}
-    for i, k in enumerate(keys):
-        a, b = prob.blocks[k]
-        arrays[f"a{i}"] = a
-        arrays[f"b{i}"] = b
-    np.savez(path, **arrays)
+    from dmrgpp_amd.problem import save_npz
+
+    save_npz(prob, path)
 
 
 def load_problem(path):
-    from dmrgpp_amd.problem import KronProblem
+    from dmrgpp_amd.problem import load_npz
 
-    d = np.load(path, allow_pickle=False)
-    keys = [tuple(int(v) for v in k) for k in d["keys"]]
-    blocks = {k: (np.asfortranarray(d[f"a{i}"]), np.asfortranarray(d[f"b{i}"]))
-              for i, k in enumerate(keys)}
-    return KronProblem(L=d["L"], R=d["R"], nC=int(d["nC"][0]), blocks=blocks,
-                       name="heisenberg-ladder-32x2-m2048")
+    return load_npz(path, name="heisenberg-ladder-32x2-m2048")
 
 
 class Recorder:

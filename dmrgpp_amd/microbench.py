@@ -1,0 +1,67 @@
+"""Driver-free matvec microbenchmarks on one GPU:
+
+  (i)   the plugin benchmark's shape (GPUPlugin bench_BatchedGemm.cpp:195-245): 40 patches,
+        34 of size 60 and 6 of size 900, one operator, block diagonal, U(-1,1) values;
+  (ii)  the plugin test driver's shape (GPUPlugin/test_vbatch.cpp): comb patch sizes for 8+8
+        sites at 8 up / 8 down, keep 8000 / 2000 states, two operators, every block dense,
+        X[i] = (i + 1) / n;
+  (iii) a replay of a saved superblock (bench.py --save-problem), e.g. the m=2048 ladder.
+
+  python -m dmrgpp_amd.microbench [--replay F.npz] [--iters 30]
+
+Each case reports the average matvec time from HIP events (one warm-up, then iters calls) and
+the algorithmic TFLOP/s in FP64, FP32 and BF16x3.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import numpy as np
+
+from . import problems
+
+
+def run_case(name, prob, x, iters, log):
+    from . import BatchedGemm, DeviceArray
+
+    op = BatchedGemm(prob, "fp64")
+    info = op.info()
+    dx, dy = DeviceArray.from_host(x), DeviceArray(prob.n)
+    out = {"case": name, "n": prob.n, "npatches": prob.npatches, "nterms": info["nterms"],
+           "flops": info["flops_algorithmic"]}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        op.set_precision(prec)
+        op.profile(dx, dy, 1)
+        _, _, ms = op.profile(dx, dy, iters)
+        out[prec] = {"ms": round(ms, 4), "tflops": round(info["flops_algorithmic"] / (ms * 1e-3)
+                                                         / 1e12, 2)}
+        log(f"{name:28s} {prec:7s} {ms:8.3f} ms  {out[prec]['tflops']:7.2f} TF/s")
+    op.close()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--replay", default=None, help="superblock .npz written by bench.py")
+    ap.add_argument("--iters", type=int, default=30)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    res = []
+    p1 = problems.block_diagonal_problem()
+    res.append(run_case("bench_BatchedGemm-40-patches", p1,
+                        np.random.default_rng(42).uniform(-1, 1, p1.n), a.iters, log))
+    p2 = problems.comb_problem()
+    res.append(run_case("test_vbatch-comb-8+8", p2, (np.arange(p2.n) + 1.0) / p2.n, a.iters, log))
+    if a.replay:
+        from .problem import load_npz
+
+        p3 = load_npz(a.replay, name="superblock")
+        res.append(run_case("replay:" + p3.name, p3,
+                            np.random.default_rng(1234).uniform(-1, 1, p3.n), a.iters, log))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

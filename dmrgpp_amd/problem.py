@@ -86,6 +86,29 @@ class KronProblem:
         return y
 
 
+def save_npz(prob: KronProblem, path: str):
+    """Writes the problem (patch sizes, nC, every block) to an .npz file."""
+    keys = sorted(prob.blocks)
+    arrays = {"L": prob.L, "R": prob.R, "nC": np.array([prob.nC]),
+              "keys": np.array(keys, dtype=np.int64).reshape(-1, 3)}
+    for i, k in enumerate(keys):
+        a, b = prob.blocks[k]
+        arrays[f"a{i}"] = a
+        arrays[f"b{i}"] = b
+    np.savez(path, **arrays)
+
+
+def load_npz(path: str, name: str = "") -> KronProblem:
+    """Reads a problem written by save_npz (no pickled objects)."""
+    d = np.load(path, allow_pickle=False)
+    keys = [tuple(int(v) for v in k) for k in d["keys"]]
+    blocks = {k: (np.asfortranarray(d[f"a{i}"]), np.asfortranarray(d[f"b{i}"]))
+              for i, k in enumerate(keys)}
+    dt = np.complex128 if any(np.iscomplexobj(a) for a, _ in blocks.values()) else np.float64
+    return KronProblem(L=d["L"], R=d["R"], nC=int(d["nC"][0]), blocks=blocks, dtype=dt,
+                       name=name or path)
+
+
 class PluginArgs:
     """ctypes views of a KronProblem in the BatchedGemm argument layout. Keeps the arrays alive."""
 

@@ -178,3 +178,61 @@ def hermitian_problem(L, R, nC: int, density: float = 0.5, seed: int = 0,
         blocks[(J, I, k + nC)] = (np.conj(a.T), np.conj(b.T))
     return KronProblem(L=base.L, R=base.R, nC=2 * nC, blocks=blocks, dtype=dtype,
                        name="hermitian")
+
+
+def comb_patch_sizes(left_size: int = 8, right_size: int = 8, target_up: int = 8,
+                     target_down: int = 8, keep_left: int = 8000, keep_right: int = 2000):
+    """Patch sizes estimated from state counting, as the plugin's own test driver builds them
+    (GPUPlugin/gen_patches_comb.cpp): a patch per (n_up, n_down) split of the target between
+    left and right; each side's share of kept states is proportional to
+    C(size, n_up) C(size, n_down) among the valid splits, rounded; patches with an empty side
+    are dropped."""
+    from math import comb
+
+    max_lu, max_ld = min(left_size, target_up), min(left_size, target_down)
+    max_ru, max_rd = min(right_size, target_up), min(right_size, target_down)
+    valid = [(nu, nd) for nd in range(max_ld + 1) for nu in range(max_lu + 1)
+             if target_up - nu <= max_ru and target_down - nd <= max_rd
+             and target_up - nu >= 0 and target_down - nd >= 0]
+    lw = {(nu, nd): comb(left_size, nu) * comb(left_size, nd) for nu, nd in valid}
+    rw = {(target_up - nu, target_down - nd): comb(right_size, target_up - nu)
+          * comb(right_size, target_down - nd) for nu, nd in valid}
+    tl, tr = float(sum(lw.values())), float(sum(rw.values()))
+    L, R = [], []
+    for nu, nd in valid:
+        ls = int(np.floor(lw[(nu, nd)] / tl * keep_left + 0.5))
+        rs = int(np.floor(rw[(target_up - nu, target_down - nd)] / tr * keep_right + 0.5))
+        if ls >= 1 and rs >= 1:
+            L.append(ls)
+            R.append(rs)
+    return L, R
+
+
+def comb_problem(noperator: int = 2, left_size: int = 8, keep: int = 2000) -> KronProblem:
+    """The plugin test driver's problem shape (GPUPlugin/test_vbatch.cpp:35-240): comb patch
+    sizes with keep_left = 4 keep, keep_right = keep, every (I, J, k) block dense, with the
+    values of GPUPlugin/setup_matrix.cpp: entry (ia, ja) of operator k of the assembled
+    left (right) matrix is ia + (ja - 1) * total + (k - 1) * total^2 (1-based, total = the
+    summed patch sizes of that side)."""
+    L, R = comb_patch_sizes(left_size, left_size, left_size, left_size, 4 * keep, keep)
+    npch = len(L)
+
+    def side(sizes):
+        start = np.concatenate([[0], np.cumsum(sizes)])
+        total = int(start[-1])
+        return start, total
+
+    ls, lt = side(L)
+    rs, rt = side(R)
+    blocks = {}
+    for k in range(noperator):
+        for J in range(npch):
+            for I in range(npch):
+                ia = np.arange(ls[I], ls[I + 1])[:, None] + 1.0
+                ja = np.arange(ls[J], ls[J + 1])[None, :] + 1.0
+                a = ia + (ja - 1.0) * lt + float(k) * lt * lt
+                ib = np.arange(rs[I], rs[I + 1])[:, None] + 1.0
+                jb = np.arange(rs[J], rs[J + 1])[None, :] + 1.0
+                b = ib + (jb - 1.0) * rt + float(k) * rt * rt
+                blocks[(I, J, k)] = (np.asfortranarray(a), np.asfortranarray(b))
+    return KronProblem(L=L, R=R, nC=noperator, blocks=blocks, name="test_vbatch-comb")

@@ -204,3 +204,14 @@ def test_krylov_oracle_unitary_and_time_reversal():
     assert abs(np.linalg.norm(out[0]) - np.linalg.norm(phi)) < 1e-10 * np.linalg.norm(phi)
     back, _ = oracle.krylov(full, out[0], [0.1], Eg=-3.0, time_direction=-1.0)
     assert np.linalg.norm(back[0] - phi) < 1e-9 * np.linalg.norm(phi)
+
+
+def test_comb_patch_sizes_follow_state_counting():
+    # GPUPlugin/gen_patches_comb.cpp for the test_vbatch defaults (8+8 sites, 8 up 8 down,
+    # keep 8000 / 2000): every (n_up, n_down) split is valid; sizes are the rounded shares of
+    # C(8, n_up) C(8, n_down) / 2^16, so the largest left patch is round(70*70/65536*8000).
+    from dmrgpp_amd import problems
+    L, R = problems.comb_patch_sizes()
+    assert max(L) == round(70 * 70 / 65536 * 8000) and max(R) == round(70 * 70 / 65536 * 2000)
+    assert abs(sum(L) - 8000) < 100 and abs(sum(R) - 2000) < 60
+    assert len(L) == len(R) and all(v >= 1 for v in L + R)
