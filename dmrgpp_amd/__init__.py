@@ -469,8 +469,10 @@ class BatchedGemm:
         _check(lib().dmk_matvec_sector_d(self.handle, vin.ptr, vout.ptr), "dmk_matvec_sector_d")
 
     def apply_z(self, vin: DeviceArray, vout: DeviceArray):
-        """vout = H vin on interleaved complex device vectors (2n doubles each)."""
+        """vout = H vin on interleaved complex device vectors (2n doubles each); waits for the
+        result (the C call itself is asynchronous on the handle's stream)."""
         _check(lib().dmk_apply_z(self.handle, vin.ptr, vout.ptr), "dmk_apply_z")
+        self.sync()
 
     def krylov_evolve(self, phi: DeviceArray, out: DeviceArray, nvec: int, times,
                       ld: int | None = None, **kw):

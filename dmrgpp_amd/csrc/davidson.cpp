@@ -4,9 +4,9 @@
 // Solver.h:154-158); this is the learned-preconditioner variant requested beside the Lanczos
 // path. Each iteration: Rayleigh-Ritz on the basis V (projected matrix G = V^T H V, kept on the
 // host in FP64, diagonalized by Jacobi), Ritz vector u and residual r = H u - theta u in one
-// fused pass, a correction t from r (diagonal: r / (theta - diag H); learned: the MLP evaluated
-// on per-element features), two rounds of classical Gram-Schmidt against V, and one matvec for
-// the new basis vector. When the basis is full it restarts from the Ritz vector and the previous
+// fused pass, a correction t from r (diagonal: Olsen's M^-1 (eps u - r) with M = diag H - theta;
+// learned: the MLP evaluated on per-element features), two rounds of classical Gram-Schmidt
+// against V, and one matvec for the new basis vector. When the basis is full it restarts from the Ritz vector and the previous
 // Ritz vector. The stop test is the residual norm of the unit Ritz vector, and the reported
 // residual is recomputed with a fresh FP64 matvec.
 //
@@ -54,6 +54,9 @@ void launch_residual(const double* hx, const double* x, double e, int64_t n, dou
                      double* out, hipStream_t s);
 void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
 void launch_fill(double* x, double v, int64_t n, hipStream_t s);
+void launch_precond_olsen(const double* r, const double* u, const double* D, double theta,
+                          double floor_, int64_t n, double* partial, double* sums, double* t,
+                          hipStream_t s);
 
 double dense_lowest(const std::vector<double>& G, int k, std::vector<double>& vec)
 {
@@ -271,7 +274,8 @@ DavidsonReport davidson_ground(Plan& plan, const double* init, double* zout,
 			if (p.precond == PRECOND_NONE)
 				launch_copy(t, r, n, s);
 			else if (p.precond == PRECOND_DIAG)
-				launch_precond_diag(r, D, theta, 1e-8 * std::max(1.0, std::fabs(theta)), n, t, s);
+				launch_precond_olsen(r, u, D, theta, 1e-8 * std::max(1.0, std::fabs(theta)), n, part,
+				                     MISC + 2, t, s);
 			else
 				launch_precond_mlp(mlp->params(), r, u, D, theta, sD, rms_r, rms_u, n, t, s);
 			DMK_HIP_CHECK(hipEventRecord(e1, s));

@@ -12,7 +12,7 @@ namespace dmk {
 enum Precond : int
 {
 	PRECOND_NONE = 0, // t = r
-	PRECOND_DIAG = 1, // t = r / (theta - diag(H))
+	PRECOND_DIAG = 1, // Olsen: t = M^-1 (eps u - r), M = diag(H) - theta, t orthogonal to u
 	PRECOND_MLP  = 2, // t = MLP(features) * rms(r) / s_D
 };
 

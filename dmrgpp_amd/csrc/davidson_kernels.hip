@@ -184,6 +184,61 @@ __global__ __launch_bounds__(kDT) void precond_diag_kernel(const double* __restr
 	}
 }
 
+// Olsen's form of the diagonal correction, t = M^-1 (eps u - r) with M = diag(H) - theta and
+// eps = (u . M^-1 r) / (u . M^-1 u), which makes t orthogonal to u (Davidson's plain
+// M^-1 r can stall when M is close to a multiple of the identity). Pass 1: partial sums of
+// u . M^-1 r (slot 0) and u . M^-1 u (slot 1); |M| is kept at least floor.
+__global__ __launch_bounds__(kDT) void olsen_dots_kernel(const double* __restrict__ r,
+                                                         const double* __restrict__ u,
+                                                         const double* __restrict__ D,
+                                                         double theta, double floor_, int64_t n,
+                                                         double* __restrict__ partial)
+{
+	double a = 0, b = 0;
+	for (int64_t i = int64_t(blockIdx.x) * kDT + threadIdx.x; i < n; i += int64_t(kDBlk) * kDT) {
+		double d = D[i] - theta;
+		if (fabs(d) < floor_)
+			d = (d < 0) ? -floor_ : floor_;
+		a += u[i] * r[i] / d;
+		b += u[i] * u[i] / d;
+	}
+	a = dv_block_sum(a);
+	b = dv_block_sum(b);
+	if (threadIdx.x == 0) {
+		partial[blockIdx.x]         = a;
+		partial[kDBlk + blockIdx.x] = b;
+	}
+}
+
+// Pass 2: t = (eps u - r) / M with eps = sums[0] / sums[1] (0 when the denominator is 0).
+__global__ __launch_bounds__(kDT) void olsen_apply_kernel(const double* __restrict__ r,
+                                                          const double* __restrict__ u,
+                                                          const double* __restrict__ D,
+                                                          double theta, double floor_,
+                                                          const double* __restrict__ sums,
+                                                          int64_t n, double* __restrict__ t)
+{
+	const double eps = (sums[1] != 0.0) ? sums[0] / sums[1] : 0.0;
+	for (int64_t i = int64_t(blockIdx.x) * kDT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kDT) {
+		double d = D[i] - theta;
+		if (fabs(d) < floor_)
+			d = (d < 0) ? -floor_ : floor_;
+		t[i] = (eps * u[i] - r[i]) / d;
+	}
+}
+
+void launch_precond_olsen(const double* r, const double* u, const double* D, double theta,
+                          double floor_, int64_t n, double* partial, double* sums, double* t,
+                          hipStream_t s)
+{
+	hipLaunchKernelGGL(olsen_dots_kernel, dim3(kDBlk), dim3(kDT), 0, s, r, u, D, theta, floor_, n,
+	                   partial);
+	hipLaunchKernelGGL(dv_reduce_kernel, dim3(2), dim3(kDT), 0, s, partial, sums, 0);
+	hipLaunchKernelGGL(olsen_apply_kernel, dim3(grid_n(n)), dim3(kDT), 0, s, r, u, D, theta, floor_,
+	                   sums, n, t);
+	DMK_HIP_CHECK(hipGetLastError());
+}
+
 namespace {
 // The four features of element i: (D_i - theta) / sD, r_i / rms_r, u_i / rms_u, and the
 // diagonal correction in the same units, clipped.

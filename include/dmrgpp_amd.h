@@ -324,7 +324,7 @@ int  dmk_mlp_train(dmk_mlp* mlp, int iters, int64_t batch, float lr, double* los
 enum dmk_precond
 {
 	DMK_PRECOND_NONE = 0, /* t = r */
-	DMK_PRECOND_DIAG = 1, /* t = r / (theta - diag H) */
+	DMK_PRECOND_DIAG = 1, /* Olsen: t = M^-1 (eps u - r), M = diag H - theta, t orthogonal to u */
 	DMK_PRECOND_MLP  = 2, /* t = MLP(features) in units of rms(r) / rms(diag H - theta) */
 };
 

@@ -106,7 +106,7 @@ def test_gpu_davidson_kron31_matches_ed(precond):
     init = dm.DeviceArray.from_host(np.random.default_rng(1234).uniform(-1, 1, full.n))
     z = dm.DeviceArray(full.n)
     r = op.davidson_ground(init, z, precond=precond, tolerance=1e-10, max_basis=12)
-    assert r["converged"] == 1
+    assert r["converged"] == 1, r
     assert abs(r["energy"] - (-3.374932598687897)) < 1e-10
     assert r["residual"] < 1e-9
     zz = z.download()
