@@ -13,7 +13,8 @@
 //
 // Training pairs: with harvest set, up to harvest_iters iterates u_j are kept together with the
 // features of harvest_samples sampled elements; once the eigenvector psi is known, the exact
-// correction t* = psi / (psi . u_j) - u_j of each iterate gives the targets.
+// correction t* = psi / (psi . u_j) - u_j of each iterate, minus Olsen's correction, gives the
+// targets: the network learns what the diagonal method misses.
 #include "davidson.h"
 
 #include "dmk_internal.h"
@@ -37,14 +38,17 @@ void launch_ritz(const double* V, const double* W, int64_t ld, const double* sv,
 void launch_precond_diag(const double* r, const double* D, double theta, double floor_, int64_t n,
                          double* t, hipStream_t s);
 void launch_precond_mlp(const float* params, const double* r, const double* u, const double* D,
-                        double theta, double sD, double rms_r, double rms_u, int64_t n, double* t,
-                        hipStream_t s);
+                        double theta, double floor_, const double* sums, double sD, double rms_r,
+                        double rms_u, int64_t n, double* t, hipStream_t s);
+void launch_olsen_sums(const double* r, const double* u, const double* D, double theta,
+                       double floor_, int64_t n, double* partial, double* sums, hipStream_t s);
 void launch_harvest_features(const double* r, const double* u, const double* D, double theta,
-                             double sD, double rms_r, double rms_u, int64_t n, int64_t phase,
-                             int64_t stride, int64_t count, float* x_out, hipStream_t s);
-void launch_harvest_targets(const double* psi, const double* u, const double* pu, double unit_inv,
-                            int64_t n, int64_t phase, int64_t stride, int64_t count, float* t_out,
-                            hipStream_t s);
+                             double floor_, const double* sums, double sD, double rms_r,
+                             double rms_u, int64_t n, int64_t phase, int64_t stride, int64_t count,
+                             float* x_out, float* tols_out, hipStream_t s);
+void launch_harvest_targets(const double* psi, const double* u, const double* pu, const float* tols,
+                            double unit_inv, int64_t n, int64_t phase, int64_t stride,
+                            int64_t count, float* t_out, hipStream_t s);
 void launch_dot(const double* x, const double* y, int64_t n, double* partial, double* out,
                 int mode, hipStream_t s);
 int  reduction_blocks();
@@ -57,8 +61,7 @@ void launch_residual(const double* hx, const double* x, double e, int64_t n, dou
 void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
 void launch_fill(double* x, double v, int64_t n, hipStream_t s);
 void launch_precond_olsen(const double* r, const double* u, const double* D, double theta,
-                          double floor_, int64_t n, double* partial, double* sums, double* t,
-                          hipStream_t s);
+                          double floor_, int64_t n, const double* sums, double* t, hipStream_t s);
 
 double dense_lowest(const std::vector<double>& G, int k, std::vector<double>& vec)
 {
@@ -190,6 +193,8 @@ DavidsonReport davidson_ground(Plan& plan, const double* init, double* zout,
 	double*       Ubuf   = HI ? B.alloc(size_t(HI) * size_t(n)) : nullptr;
 	float*        Xst    = HI ? reinterpret_cast<float*>(B.alloc(size_t(HI) * size_t(HS) * kMlpIn / 2 + 1))
 	                          : nullptr;
+	float*        Tst    = HI ? reinterpret_cast<float*>(B.alloc(size_t(HI) * size_t(HS) / 2 + 1))
+	                          : nullptr; // Olsen's correction at the samples, in units
 	double*       d_pu   = HI ? B.alloc(size_t(HI)) : nullptr;
 	std::vector<HarvestMeta> meta;
 
@@ -257,12 +262,17 @@ DavidsonReport davidson_ground(Plan& plan, const double* init, double* zout,
 			const double rms_r = rnorm / std::sqrt(double(n));
 			const double rms_u = unorm / std::sqrt(double(n));
 			const double sD    = std::max(1e-12, std::sqrt(std::max(0.0, meanD2 - 2.0 * theta * meanD + theta * theta)));
+			const double floor_ = 1e-8 * std::max(1.0, std::fabs(theta));
+			double*      SUMS   = MISC + 2; // Olsen's u . M^-1 r and u . M^-1 u
+			if (p.precond != PRECOND_NONE || HI)
+				launch_olsen_sums(r, u, D, theta, floor_, n, part, SUMS, s);
 			if (HI && int(meta.size()) < HI && rnorm > 0) {
 				const int     j     = int(meta.size());
 				const int64_t phase = (int64_t(j) * 7919 + 13) % stride;
 				launch_copy(Ubuf + int64_t(j) * n, u, n, s);
-				launch_harvest_features(r, u, D, theta, sD, rms_r, rms_u, n, phase, stride, HS,
-				                        Xst + int64_t(j) * HS * kMlpIn, s);
+				launch_harvest_features(r, u, D, theta, floor_, SUMS, sD, rms_r, rms_u, n, phase,
+				                        stride, HS, Xst + int64_t(j) * HS * kMlpIn,
+				                        Tst + int64_t(j) * HS, s);
 				meta.push_back(HarvestMeta{ theta, sD, rms_r, phase });
 			}
 			if (res < p.tolerance) {
@@ -276,10 +286,10 @@ DavidsonReport davidson_ground(Plan& plan, const double* init, double* zout,
 			if (p.precond == PRECOND_NONE)
 				launch_copy(t, r, n, s);
 			else if (p.precond == PRECOND_DIAG)
-				launch_precond_olsen(r, u, D, theta, 1e-8 * std::max(1.0, std::fabs(theta)), n, part,
-				                     MISC + 2, t, s);
+				launch_precond_olsen(r, u, D, theta, floor_, n, SUMS, t, s);
 			else
-				launch_precond_mlp(mlp->params(), r, u, D, theta, sD, rms_r, rms_u, n, t, s);
+				launch_precond_mlp(mlp->params(), r, u, D, theta, floor_, SUMS, sD, rms_r, rms_u, n,
+				                   t, s);
 			DMK_HIP_CHECK(hipEventRecord(e1, s));
 			DMK_HIP_CHECK(hipEventSynchronize(e1));
 			{
@@ -390,7 +400,7 @@ DavidsonReport davidson_ground(Plan& plan, const double* init, double* zout,
 				                             Xst + int64_t(j) * HS * kMlpIn,
 				                             size_t(got) * kMlpIn * sizeof(float),
 				                             hipMemcpyDeviceToDevice, s));
-				launch_harvest_targets(zout, Ubuf + int64_t(j) * n, d_pu + j,
+				launch_harvest_targets(zout, Ubuf + int64_t(j) * n, d_pu + j, Tst + int64_t(j) * HS,
 				                       meta[size_t(j)].sD / meta[size_t(j)].rms_r, n,
 				                       meta[size_t(j)].phase, stride, got,
 				                       mlp->samples_t() + first, s);

@@ -228,35 +228,34 @@ __global__ __launch_bounds__(kDT) void olsen_apply_kernel(const double* __restri
 }
 
 void launch_precond_olsen(const double* r, const double* u, const double* D, double theta,
-                          double floor_, int64_t n, double* partial, double* sums, double* t,
-                          hipStream_t s)
+                          double floor_, int64_t n, const double* sums, double* t, hipStream_t s)
 {
-	hipLaunchKernelGGL(olsen_dots_kernel, dim3(kDBlk), dim3(kDT), 0, s, r, u, D, theta, floor_, n,
-	                   partial);
-	hipLaunchKernelGGL(dv_reduce_kernel, dim3(2), dim3(kDT), 0, s, partial, sums, 0);
 	hipLaunchKernelGGL(olsen_apply_kernel, dim3(grid_n(n)), dim3(kDT), 0, s, r, u, D, theta, floor_,
 	                   sums, n, t);
 	DMK_HIP_CHECK(hipGetLastError());
 }
 
 namespace {
-// The four features of element i: (D_i - theta) / sD, r_i / rms_r, u_i / rms_u, and the
-// diagonal correction in the same units, clipped.
-__device__ inline void features(double Di, double ri, double ui, double theta, double sD,
-                                double rms_r, double rms_u, float f[kMlpIn])
+// Olsen's correction of element i, t_ols = (eps u - r) / M with M = D - theta kept at least
+// floor in magnitude.
+__device__ inline double olsen_elem(double Di, double ri, double ui, double theta, double floor_,
+                                    double eps)
 {
-	const float f0 = float((Di - theta) / sD);
-	const float f1 = float(ri / rms_r);
-	const float f2 = float(ui / rms_u);
-	float       d  = f0;
-	if (fabsf(d) < 0.05f)
-		d = (d < 0.0f) ? -0.05f : 0.05f;
-	float f3 = -f1 / d;
-	f3       = fminf(fmaxf(f3, -10.0f), 10.0f);
-	f[0]     = f0;
-	f[1]     = f1;
-	f[2]     = f2;
-	f[3]     = f3;
+	double d = Di - theta;
+	if (fabs(d) < floor_)
+		d = (d < 0) ? -floor_ : floor_;
+	return (eps * ui - ri) / d;
+}
+
+// The four features of element i: (D_i - theta) / sD, r_i / rms_r, u_i / rms_u and Olsen's
+// correction in units of rms_r / sD, clipped to [-10, 10].
+__device__ inline void features(double Di, double ri, double ui, double theta, double sD,
+                                double rms_r, double rms_u, double tols_unit, float f[kMlpIn])
+{
+	f[0] = float((Di - theta) / sD);
+	f[1] = float(ri / rms_r);
+	f[2] = float(ui / rms_u);
+	f[3] = float(fmin(fmax(tols_unit, -10.0), 10.0));
 }
 
 __device__ inline float mlp_eval(const float* __restrict__ P, const float x[kMlpIn])
@@ -282,56 +281,67 @@ __device__ inline float mlp_eval(const float* __restrict__ P, const float x[kMlp
 }
 } // namespace
 
-// t_i = MLP(features_i) * rms_r / sD (the network predicts the correction in units of
-// rms_r / sD, the scale of the diagonal correction).
+// t_i = t_ols,i + MLP(features_i) * rms_r / sD: the network learns a correction on top of
+// Olsen's diagonal one, in units of rms_r / sD (its output layer starts at zero, so an
+// untrained network is exactly the diagonal method). eps = sums[0] / sums[1] from the Olsen
+// dot products of this iterate.
 __global__ __launch_bounds__(kDT) void precond_mlp_kernel(const float* __restrict__ params,
                                                           const double* __restrict__ r,
                                                           const double* __restrict__ u,
                                                           const double* __restrict__ D,
-                                                          double theta, double sD, double rms_r,
-                                                          double rms_u, int64_t n,
-                                                          double* __restrict__ t)
+                                                          double theta, double floor_,
+                                                          const double* __restrict__ sums,
+                                                          double sD, double rms_r, double rms_u,
+                                                          int64_t n, double* __restrict__ t)
 {
 	__shared__ float P[kMlpParams];
 	for (int i = threadIdx.x; i < kMlpParams; i += kDT)
 		P[i] = params[i];
 	__syncthreads();
+	const double eps  = (sums[1] != 0.0) ? sums[0] / sums[1] : 0.0;
 	const double unit = rms_r / sD;
 	for (int64_t i = int64_t(blockIdx.x) * kDT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kDT) {
-		float f[kMlpIn];
-		features(D[i], r[i], u[i], theta, sD, rms_r, rms_u, f);
-		t[i] = double(mlp_eval(P, f)) * unit;
+		const double tols = olsen_elem(D[i], r[i], u[i], theta, floor_, eps);
+		float        f[kMlpIn];
+		features(D[i], r[i], u[i], theta, sD, rms_r, rms_u, tols / unit, f);
+		t[i] = tols + double(mlp_eval(P, f)) * unit;
 	}
 }
 
 // Features of `count` sampled elements i_q = (phase + q * stride) mod n of one iterate
-// (x_out row major, kMlpIn per sample) and the sampled u values (u_out).
+// (x_out row major, kMlpIn per sample) and Olsen's correction there in units (tols_out).
 __global__ __launch_bounds__(kDT) void harvest_features_kernel(
     const double* __restrict__ r, const double* __restrict__ u, const double* __restrict__ D,
-    double theta, double sD, double rms_r, double rms_u, int64_t n, int64_t phase, int64_t stride,
-    int64_t count, float* __restrict__ x_out)
+    double theta, double floor_, const double* __restrict__ sums, double sD, double rms_r,
+    double rms_u, int64_t n, int64_t phase, int64_t stride, int64_t count,
+    float* __restrict__ x_out, float* __restrict__ tols_out)
 {
+	const double eps  = (sums[1] != 0.0) ? sums[0] / sums[1] : 0.0;
+	const double unit = rms_r / sD;
 	for (int64_t q = int64_t(blockIdx.x) * kDT + threadIdx.x; q < count; q += int64_t(gridDim.x) * kDT) {
-		const int64_t i = (phase + q * stride) % n;
+		const int64_t i    = (phase + q * stride) % n;
+		const double  tols = olsen_elem(D[i], r[i], u[i], theta, floor_, eps) / unit;
 		float         f[kMlpIn];
-		features(D[i], r[i], u[i], theta, sD, rms_r, rms_u, f);
+		features(D[i], r[i], u[i], theta, sD, rms_r, rms_u, tols, f);
 #pragma unroll
 		for (int c = 0; c < kMlpIn; ++c)
 			x_out[q * kMlpIn + c] = f[c];
+		tols_out[q] = float(tols);
 	}
 }
 
 // Targets of the sampled elements once the eigenvector psi is known: the exact correction
-// t* = psi / (psi . u) - u of the iterate u, in units of rms_r / sD, clipped to [-20, 20].
+// t* = psi / (psi . u) - u of the iterate u minus Olsen's correction, in units of rms_r / sD,
+// clipped to [-20, 20].
 __global__ __launch_bounds__(kDT) void harvest_targets_kernel(
     const double* __restrict__ psi, const double* __restrict__ u, const double* __restrict__ pu,
-    double unit_inv, int64_t n, int64_t phase, int64_t stride, int64_t count,
-    float* __restrict__ t_out)
+    const float* __restrict__ tols, double unit_inv, int64_t n, int64_t phase, int64_t stride,
+    int64_t count, float* __restrict__ t_out)
 {
 	const double inv = 1.0 / pu[0];
 	for (int64_t q = int64_t(blockIdx.x) * kDT + threadIdx.x; q < count; q += int64_t(gridDim.x) * kDT) {
 		const int64_t i = (phase + q * stride) % n;
-		double        y = (psi[i] * inv - u[i]) * unit_inv;
+		double        y = (psi[i] * inv - u[i]) * unit_inv - double(tols[q]);
 		y               = fmin(fmax(y, -20.0), 20.0);
 		t_out[q]        = float(y);
 	}
@@ -388,36 +398,47 @@ void launch_precond_diag(const double* r, const double* D, double theta, double 
 }
 
 void launch_precond_mlp(const float* params, const double* r, const double* u, const double* D,
-                        double theta, double sD, double rms_r, double rms_u, int64_t n, double* t,
-                        hipStream_t s)
+                        double theta, double floor_, const double* sums, double sD, double rms_r,
+                        double rms_u, int64_t n, double* t, hipStream_t s)
 {
 	unsigned g = grid_n(n);
 	if (g > 2048)
 		g = 2048;
-	hipLaunchKernelGGL(precond_mlp_kernel, dim3(g), dim3(kDT), 0, s, params, r, u, D, theta, sD,
-	                   rms_r, rms_u, n, t);
+	hipLaunchKernelGGL(precond_mlp_kernel, dim3(g), dim3(kDT), 0, s, params, r, u, D, theta, floor_,
+	                   sums, sD, rms_r, rms_u, n, t);
+	DMK_HIP_CHECK(hipGetLastError());
+}
+
+void launch_olsen_sums(const double* r, const double* u, const double* D, double theta,
+                       double floor_, int64_t n, double* partial, double* sums, hipStream_t s)
+{
+	hipLaunchKernelGGL(olsen_dots_kernel, dim3(kDBlk), dim3(kDT), 0, s, r, u, D, theta, floor_, n,
+	                   partial);
+	hipLaunchKernelGGL(dv_reduce_kernel, dim3(2), dim3(kDT), 0, s, partial, sums, 0);
 	DMK_HIP_CHECK(hipGetLastError());
 }
 
 void launch_harvest_features(const double* r, const double* u, const double* D, double theta,
-                             double sD, double rms_r, double rms_u, int64_t n, int64_t phase,
-                             int64_t stride, int64_t count, float* x_out, hipStream_t s)
+                             double floor_, const double* sums, double sD, double rms_r,
+                             double rms_u, int64_t n, int64_t phase, int64_t stride, int64_t count,
+                             float* x_out, float* tols_out, hipStream_t s)
 {
 	if (count <= 0)
 		return;
 	hipLaunchKernelGGL(harvest_features_kernel, dim3(grid_n(count)), dim3(kDT), 0, s, r, u, D,
-	                   theta, sD, rms_r, rms_u, n, phase, stride, count, x_out);
+	                   theta, floor_, sums, sD, rms_r, rms_u, n, phase, stride, count, x_out,
+	                   tols_out);
 	DMK_HIP_CHECK(hipGetLastError());
 }
 
-void launch_harvest_targets(const double* psi, const double* u, const double* pu, double unit_inv,
-                            int64_t n, int64_t phase, int64_t stride, int64_t count, float* t_out,
-                            hipStream_t s)
+void launch_harvest_targets(const double* psi, const double* u, const double* pu, const float* tols,
+                            double unit_inv, int64_t n, int64_t phase, int64_t stride,
+                            int64_t count, float* t_out, hipStream_t s)
 {
 	if (count <= 0)
 		return;
 	hipLaunchKernelGGL(harvest_targets_kernel, dim3(grid_n(count)), dim3(kDT), 0, s, psi, u, pu,
-	                   unit_inv, n, phase, stride, count, t_out);
+	                   tols, unit_inv, n, phase, stride, count, t_out);
 	DMK_HIP_CHECK(hipGetLastError());
 }
 

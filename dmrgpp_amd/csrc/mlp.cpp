@@ -23,7 +23,8 @@ Mlp::Mlp(uint64_t seed, int64_t sample_capacity)
 	cap_      = sample_capacity;
 	d_x_      = dev(size_t(cap_) * kMlpIn);
 	d_t_      = dev(size_t(cap_));
-	// Glorot-style uniform initialization, zero biases, from a fixed-seed generator.
+	// Glorot-style uniform initialization of the hidden layers, zero biases, from a fixed-seed
+	// generator.
 	std::vector<float>                    p(kMlpParams, 0.0f);
 	std::mt19937_64                       rng(seed);
 	auto fill = [&](int off, int fan_in, int fan_out) {
@@ -34,7 +35,7 @@ Mlp::Mlp(uint64_t seed, int64_t sample_capacity)
 	};
 	fill(kMlpOffW1, kMlpIn, kMlpHidden);
 	fill(kMlpOffW2, kMlpHidden, kMlpHidden);
-	fill(kMlpOffW3, kMlpHidden, 1);
+	// W3 and b3 start at zero: an untrained network predicts no correction.
 	set_params(p.data());
 	std::vector<float> zero(kMlpParams, 0.0f);
 	DMK_HIP_CHECK(hipMemcpy(d_m_, zero.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));

@@ -218,7 +218,7 @@ TUNINGS = [
          xcd_order=1, mfma32=1),
     dict(stage1_tile_m=128, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=64, splitk=48,
          xcd_order=1, mfma32=0),
-    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=1024,
+    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=64,
          xcd_order=0, mfma32=1),
 ]
 
