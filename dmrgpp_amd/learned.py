@@ -5,9 +5,12 @@ correction at the same FP64 residual.
   python -m dmrgpp_amd.learned --config hubbard-ladder-8x2 [--m M] [--train-sweeps 1]
 
 Phase 1 runs the first sweeps with Davidson (diagonal correction) and harvests training pairs
-from every converged step; the MLP is then trained on the GPU. Phase 2 runs the remaining
-sweeps three times from the same block stores (Lanczos, Davidson + diagonal, Davidson + MLP),
-each step started from the same transformed wave function, and counts matvecs and time.
+from every converged step; the MLP is then trained on the GPU. Validation: the last training
+superblock is solved again with the diagonal and with the learned correction; the learned one
+is kept only if it needs fewer matvecs there (otherwise the diagonal method is the fallback).
+Phase 2 runs the remaining sweeps from the same block stores with plain Lanczos, Davidson +
+diagonal, Davidson + MLP and the validated choice, each step started from the same transformed
+wave function, and counts matvecs and time.
 """
 from __future__ import annotations
 
@@ -22,14 +25,17 @@ from . import models, sweep
 
 
 class DavidsonSolver:
-    """sweep.run solver: Davidson with the given correction; harvests into mlp when asked."""
+    """sweep.run solver: Davidson with the given correction; harvests into mlp when asked and
+    keeps the last problem it solved (for validation)."""
 
     def __init__(self, precond="diag", mlp=None, harvest=False, tol=1e-9, max_basis=24,
                  max_matvecs=600, seed=1234):
         self.precond, self.mlp, self.harvest = precond, mlp, harvest
         self.tol, self.max_basis, self.max_matvecs, self.seed = tol, max_basis, max_matvecs, seed
+        self.last = None
 
     def __call__(self, prob, init=None):
+        self.last = prob
         from . import BatchedGemm, DeviceArray
 
         own = not isinstance(init, DeviceArray)
@@ -112,14 +118,25 @@ def compare(name: str, m=None, train_sweeps=1, test_sweeps=1, tol=1e-9, train_it
     sweep.run(model, lat, target, m, [m] * train_sweeps, harvest, log=log, state=state, wft=True)
     t1 = time.perf_counter()
     nsamp = mlp.samples()
+    _, loss0 = mlp.train(0, batch=8192, lr=2e-3)  # loss of the untrained (zero) correction
     loss_last, loss_all = mlp.train(train_iters, batch=8192, lr=2e-3)
     t2 = time.perf_counter()
     log(f"harvested {nsamp} pairs in {t1 - t0:.1f} s; trained {train_iters} Adam steps in "
-        f"{t2 - t1:.2f} s, loss {loss_all:.4g}")
-    out = {"config": name, "m": m, "samples": nsamp, "train_s": t2 - t1, "loss": loss_all}
+        f"{t2 - t1:.2f} s, loss {loss0:.4g} -> {loss_all:.4g}")
+    # Validation on the last training superblock: keep the learned correction only if it wins.
+    val = {}
+    for pc in ("diag", "mlp"):
+        e, _, r = DavidsonSolver(pc, mlp=mlp, tol=tol)(harvest.last)
+        r["z_dev"].free()
+        val[pc] = r["matvecs"]
+    chosen = "mlp" if val["mlp"] < val["diag"] else "diag"
+    log(f"validation matvecs: diag {val['diag']}, mlp {val['mlp']} -> {chosen}")
+    out = {"config": name, "m": m, "samples": nsamp, "train_s": t2 - t1, "loss_untrained": loss0,
+           "loss": loss_all, "validation_matvecs": val, "chosen": chosen}
     solvers = {"lanczos": LanczosResidualSolver(tol=tol),
                "davidson-diag": DavidsonSolver("diag", tol=tol),
-               "davidson-mlp": DavidsonSolver("mlp", mlp=mlp, tol=tol)}
+               "davidson-mlp": DavidsonSolver("mlp", mlp=mlp, tol=tol),
+               "davidson-learned": DavidsonSolver(chosen, mlp=mlp, tol=tol)}
     for key, solver in solvers.items():
         st = sweep.clone_state(state)
         ts = time.perf_counter()
@@ -130,7 +147,11 @@ def compare(name: str, m=None, train_sweeps=1, test_sweeps=1, tol=1e-9, train_it
         res = max(r.info.get("residual", 0.0) for r in recs)
         out[key] = {"energy": e, "matvecs": mv, "wall_s": te - ts, "max_residual": res,
                     "steps": len(recs)}
-        log(f"{key:14s} E={e:.12f} matvecs={mv:6d} wall={te - ts:7.2f}s max_res={res:.2e}")
+        log(f"{key:16s} E={e:.12f} matvecs={mv:6d} wall={te - ts:7.2f}s max_res={res:.2e}")
+    out["gain_matvecs_vs_diag"] = (out["davidson-diag"]["matvecs"]
+                                   / max(1, out["davidson-learned"]["matvecs"]))
+    out["gain_matvecs_vs_lanczos"] = (out["lanczos"]["matvecs"]
+                                      / max(1, out["davidson-learned"]["matvecs"]))
     return out
 
 

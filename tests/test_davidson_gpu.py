@@ -125,3 +125,23 @@ def test_gpu_davidson_matches_lanczos_on_qn_problem():
     assert dv["converged"] == 1 and dv["residual"] < 2e-9
     assert lz["residual"] < 2e-9
     assert abs(dv["energy"] - lz["energy"]) < 1e-10 * abs(lz["energy"])
+
+
+@pytest.mark.gpu
+def test_gpu_learned_preconditioner_comparison():
+    # Train on the first sweep of a Hubbard chain, then compare plain Lanczos, Davidson with the
+    # diagonal correction and Davidson with the validated learned correction on the next sweep,
+    # all at FP64 residual 1e-9: equal energies, and the learned method (the network, or the
+    # diagonal fallback when validation rejects it) needs no more matvecs than the diagonal one
+    # beyond a 10% margin.
+    from dmrgpp_amd import learned
+
+    out = learned.compare("hubbard-chain-12", train_sweeps=1, test_sweeps=1, train_iters=1500,
+                          log=lambda s: None)
+    e = out["lanczos"]["energy"]
+    for key in ("davidson-diag", "davidson-mlp", "davidson-learned"):
+        assert abs(out[key]["energy"] - e) < 1e-9 * abs(e)
+        assert out[key]["max_residual"] < 1.1e-9
+    assert out["samples"] > 0 and out["loss"] <= out["loss_untrained"]
+    assert out["davidson-learned"]["matvecs"] <= 1.1 * out["davidson-diag"]["matvecs"]
+    assert out["lanczos"]["matvecs"] > out["davidson-diag"]["matvecs"]
