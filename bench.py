@@ -103,6 +103,11 @@ def main():
                     help="finite-sweep DMRG steps timed per precision mode (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--save-problem", default=None, help="write the superblock to this .npz")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of autotuning each mode")
+    ap.add_argument("--save-tuning", default=None, help="write the chosen tunings (JSON)")
+    ap.add_argument("--load-tuning", default=None,
+                    help="apply tunings from this JSON instead of autotuning (profiled runs)")
     ap.add_argument("--load-problem", default=None,
                     help="read the superblock from this .npz instead of growing it (no sweep)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
@@ -159,10 +164,19 @@ def main():
     dx = dm.DeviceArray.from_host(x)
     dy = dm.DeviceArray(prob.n)
 
-    # ---- timed matvecs
+    # ---- timed matvecs (each mode with the autotuner's launch shapes)
     results = {}
+    tunings = {}
+    loaded = json.load(open(args.load_tuning)) if args.load_tuning else None
     for prec in ("fp64", "fp32", "bf16x3"):
         op.set_precision(prec)
+        if loaded:
+            op.set_tuning(**loaded[prec])
+        elif not args.no_autotune:
+            t_at = time.perf_counter()
+            op.autotune(dx, dy, 3)
+            log(f"{prec}: autotuned in {time.perf_counter() - t_at:.2f} s -> {op.tuning()}")
+        tunings[prec] = op.tuning()
         for _ in range(args.warmup):
             op.apply_Htarget(dx, dy)
         op.sync()
@@ -177,6 +191,8 @@ def main():
                          "ms_stage2": ms2, "ms_total_events": mst}
         log(f"{prec}: {1e3 * dt / args.steps:.3f} ms/matvec (events: stage1 {ms1:.3f} ms, "
             f"stage2 {ms2:.3f} ms, total {mst:.3f} ms)")
+    if args.save_tuning:
+        json.dump(tunings, open(args.save_tuning, "w"))
     y64 = results["fp64"]["y"]
     for prec in ("fp32", "bf16x3"):
         results[prec]["rel_err"] = float(np.linalg.norm(results[prec]["y"] - y64)
@@ -277,7 +293,7 @@ def main():
                          "tflops_per_matvec": round(info["flops_algorithmic"]
                                                     / (results[prec]["ms_total_events"] * 1e-3)
                                                     / 1e12, 3),
-                         "roofline": roofline_of(prec)}
+                         "roofline": roofline_of(prec), "tuning": tunings[prec]}
                   for prec in ("fp64", "fp32", "bf16x3")},
         "ground_state": {m: {"energy": lz[m]["energy"], "rel_dE_vs_fp64": e_rel[m],
                              "gate_1e-10": bool(e_rel[m] < 1e-10),

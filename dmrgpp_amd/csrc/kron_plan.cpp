@@ -529,8 +529,13 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	DMK_HIP_CHECK(e3);
 	// A tuning the autotuner chose for this histogram earlier in the process, else the default.
 	Tuning t;
-	cached_tuning(histogram_key(), t);
+	cached_tuning(tuning_key(), t);
 	retune(t);
+}
+
+std::string Plan::tuning_key() const
+{
+	return histogram_key() + "|p" + std::to_string(precision_);
 }
 
 Plan::~Plan()
@@ -805,6 +810,11 @@ void Plan::set_precision(int p)
 		throw std::runtime_error("dmk_set_precision: unsupported precision mode "
 		                         + std::to_string(p));
 	precision_ = p;
+	{
+		Tuning t = tuning_;
+		if (cached_tuning(tuning_key(), t) && !(t == tuning_))
+			retune(t);
+	}
 	if (p == PREC_FP32)
 		ensure_fp32();
 	if (p == PREC_BF16X3)
@@ -927,17 +937,19 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 	const int nvar = (precision_ == PREC_FP32) ? 2 : 1; // FP32: 16x16x4 and 32x32x2 MFMA
 	for (int var = 0; var < nvar; ++var)
 		for (int bm1 : { 64, 128 })
-			for (int bm2 : { 64, 128 })
-				for (int sk : { 0, 512, 1024, 2048 }) {
-					Tuning t;
-					t.bm1    = bm1;
-					t.bn1    = 64;
-					t.bm2    = bm2;
-					t.bn2    = 64;
-					t.splitk = sk;
-					t.mfma32 = var;
-					cands.push_back(t);
-				}
+			for (int bn1 : { 64, 128 })
+				for (int bm2 : { 64, 128 })
+					for (int bn2 : { 64, 128 })
+						for (int sk : { 0, 1024, 2048 }) {
+							Tuning t;
+							t.bm1    = bm1;
+							t.bn1    = bn1;
+							t.bm2    = bm2;
+							t.bn2    = bn2;
+							t.splitk = sk;
+							t.mfma32 = var;
+							cands.push_back(t);
+						}
 	Tuning best     = start;
 	double best_ms  = 1e300;
 	for (const Tuning& t : cands) {
@@ -951,7 +963,7 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		}
 	}
 	retune(best);
-	store_tuning(histogram_key(), best);
+	store_tuning(tuning_key(), best);
 	return best;
 }
 

@@ -53,6 +53,12 @@ struct Tuning {
 	int splitk = 1024;
 	int xcd    = 0; // 1: XCD-aware tile order
 	int mfma32 = 0; // 1: FP32 mode on 32x32x2 MFMA blocks
+
+	bool operator==(const Tuning& o) const
+	{
+		return bm1 == o.bm1 && bn1 == o.bn1 && bm2 == o.bm2 && bn2 == o.bn2 && splitk == o.splitk
+		       && xcd == o.xcd && mfma32 == o.mfma32;
+	}
 };
 
 struct StageTables {
@@ -99,6 +105,8 @@ public:
 
 	// Signature of the block-size histogram (log2 bins of the stage-2 task shapes).
 	std::string histogram_key() const;
+	// The autotuner cache key: the histogram signature and the precision mode.
+	std::string tuning_key() const;
 
 	const PlanStats& stats() const { return stats_; }
 	hipStream_t      stream() const { return stream_; }
