@@ -527,10 +527,15 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	DMK_HIP_CHECK(e1);
 	DMK_HIP_CHECK(e2);
 	DMK_HIP_CHECK(e3);
-	// A tuning the autotuner chose for this histogram earlier in the process, else the default.
-	Tuning t;
-	cached_tuning(tuning_key(), t);
-	retune(t);
+	// Per precision mode: a tuning the autotuner chose for this histogram earlier in the
+	// process, else the default.
+	const int p0 = precision_;
+	for (int p = 0; p < 3; ++p) {
+		precision_ = p;
+		cached_tuning(tuning_key(), tuning_of_[p]);
+	}
+	precision_ = p0;
+	retune(tuning_of_[p0]);
 }
 
 std::string Plan::tuning_key() const
@@ -640,7 +645,8 @@ void Plan::retune(const Tuning& t)
 		throw std::invalid_argument("dmk tuning: mfma32 must be 0 or 1");
 	if (stream_)
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
-	tuning_ = t;
+	tuning_                 = t;
+	tuning_of_[precision_] = t;
 
 	// Stage 1 as planned.
 	st1_.release();
@@ -810,11 +816,8 @@ void Plan::set_precision(int p)
 		throw std::runtime_error("dmk_set_precision: unsupported precision mode "
 		                         + std::to_string(p));
 	precision_ = p;
-	{
-		Tuning t = tuning_;
-		if (cached_tuning(tuning_key(), t) && !(t == tuning_))
-			retune(t);
-	}
+	if (!(tuning_of_[p] == tuning_))
+		retune(tuning_of_[p]);
 	if (p == PREC_FP32)
 		ensure_fp32();
 	if (p == PREC_BF16X3)

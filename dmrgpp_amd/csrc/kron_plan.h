@@ -95,7 +95,9 @@ public:
 	void set_precision(int p);
 	int  precision() const { return precision_; }
 
-	// Rebuilds the launch tables for a new tuning (tile shapes, split-k, order).
+	// Rebuilds the launch tables for a new tuning (tile shapes, split-k, order). The tuning
+	// belongs to the current precision mode: set_precision switches to the tuning last set
+	// (or autotuned, or cached, or default) for the mode it selects.
 	void          retune(const Tuning& t);
 	const Tuning& tuning() const { return tuning_; }
 
@@ -130,6 +132,7 @@ private:
 
 	int         precision_ = PREC_FP64;
 	Tuning      tuning_;
+	Tuning      tuning_of_[3]; // per precision mode
 	PlanStats   stats_;
 	StageTables st1_, st2_;
 	hipStream_t stream_ = nullptr;

@@ -146,7 +146,9 @@ int dmk_set_precision(dmk_handle* handle, int precision);
 int dmk_get_info(const dmk_handle* handle, dmk_info* info);
 
 /* Tuning of the launches. No reference counterpart (the Kokkos path hard-codes its shape,
- * BatchedGemmKokkos.h:110-130). dmk_set_tuning rebuilds the launch tables. */
+ * BatchedGemmKokkos.h:110-130). dmk_set_tuning rebuilds the launch tables. A tuning belongs to
+ * the handle's current precision mode: dmk_set_precision switches to the tuning last set or
+ * autotuned for the selected mode (else the cached or the default one). */
 int dmk_get_tuning(const dmk_handle* handle, dmk_tuning* tuning);
 int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning);
 

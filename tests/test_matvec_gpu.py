@@ -276,6 +276,34 @@ def test_gpu_autotune_keeps_results():
     op2.close()
 
 
+@pytest.mark.gpu
+def test_gpu_tuning_belongs_to_its_precision():
+    # A tuning set in one precision mode is kept for that mode only: switching modes selects
+    # each mode's own tuning, and the results stay the oracle's.
+    left = {-2: 40, 0: 70, 2: 40}
+    prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(2), seed=5)
+    x = np.random.default_rng(4).uniform(-1, 1, prob.n)
+    want = oracle.plugin_apply(prob, x)
+    op = dm.BatchedGemm(prob, "fp64")
+    t64 = op.tuning()
+    big = dict(t64, stage1_tile_m=128, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=128,
+               splitk=0, mfma32=1)
+    op.set_precision("fp32")
+    op.set_tuning(**big)
+    op.set_precision("fp64")
+    assert op.tuning() == t64
+    dx, dy = dm.DeviceArray.from_host(x), dm.DeviceArray(prob.n)
+    op.apply_Htarget(dx, dy)
+    op.sync()
+    assert rel_err(dy.download(), want) < 1e-12
+    op.set_precision("fp32")
+    assert op.tuning() == big
+    op.apply_Htarget(dx, dy)
+    op.sync()
+    assert rel_err(dy.download(), want) < 1e-5
+    op.close()
+
+
 def test_set_tuning_rejects_bad_shapes():
     t = dm.Tuning(stage1_tile_m=96, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64,
                   splitk=0, xcd_order=0)
