@@ -247,9 +247,9 @@ def main():
         else:
             kname, kms, kflops = "stage1", r["ms_stage1"], info["flops_stage1"]
         achieved = kflops / (kms * 1e-3) / 1e12
-        kern = {"fp64": "grouped_gemm_kernel<double,double>",
-                "fp32": "grouped_gemm_kernel<float,float|double>",
-                "bf16x3": "grouped_gemm_bf16x3_kernel"}[prec]
+        kern = {"fp64": "panel_gemm_kernel<double,OpF64>",
+                "fp32": "panel_gemm_kernel<double,OpF32|OpF32x32>",
+                "bf16x3": "panel_bf16x3_kernel"}[prec]
         return {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[prec], 4),
                 "traffic": TRAFFIC.get(prec), "kernel": f"{kern} ({kname})",

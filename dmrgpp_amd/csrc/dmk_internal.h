@@ -60,7 +60,9 @@ struct Reduce {
 	int32_t pad;
 };
 
-// One workgroup of the reduction launch: elements [start, start + len) of reduce entry r.
+// One workgroup of the reduction launch: elements [start, start + len) of reduce entry r,
+// len <= kReduceChunk.
+constexpr int kReduceChunk = 2048;
 struct ReduceChunk {
 	int32_t r;
 	int32_t len;
@@ -74,6 +76,73 @@ struct Tile {
 	int32_t n0;
 	int32_t pad;
 };
+
+// Descriptors of the matvec kernels on K-major panels (panel_kernels.hip). Operand bases:
+enum PBaseId : int32_t
+{
+	PBASE_APACK = 0, // packed left-block factors, K-major
+	PBASE_BPACK = 1, // packed right-block factors, K-major
+	PBASE_XP    = 2, // input patches X_J column major with padded columns (K-major as a right
+	                 // operand over rows of X_J, as a left operand over columns)
+	PBASE_XT    = 3, // input patches transposed, padded rows
+	PBASE_BX    = 4, // stage-1 products, K-major stage-2 panels
+	PBASE_COUNT = 5
+};
+
+// One contribution C(MxN) += L(MxK) * R(KxN): element (m, k) of L at loff + k + m * ldl,
+// element (k, n) of R at roff + k + n * ldr. Row starts and loff/roff are multiples of 8
+// elements and rows are zero padded to a multiple of 8 past k.
+struct PSeg {
+	int64_t loff;
+	int64_t roff;
+	int32_t lbase;
+	int32_t rbase;
+	int32_t ldl;
+	int32_t ldr;
+	int32_t k;
+	int32_t pad;
+};
+
+// One output matrix: element (r, c) at coff + (r % rsplit) * sm + (r / rsplit) * n + c.
+struct PTask {
+	int64_t coff;
+	int32_t m;
+	int32_t n;
+	int32_t rsplit;
+	int32_t sm;
+	int32_t seg_begin;
+	int32_t seg_end;
+	int32_t out_is_y; // 1: the caller's output vector, 0: the workspace (compute type)
+	int32_t pad;
+};
+
+// Per-vector element strides of the per-call operands in multi-vector launches.
+struct PStrides {
+	int64_t xp;
+	int64_t xt;
+	int64_t bx;
+};
+
+// Input preparation: a 64 x 64 tile (m0, c0) of patch `patch`.
+struct PrepTile {
+	int32_t patch;
+	int32_t m0;
+	int32_t c0;
+	int32_t pad;
+};
+struct PrepPatch {
+	int64_t xoff; // X_J in the input vector (R x L column major)
+	int64_t pofs; // XP_J: element (m, c) at pofs + m + c * ldp
+	int64_t tofs; // XT_J: element (m, c) at tofs + c + m * ldt
+	int32_t R;
+	int32_t L;
+	int32_t ldp;
+	int32_t ldt;
+};
+
+// Rows of K-major panels are padded to this many elements (16 bytes of bf16).
+constexpr int kPanelPad = 8;
+inline int64_t panel_ld(int64_t k) { return (k + kPanelPad - 1) / kPanelPad * kPanelPad; }
 
 // Error reporting for the C ABI.
 void        set_last_error(const std::string& msg);

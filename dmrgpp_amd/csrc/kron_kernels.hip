@@ -1,20 +1,16 @@
-// Grouped multi-segment GEMM for CDNA4 (gfx950) on MFMA, and the small conversion kernels
-// around it.
+// Grouped multi-segment GEMM on operands in the caller's own layouts (column major, the right
+// operand optionally transposed), for the variable-size batched GEMM (vbatch.cpp, the
+// dmrg_Xgemm_vbatch replacement); the split-k reduction and the FP64 -> FP32 cast shared with
+// the matvec (panel_kernels.hip holds the matvec's own GEMM kernels).
 //
 // One launch computes a list of independent output matrices ("tasks"), each the sum of one or
-// more products ("segments") C += L * op(R). The superblock matvec uses two launches of it:
-//   stage 1: for every (I,J) with general terms, [B_k1; B_k2; ...] * X_J (row-stacked B blocks),
-//            written side by side into the stage-2 panel BX_I;
-//   stage 2: Y_I = BX_I * A_I^T  +  X_I * H_L^T (identity right factor)  +  H_R * X_I (identity
-//            left factor).
-// This is the two-stage evaluation of GPUPlugin/apply_Htarget_sparse_not_ready.cpp:199-449 with
-// the identity factors of H_L (x) I and I (x) H_R contracted directly instead of as dense GEMMs.
+// more products ("segments") C += L * op(R).
 //
 // Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a BM x BN output tile (64 or
-// 128 each way) and each wave a (BM/2) x (BN/2) sub-tile held as MFMA 16x16 accumulators. K is walked in chunks of 16 staged
-// through LDS, double buffered: the global loads of chunk c+1 are issued before the MFMAs of
-// chunk c. Every output tile is owned by exactly one workgroup and its contraction order is
-// fixed, so results are bitwise reproducible (no atomics, no split-k).
+// 128 each way) and each wave a (BM/2) x (BN/2) sub-tile held as MFMA 16x16 accumulators. K is
+// walked in chunks of 16 staged through LDS, double buffered: the global loads of chunk c+1 are
+// issued before the MFMAs of chunk c. Every output tile is owned by exactly one workgroup and
+// its contraction order is fixed, so results are bitwise reproducible (no atomics).
 #include "dmk_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -156,6 +152,34 @@ __device__ inline bool next_chunk(const Seg* __restrict__ segs, int seg_end, int
 	return seg < seg_end;
 }
 
+// The MFMAs of one staged K chunk for one wave. Partial: only the first fi_n x fj_n blocks
+// (wave-uniform bounds), for tiles that overhang the task's edge.
+template <bool Partial, typename T, typename Op, int BM, int BN, int FI, int FJ>
+__device__ inline void mma_chunk(const T (*As)[BM + kPad], const T (*Bs)[BN + kPad], int lane,
+                                 int wm, int wn, int fi_n, int fj_n,
+                                 typename Op::acc_t (&acc)[FI][FJ])
+{
+#pragma unroll
+	for (int kk = 0; kk < kBK; kk += Op::kK) {
+		const int kr = kk + lane / Op::kM;
+		T         a[FI], b[FJ];
+#pragma unroll
+		for (int i = 0; i < FI; ++i)
+			if (!Partial || i < fi_n)
+				a[i] = As[kr][wm + Op::kM * i + lane % Op::kM];
+#pragma unroll
+		for (int j = 0; j < FJ; ++j)
+			if (!Partial || j < fj_n)
+				b[j] = Bs[kr][wn + Op::kM * j + lane % Op::kM];
+#pragma unroll
+		for (int i = 0; i < FI; ++i)
+#pragma unroll
+			for (int j = 0; j < FJ; ++j)
+				if (!Partial || (i < fi_n && j < fj_n))
+					acc[i][j] = Op::mma(a[i], b[j], acc[i][j]);
+	}
+}
+
 // T: operand/accumulate type; TO: output element type (the stage-2 output of the FP32 mode is
 // written in FP64 for the FP64 Lanczos recurrence). BM x BN output tile per workgroup, each of
 // the 4 waves owning a (BM/2) x (BN/2) sub-tile of (BM/32) x (BN/32) MFMA 16x16 blocks.
@@ -203,6 +227,11 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 #pragma unroll
 		for (int j = 0; j < FJ; ++j)
 			acc[i][j] = acc_t {};
+	// MFMA blocks of this wave that overlap the task (edge tiles): the others are skipped; their
+	// accumulators stay zero and the epilogue drops them.
+	const int fi_n = min(FI, max(0, (tk.m - tl.m0 - wm + Op::kM - 1) / Op::kM));
+	const int fj_n = min(FJ, max(0, (tk.n - tl.n0 - wn + Op::kM - 1) / Op::kM));
+	const bool full = fi_n == FI && fj_n == FJ;
 
 	int seg = tk.seg_begin;
 	int k0  = 0;
@@ -227,22 +256,10 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 				load_chunk<T, BM, BN>(st, pick(ns.lbase) + ns.loff, pick(ns.rbase) + ns.roff, ns,
 				                      tl.m0, tl.n0, tk.m, tk.n, nk0, tid);
 			}
-#pragma unroll
-			for (int kk = 0; kk < kBK; kk += Op::kK) {
-				const int kr = kk + lane / Op::kM;
-				T         a[FI], b[FJ];
-#pragma unroll
-				for (int i = 0; i < FI; ++i)
-					a[i] = As[buf][kr][wm + Op::kM * i + lane % Op::kM];
-#pragma unroll
-				for (int j = 0; j < FJ; ++j)
-					b[j] = Bs[buf][kr][wn + Op::kM * j + lane % Op::kM];
-#pragma unroll
-				for (int i = 0; i < FI; ++i)
-#pragma unroll
-					for (int j = 0; j < FJ; ++j)
-						acc[i][j] = Op::mma(a[i], b[j], acc[i][j]);
-			}
+			if (full)
+				mma_chunk<false, T, Op, BM, BN, FI, FJ>(As[buf], Bs[buf], lane, wm, wn, fi_n, fj_n, acc);
+			else
+				mma_chunk<true, T, Op, BM, BN, FI, FJ>(As[buf], Bs[buf], lane, wm, wn, fi_n, fj_n, acc);
 			if (!hasNext)
 				break;
 			store_chunk<T, BM, BN>(st, As[buf ^ 1], Bs[buf ^ 1], ns.rtrans, tid);
@@ -282,6 +299,11 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(
 }
 
 // Sums split-k partials in a fixed order (s = 0, 1, ...) into the output, accumulating in TO.
+// Each thread owns kReduceLanes elements of the chunk (stride 256) and keeps their loads of one
+// piece in flight together.
+constexpr int kReduceLanes = 8;
+static_assert(kReduceChunk == 256 * kReduceLanes, "reduce chunk / block mismatch");
+
 template <typename T, typename TO>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(const Reduce* __restrict__ red,
                                                             const ReduceChunk* __restrict__ chunks,
@@ -291,15 +313,26 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const Reduce* __rest
 {
 	const ReduceChunk ch = chunks[blockIdx.x];
 	const Reduce      rd = red[ch.r];
-	ws += int64_t(blockIdx.y) * wsstride + rd.wsoff;
-	out += int64_t(blockIdx.y) * ystride + rd.yoff;
-	for (int64_t e = ch.start + threadIdx.x; e < ch.start + ch.len; e += blockDim.x) {
-		TO v = 0;
-		for (int s = 0; s < rd.nsplit; ++s)
-			v += TO(ws[int64_t(s) * rd.count + e]);
-		if (beta)
-			v += out[e];
-		out[e] = v;
+	ws += int64_t(blockIdx.y) * wsstride + rd.wsoff + ch.start;
+	out += int64_t(blockIdx.y) * ystride + rd.yoff + ch.start;
+	TO v[kReduceLanes];
+#pragma unroll
+	for (int j = 0; j < kReduceLanes; ++j)
+		v[j] = 0;
+	for (int s = 0; s < rd.nsplit; ++s) {
+		const T* p = ws + int64_t(s) * rd.count;
+#pragma unroll
+		for (int j = 0; j < kReduceLanes; ++j) {
+			const int e = int(threadIdx.x) + 256 * j;
+			if (e < ch.len)
+				v[j] += TO(p[e]);
+		}
+	}
+#pragma unroll
+	for (int j = 0; j < kReduceLanes; ++j) {
+		const int e = int(threadIdx.x) + 256 * j;
+		if (e < ch.len)
+			out[e] = beta ? v[j] + out[e] : v[j];
 	}
 }
 
@@ -383,9 +416,10 @@ void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchu
 	template void launch_splitk_reduce<T, TO>(const Reduce*, const ReduceChunk*, int, const T*,   \
 	                                          int64_t, TO*, int64_t, int, int, hipStream_t);
 DMK_INST(double, double)
-DMK_INST(float, float)
-DMK_INST(float, double)
 #undef DMK_INST
+template void launch_splitk_reduce<float, double>(const Reduce*, const ReduceChunk*, int,
+                                                  const float*, int64_t, double*, int64_t, int,
+                                                  int, hipStream_t);
 
 void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream)
 {

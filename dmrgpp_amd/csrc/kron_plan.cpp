@@ -32,22 +32,27 @@
 namespace dmk {
 
 template <typename T, typename TO>
-void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
-                         int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
-                         TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
-                         int beta, hipStream_t stream, int variant = 0);
+void launch_panel_gemm(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
+                       const Tile* tiles, int ntiles, const T* const* bases, const PStrides& vs,
+                       TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec, int beta,
+                       hipStream_t stream);
+void launch_panel_bf16x3(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
+                         const Tile* tiles, int ntiles, const void* const* bases_hi,
+                         const void* const* bases_lo, const PStrides& vs, double* out_y,
+                         int64_t ystride, float* out_f, void* out_h, void* out_l, int64_t wsstride,
+                         bool pair, int nvec, int beta, hipStream_t stream);
+template <typename E>
+void launch_prep_x(const PrepTile* tiles, int ntiles, const PrepPatch* patches, const double* x,
+                   int64_t ldx, E* xp, E* xt, const PStrides& vs, int nvec, hipStream_t stream);
+void launch_prep_x_bf16x2(const PrepTile* tiles, int ntiles, const PrepPatch* patches,
+                          const double* x, int64_t ldx, void* xph, void* xpl, void* xth, void* xtl,
+                          const PStrides& vs, int nvec, hipStream_t stream);
 template <typename T, typename TO>
 void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchunks, const T* ws,
                           int64_t wsstride, TO* out, int64_t ystride, int nvec, int beta,
                           hipStream_t stream);
 void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream);
 void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hipStream_t stream);
-void launch_grouped_gemm_bf16x3(int bm, int bn, const Task* tasks, const Seg* segs,
-                                const Tile* tiles, int ntiles, const void* const* bases_hi,
-                                const void* const* bases_lo, int64_t xstride, int64_t bxstride,
-                                double* out_y, int64_t ystride, float* out_f, void* out_h,
-                                void* out_l, int64_t wsstride, bool ws_pair, int nvec, int beta,
-                                hipStream_t stream);
 
 namespace {
 
@@ -99,7 +104,7 @@ void build_tiles_plain(StageTables& st, const std::vector<double>& cost)
 	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
 	st.tiles.clear();
 	for (int t : order) {
-		const Task& tk = st.tasks[t];
+		const PTask& tk = st.tasks[t];
 		for (int m0 = 0; m0 < std::max(tk.m, 1); m0 += kTileM)
 			for (int n0 = 0; n0 < std::max(tk.n, 1); n0 += kTileN)
 				st.tiles.push_back(Tile { t, m0, n0, 0 });
@@ -120,7 +125,7 @@ void build_tiles_xcd(StageTables& st, const std::vector<double>& cost)
 	std::vector<std::vector<Tile>> lists(kXcds);
 	std::vector<double>            load(kXcds, 0.0);
 	for (int t : order) {
-		const Task& tk = st.tasks[t];
+		const PTask& tk = st.tasks[t];
 		const int   tr = std::max(1, (tk.m + kTileM - 1) / kTileM);
 		const int   tc = std::max(1, (tk.n + kTileN - 1) / kTileN);
 		const double per_tile = cost[t] / double(tr * tc);
@@ -345,29 +350,57 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	}
 	dg_begin_[size_t(np)] = int32_t(dg_off_.size());
 
-	// Packing. A panel of patch I: L_I x K_I, (J outer, k inner) general terms; then the
-	// identity-B terms' A blocks. B stack of (I, J): q*R_I x R_J for its q general terms;
-	// then identity-A terms' B blocks.
+	// Packing, K-major (rows padded to kPanelPad elements, zero padding):
+	//  * A panel of patch I, the stage-2 left operand of Y_I^T: row n (< L_I) holds the K_I
+	//    columns (J outer, k inner, as the reference's A panels) of its general terms;
+	//  * identity-B terms: the A block row major (stage-2 left operand);
+	//  * B stack of (I, J): rows t * R_I + i hold row i of B_k(t) for its q general terms
+	//    (stage-1 left operand);
+	//  * identity-A terms: the B block row major (stage-2 right operand B^T).
 	// Packing is planned first (offsets and copy jobs) and done afterwards in parallel into a
 	// pinned staging buffer, then uploaded with one asynchronous copy per pack.
 	struct Copy {
-		double*       dst_is_b; // nullptr: A pack, else B pack (the value is a tag only)
-		int64_t       dst;      // element offset in the pack
-		const double* src;
+		int           to_b;  // 0: A pack, 1: B pack
+		int64_t       dst;   // element offset of row 0 in the pack
+		const double* src;   // column-major rows x cols block
 		int64_t       rows, cols, ldsrc, lddst;
+		int64_t       tail;  // elements zeroed after each copied row (row padding)
 	};
 	std::vector<Copy> copies;
 	int64_t           asize = 0, bsize = 0;
-	double* const     BTAG  = reinterpret_cast<double*>(1);
-	std::vector<int64_t> apanel_off(np, 0), K(np, 0), bx_off(np + 1, 0);
+	std::vector<int64_t> apanel_off(np, 0), K(np, 0), ldap(np, 0), ldbx(np, 0), bx_off(np + 1, 0);
 	for (int64_t I = 0; I < np; ++I) {
 		for (int64_t J = 0; J < np; ++J)
 			for (int64_t k = 0; k < nC; ++k)
 				if (kind[idx(I, J, k)] == GENERAL)
 					K[I] += L[J];
-		bx_off[I + 1] = bx_off[I] + R[I] * K[I];
+		ldap[I]       = panel_ld(K[I]);
+		ldbx[I]       = panel_ld(K[I]);
+		bx_off[I + 1] = bx_off[I] + R[I] * ldbx[I];
 	}
 	stats_.bx_elems = bx_off[np];
+
+	// Input panels: XP_J (R_J x L_J, columns padded to panel_ld(R_J)) and XT_J (X_J transposed,
+	// rows padded to panel_ld(L_J)), with the 64 x 64 tiles of their preparation.
+	prep_patches_.assign(size_t(np), PrepPatch {});
+	prep_tiles_.clear();
+	xp_elems_ = 0;
+	xt_elems_ = 0;
+	for (int64_t J = 0; J < np; ++J) {
+		PrepPatch& pp = prep_patches_[size_t(J)];
+		pp.xoff       = off_[J];
+		pp.R          = int32_t(R[J]);
+		pp.L          = int32_t(L[J]);
+		pp.ldp        = int32_t(panel_ld(R[J]));
+		pp.ldt        = int32_t(panel_ld(L[J]));
+		pp.pofs       = xp_elems_;
+		pp.tofs       = xt_elems_;
+		xp_elems_ += int64_t(pp.ldp) * L[J];
+		xt_elems_ += int64_t(pp.ldt) * R[J];
+		for (int m0 = 0; m0 < pp.ldp; m0 += 64)
+			for (int c0 = 0; c0 < pp.ldt; c0 += 64)
+				prep_tiles_.push_back(PrepTile { int32_t(J), m0, c0, 0 });
+	}
 
 	t1_tasks_.clear();
 	t1_segs_.clear();
@@ -379,7 +412,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	for (int64_t I = 0; I < np; ++I) {
 		// A panel for the stage-2 GEMM of patch I.
 		apanel_off[I] = asize;
-		asize += L[I] * K[I];
+		asize += L[I] * ldap[I];
 		int64_t col = 0;
 		for (int64_t J = 0; J < np; ++J) {
 			// Stage-1 task for (I, J): the row-stacked B blocks of its general terms.
@@ -389,33 +422,34 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 					ks.push_back(k);
 			if (ks.empty())
 				continue;
-			const int64_t q       = int64_t(ks.size());
-			const int64_t boff    = bsize;
-			const int64_t ldstack = q * R[I];
-			bsize += ldstack * R[J];
+			const int64_t q    = int64_t(ks.size());
+			const int64_t boff = bsize;
+			const int64_t ldbs = panel_ld(R[J]);
+			bsize += q * R[I] * ldbs;
 			for (int64_t t = 0; t < q; ++t) {
-				const size_t id = idx(I, J, ks[t]);
-				copies.push_back(Copy { nullptr, apanel_off[I] + (col + t * L[J]) * L[I],
-				                        Amatrix[id], L[I], L[J], int64_t(ldA[id]), L[I] });
-				copies.push_back(Copy { BTAG, boff + t * R[I], Bmatrix[id], R[I], R[J],
-				                        int64_t(ldB[id]), ldstack });
+				const size_t  id   = idx(I, J, ks[t]);
+				const int64_t c0   = col + t * L[J];
+				const bool    last = (c0 + L[J] == K[I]);
+				copies.push_back(Copy { 0, apanel_off[I] + c0, Amatrix[id], L[I], L[J],
+				                        int64_t(ldA[id]), ldap[I], last ? ldap[I] - K[I] : 0 });
+				copies.push_back(Copy { 1, boff + t * R[I] * ldbs, Bmatrix[id], R[I], R[J],
+				                        int64_t(ldB[id]), ldbs, ldbs - R[J] });
 			}
-			Task tk {};
-			tk.coff      = bx_off[I] + col * R[I];
+			PTask tk {};
+			tk.coff      = bx_off[I] + col;
 			tk.m         = int32_t(q * R[I]);
 			tk.n         = int32_t(L[J]);
-			tk.ldc       = int32_t(R[I]);
 			tk.rsplit    = int32_t(R[I]);
+			tk.sm        = int32_t(ldbx[I]);
 			tk.seg_begin = int32_t(t1_segs_.size());
-			Seg s {};
-			s.lbase = BASE_BPACK;
+			PSeg s {};
+			s.lbase = PBASE_BPACK;
 			s.loff  = boff;
-			s.ldl   = int32_t(ldstack);
-			s.rbase = BASE_X;
-			s.roff  = off_[J];
-			s.ldr   = int32_t(R[J]);
+			s.ldl   = int32_t(ldbs);
+			s.rbase = PBASE_XP;
+			s.roff  = prep_patches_[size_t(J)].pofs;
+			s.ldr   = prep_patches_[size_t(J)].ldp;
 			s.k     = int32_t(R[J]);
-			s.rtrans = 0;
 			t1_segs_.push_back(s);
 			tk.seg_end  = int32_t(t1_segs_.size());
 			tk.out_is_y = 0;
@@ -427,26 +461,27 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			col += q * L[J];
 		}
 
-		// Stage-2 task for patch I.
-		Task tk {};
+		// Stage-2 task for patch I, computed as Y_I^T (L_I x R_I): element (r, c) of the task is
+		// Y_I(c, r), at off_I + c + r * R_I.
+		PTask tk {};
 		tk.coff      = off_[I];
-		tk.m         = int32_t(R[I]);
-		tk.n         = int32_t(L[I]);
-		tk.ldc       = int32_t(R[I]);
-		tk.rsplit    = int32_t(R[I]);
+		tk.m         = int32_t(L[I]);
+		tk.n         = int32_t(R[I]);
+		tk.rsplit    = int32_t(L[I]);
+		tk.sm        = int32_t(R[I]);
 		tk.seg_begin = int32_t(t2_segs_.size());
 		tk.out_is_y  = 1;
 		double cost  = 0;
 		if (K[I] > 0) {
-			Seg s {};
-			s.lbase  = BASE_BX;
-			s.loff   = bx_off[I];
-			s.ldl    = int32_t(R[I]);
-			s.rbase  = BASE_APACK;
-			s.roff   = apanel_off[I];
-			s.ldr    = int32_t(L[I]);
-			s.k      = int32_t(K[I]);
-			s.rtrans = 1;
+			// Y_I^T += A_I * BX_I^T.
+			PSeg s {};
+			s.lbase = PBASE_APACK;
+			s.loff  = apanel_off[I];
+			s.ldl   = int32_t(ldap[I]);
+			s.rbase = PBASE_BX;
+			s.roff  = bx_off[I];
+			s.ldr   = int32_t(ldbx[I]);
+			s.k     = int32_t(K[I]);
 			t2_segs_.push_back(s);
 			cost += 2.0 * double(R[I]) * double(L[I]) * double(K[I]);
 			stats_.bytes_bx += 2.0 * 8.0 * double(R[I]) * double(K[I]);
@@ -455,37 +490,37 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			for (int64_t k = 0; k < nC; ++k) {
 				const size_t id = idx(I, J, k);
 				if (kind[id] == IDENT_B) {
-					// Y_I += X_J * A^T: L = X_J (R_J x L_J), R = A (L_I x L_J) transposed.
+					// Y_I += X_J * A^T, i.e. Y_I^T += A * X_J^T: left A (L_I x L_J), right XT_J.
+					const int64_t lda  = panel_ld(L[J]);
 					const int64_t aoff = asize;
-					asize += L[I] * L[J];
-					copies.push_back(Copy { nullptr, aoff, Amatrix[id], L[I], L[J],
-					                        int64_t(ldA[id]), L[I] });
-					Seg s {};
-					s.lbase  = BASE_X;
-					s.loff   = off_[J];
-					s.ldl    = int32_t(R[J]);
-					s.rbase  = BASE_APACK;
-					s.roff   = aoff;
-					s.ldr    = int32_t(L[I]);
-					s.k      = int32_t(L[J]);
-					s.rtrans = 1;
+					asize += L[I] * lda;
+					copies.push_back(Copy { 0, aoff, Amatrix[id], L[I], L[J], int64_t(ldA[id]),
+					                        lda, lda - L[J] });
+					PSeg s {};
+					s.lbase = PBASE_APACK;
+					s.loff  = aoff;
+					s.ldl   = int32_t(lda);
+					s.rbase = PBASE_XT;
+					s.roff  = prep_patches_[size_t(J)].tofs;
+					s.ldr   = prep_patches_[size_t(J)].ldt;
+					s.k     = int32_t(L[J]);
 					t2_segs_.push_back(s);
 					cost += 2.0 * double(R[I]) * double(L[I]) * double(L[J]);
 				} else if (kind[id] == IDENT_A) {
-					// Y_I += B * X_J: L = B (R_I x R_J), R = X_J (R_J x L_J).
+					// Y_I += B * X_J, i.e. Y_I^T += X_J^T * B^T: left XP_J, right B (R_I x R_J).
+					const int64_t ldb  = panel_ld(R[J]);
 					const int64_t boff = bsize;
-					bsize += R[I] * R[J];
-					copies.push_back(Copy { BTAG, boff, Bmatrix[id], R[I], R[J],
-					                        int64_t(ldB[id]), R[I] });
-					Seg s {};
-					s.lbase  = BASE_BPACK;
-					s.loff   = boff;
-					s.ldl    = int32_t(R[I]);
-					s.rbase  = BASE_X;
-					s.roff   = off_[J];
-					s.ldr    = int32_t(R[J]);
-					s.k      = int32_t(R[J]);
-					s.rtrans = 0;
+					bsize += R[I] * ldb;
+					copies.push_back(Copy { 1, boff, Bmatrix[id], R[I], R[J], int64_t(ldB[id]),
+					                        ldb, ldb - R[J] });
+					PSeg s {};
+					s.lbase = PBASE_XP;
+					s.loff  = prep_patches_[size_t(J)].pofs;
+					s.ldl   = prep_patches_[size_t(J)].ldp;
+					s.rbase = PBASE_BPACK;
+					s.roff  = boff;
+					s.ldr   = int32_t(ldb);
+					s.k     = int32_t(R[J]);
 					t2_segs_.push_back(s);
 					cost += 2.0 * double(R[I]) * double(R[J]) * double(L[I]);
 				}
@@ -513,10 +548,23 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	double*      ha     = static_cast<double*>(pool_host_alloc(abytes + bbytes));
 	double*      hb     = ha + std::max<int64_t>(asize, 1);
 	parallel_for(copies.size(), [&](size_t t) {
+		// Row-major destination from a column-major source, in 32 x 32 blocks.
 		const Copy& c = copies[t];
-		double*     d = (c.dst_is_b ? hb : ha) + c.dst;
-		for (int64_t j = 0; j < c.cols; ++j)
-			std::memcpy(d + j * c.lddst, c.src + j * c.ldsrc, size_t(c.rows) * sizeof(double));
+		double*     d = (c.to_b ? hb : ha) + c.dst;
+		for (int64_t i0 = 0; i0 < c.rows; i0 += 32) {
+			const int64_t i1 = std::min(c.rows, i0 + 32);
+			for (int64_t j0 = 0; j0 < c.cols; j0 += 32) {
+				const int64_t j1 = std::min(c.cols, j0 + 32);
+				for (int64_t j = j0; j < j1; ++j) {
+					const double* sc = c.src + j * c.ldsrc;
+					for (int64_t i = i0; i < i1; ++i)
+						d[i * c.lddst + j] = sc[i];
+				}
+			}
+			if (c.tail > 0)
+				for (int64_t i = i0; i < i1; ++i)
+					std::memset(d + i * c.lddst + c.cols, 0, size_t(c.tail) * sizeof(double));
+		}
 	});
 	d_apack_ = static_cast<double*>(pool_device_alloc(abytes));
 	d_bpack_ = static_cast<double*>(pool_device_alloc(bbytes));
@@ -527,6 +575,8 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	DMK_HIP_CHECK(e1);
 	DMK_HIP_CHECK(e2);
 	DMK_HIP_CHECK(e3);
+	d_prep_tiles_   = dev_alloc_copy(prep_tiles_);
+	d_prep_patches_ = dev_alloc_copy(prep_patches_);
 	// Per precision mode: a tuning the autotuner chose for this histogram earlier in the
 	// process, else the default.
 	const int p0 = precision_;
@@ -555,7 +605,9 @@ Plan::~Plan()
 	pool_device_free(d_bpack16_);
 	pool_device_free(d_bx_);
 	pool_device_free(d_ws_);
-	pool_device_free(d_x32_);
+	pool_device_free(d_xin_);
+	pool_device_free(d_prep_tiles_);
+	pool_device_free(d_prep_patches_);
 	pool_device_free(d_reduce_);
 	pool_device_free(d_chunks_);
 	pool_device_free(d_dg_);
@@ -625,7 +677,7 @@ std::string Plan::histogram_key() const
 		items.push_back(std::to_string(bin(t2_tasks_[t].m)) + "." + std::to_string(bin(t2_tasks_[t].n))
 		                + "." + std::to_string(bin(k)));
 	}
-	for (const Task& tk : t1_tasks_)
+	for (const PTask& tk : t1_tasks_)
 		items.push_back("s" + std::to_string(bin(tk.m)) + "." + std::to_string(bin(tk.n)));
 	std::sort(items.begin(), items.end());
 	std::string key;
@@ -669,15 +721,18 @@ void Plan::retune(const Tuning& t)
 	int64_t             ws = 0;
 	stats_.split_tasks     = 0;
 	for (size_t ti = 0; ti < t2_tasks_.size(); ++ti) {
-		const Task& tk   = t2_tasks_[ti];
-		int64_t     ktot = 0;
+		const PTask& tk = t2_tasks_[ti];
+		// The task's segments laid end to end along K, each starting at a multiple of 16, so that
+		// cuts at multiples of 16 fall at multiples of 16 within every segment (piece rows stay
+		// 16-byte aligned and every piece but a segment's last ends on a whole vector).
+		int64_t ktot = 0;
 		for (int s = tk.seg_begin; s < tk.seg_end; ++s)
-			ktot += t2_segs_[size_t(s)].k;
+			ktot = (ktot + 15) / 16 * 16 + t2_segs_[size_t(s)].k;
 		int nsplit = 1;
 		if (t.splitk > 0 && ktot >= 2 * int64_t(t.splitk))
 			nsplit = int(std::min<int64_t>(16, ktot / t.splitk));
 		if (nsplit <= 1) {
-			Task c      = tk;
+			PTask c     = tk;
 			c.seg_begin = int32_t(st2_.segs.size());
 			for (int s = tk.seg_begin; s < tk.seg_end; ++s)
 				st2_.segs.push_back(t2_segs_[size_t(s)]);
@@ -687,6 +742,8 @@ void Plan::retune(const Tuning& t)
 			continue;
 		}
 		stats_.split_tasks++;
+		// Partials in the output's own element order (element (r, c) at c + r * n), so the
+		// reduction is elementwise.
 		const int64_t count = int64_t(tk.m) * int64_t(tk.n);
 		Reduce        rd {};
 		rd.yoff   = tk.coff;
@@ -694,32 +751,33 @@ void Plan::retune(const Tuning& t)
 		rd.count  = count;
 		rd.nsplit = nsplit;
 		reduce_.push_back(rd);
-		// Cut the concatenated K range [0, ktot) at multiples of 16.
+		// Cut the laid-out K range [0, ktot) at multiples of 16.
 		std::vector<int64_t> cuts(size_t(nsplit) + 1, 0);
 		for (int p = 1; p < nsplit; ++p)
 			cuts[size_t(p)] = ((ktot * p / nsplit) / 16) * 16;
 		cuts[size_t(nsplit)] = ktot;
 		for (int p = 0; p < nsplit; ++p) {
-			Task c      = tk;
+			PTask c     = tk;
 			c.coff      = ws + int64_t(p) * count;
-			c.ldc       = tk.m;
 			c.rsplit    = tk.m;
+			c.sm        = tk.n;
 			c.out_is_y  = 0;
 			c.seg_begin = int32_t(st2_.segs.size());
-			int64_t base = 0; // K offset of the current segment in the concatenation
+			int64_t end = 0; // end of the previous segment in the layout
 			for (int s = tk.seg_begin; s < tk.seg_end; ++s) {
-				const Seg& sg = t2_segs_[size_t(s)];
-				const int64_t a = std::max(cuts[size_t(p)], base);
-				const int64_t b = std::min(cuts[size_t(p) + 1], base + sg.k);
+				const PSeg&   sg   = t2_segs_[size_t(s)];
+				const int64_t base = (end + 15) / 16 * 16; // this segment's start
+				const int64_t a    = std::max(cuts[size_t(p)], base);
+				const int64_t b  = std::min(cuts[size_t(p) + 1], base + sg.k);
 				if (b > a) {
-					Seg piece = sg;
-					const int64_t ka = a - base;
-					piece.k          = int32_t(b - a);
-					piece.loff += ka * sg.ldl;                       // L is M x K: column ka
-					piece.roff += sg.rtrans ? ka * sg.ldr : ka;      // R is N x K or K x N
+					PSeg          piece = sg;
+					const int64_t ka    = a - base; // both operands are K-major: shift k by ka
+					piece.k             = int32_t(b - a);
+					piece.loff += ka;
+					piece.roff += ka;
 					st2_.segs.push_back(piece);
 				}
-				base += sg.k;
+				end = base + sg.k;
 			}
 			c.seg_end = int32_t(st2_.segs.size());
 			st2_.tasks.push_back(c);
@@ -731,8 +789,10 @@ void Plan::retune(const Tuning& t)
 	build_tiles(st2_, cost, t.xcd);
 	chunks_.clear();
 	for (size_t r = 0; r < reduce_.size(); ++r)
-		for (int64_t s0 = 0; s0 < reduce_[r].count; s0 += 4096)
-			chunks_.push_back(ReduceChunk { int32_t(r), int32_t(std::min<int64_t>(4096, reduce_[r].count - s0)), s0 });
+		for (int64_t s0 = 0; s0 < reduce_[r].count; s0 += kReduceChunk)
+			chunks_.push_back(ReduceChunk { int32_t(r),
+			                                int32_t(std::min<int64_t>(kReduceChunk, reduce_[r].count - s0)),
+			                                s0 });
 	pool_device_free(d_reduce_);
 	pool_device_free(d_chunks_);
 	d_reduce_ = dev_alloc_copy(reduce_);
@@ -749,33 +809,48 @@ void Plan::retune(const Tuning& t)
 
 void Plan::ensure_scratch(int nvec)
 {
+	// FP64: doubles; FP32: floats; BF16x3: (hi, lo) bf16 arrays, 4 bytes per element in all.
 	const size_t elem  = (precision_ == PREC_FP64) ? sizeof(double) : sizeof(float);
 	const size_t bytes = size_t(std::max<int64_t>(stats_.bx_elems, 1)) * elem * size_t(nvec);
-	if (bytes > bx_bytes_) {
+	const size_t wbytes = size_t(std::max<int64_t>(stats_.ws_elems, 1)) * elem * size_t(nvec);
+	const size_t xbytes = size_t(std::max<int64_t>(xp_elems_ + xt_elems_, 1)) * elem * size_t(nvec);
+	const bool   grow_bx = bytes > bx_bytes_;
+	if (grow_bx || wbytes > ws_bytes_ || xbytes > xin_bytes_ || bx_prec_ != precision_) {
+		hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+		DMK_HIP_CHECK(hipStreamIsCapturing(stream_, &cs));
+		if (cs != hipStreamCaptureStatusNone)
+			throw std::runtime_error("dmk: matvec scratch must be set up before stream capture "
+			                         "(apply once outside the capture)");
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+	}
+	if (grow_bx) {
 		pool_device_free(d_bx_);
 		d_bx_     = nullptr;
 		bx_bytes_ = 0;
 		d_bx_     = pool_device_alloc(bytes);
 		bx_bytes_ = bytes;
+		bx_prec_  = -1;
 	}
-	const size_t wbytes = size_t(std::max<int64_t>(stats_.ws_elems, 1)) * elem * size_t(nvec);
-	if (wbytes > ws_bytes_) {
+	if (bx_prec_ != precision_) {
+		// The stage-2 panels are read in whole 16-byte vectors past the end of each row's K
+		// range: the row padding must hold zeros in this precision's layout.
+		DMK_HIP_CHECK(hipMemsetAsync(d_bx_, 0, bx_bytes_, stream_));
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+		bx_prec_ = precision_;
+	}
+	if (wbytes > ws_bytes_) {
 		pool_device_free(d_ws_);
 		d_ws_     = nullptr;
 		ws_bytes_ = 0;
 		d_ws_     = pool_device_alloc(wbytes);
 		ws_bytes_ = wbytes;
 	}
-	if (precision_ != PREC_FP64 && nvec > x32_cap_) {
-		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
-		pool_device_free(d_x32_);
-		d_x32_   = nullptr;
-		x32_cap_ = 0;
-		d_x32_   = static_cast<float*>(pool_device_alloc(size_t(std::max<int64_t>(stats_.n, 1))
-		                                                 * sizeof(float) * size_t(nvec)));
-		x32_cap_ = nvec;
+	if (xbytes > xin_bytes_) {
+		pool_device_free(d_xin_);
+		d_xin_     = nullptr;
+		xin_bytes_ = 0;
+		d_xin_     = pool_device_alloc(xbytes);
+		xin_bytes_ = xbytes;
 	}
 }
 
@@ -825,66 +900,75 @@ void Plan::set_precision(int p)
 	ensure_scratch(1);
 }
 
-// Stage 1, stage 2 and the split-k reduction on the plan's stream; an event is recorded
-// between the stages when mid is given.
+// Input preparation, stage 1, stage 2 and the split-k reduction on the plan's stream; an event
+// is recorded between the stages when mid is given.
 void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec,
                   int beta, hipEvent_t mid)
 {
-	const int64_t bxs = stats_.bx_elems;
-	const int64_t wss = stats_.ws_elems;
+	const int64_t  bxs = stats_.bx_elems;
+	const int64_t  wss = stats_.ws_elems;
+	const int      npt = int(prep_tiles_.size());
+	const PStrides vs { xp_elems_, xt_elems_, bxs };
+	const int      v1  = tuning_.mfma32;
 	if (precision_ == PREC_FP64) {
-		double*       bx                = static_cast<double*>(d_bx_);
-		double*       ws                = static_cast<double*>(d_ws_);
-		const double* bases[BASE_COUNT] = { d_apack_, d_bpack_, vin, bx };
-		launch_grouped_gemm<double, double>(st1_.bm, st1_.bn, st1_.d_tasks, st1_.d_segs,
-		                                    st1_.d_tiles, int(st1_.tiles.size()), bases, ldin, bxs,
-		                                    vout, ldout, bx, bxs, nvec, 0, stream_);
+		double* bx = static_cast<double*>(d_bx_);
+		double* ws = static_cast<double*>(d_ws_);
+		double* xp = static_cast<double*>(d_xin_);
+		double* xt = xp + int64_t(nvec) * xp_elems_;
+		launch_prep_x<double>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
+		                      stream_);
+		const double* bases[PBASE_COUNT] = { d_apack_, d_bpack_, xp, xt, bx };
+		launch_panel_gemm<double, double>(st1_.bm, st1_.bn, 0, st1_.d_tasks, st1_.d_segs,
+		                                  st1_.d_tiles, int(st1_.tiles.size()), bases, vs, vout,
+		                                  ldout, bx, bxs, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
-		launch_grouped_gemm<double, double>(st2_.bm, st2_.bn, st2_.d_tasks, st2_.d_segs,
-		                                    st2_.d_tiles, int(st2_.tiles.size()), bases, ldin, bxs,
-		                                    vout, ldout, ws, wss, nvec, beta, stream_);
+		launch_panel_gemm<double, double>(st2_.bm, st2_.bn, 0, st2_.d_tasks, st2_.d_segs,
+		                                  st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
+		                                  ldout, ws, wss, nvec, beta, stream_);
 		launch_splitk_reduce<double, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                     vout, ldout, nvec, beta, stream_);
 	} else if (precision_ == PREC_BF16X3) {
-		// Input split into (hi, lo) bf16 pairs in the FP32 input buffer (same byte size).
-		uint16_t*     xh = reinterpret_cast<uint16_t*>(d_x32_);
-		uint16_t*     xl = xh + int64_t(nvec) * stats_.n;
-		uint16_t*     bh = static_cast<uint16_t*>(d_bx_);
-		uint16_t*     bl = bh + int64_t(nvec) * bxs;
+		// (hi, lo) bf16 arrays: hi parts of all vectors, then the lo parts.
+		uint16_t*     xph = static_cast<uint16_t*>(d_xin_);
+		uint16_t*     xpl = xph + int64_t(nvec) * xp_elems_;
+		uint16_t*     xth = xpl + int64_t(nvec) * xp_elems_;
+		uint16_t*     xtl = xth + int64_t(nvec) * xt_elems_;
+		uint16_t*     bh  = static_cast<uint16_t*>(d_bx_);
+		uint16_t*     bl  = bh + int64_t(nvec) * bxs;
 		const int64_t na = stats_.apack_elems, nb = stats_.bpack_elems;
-		for (int v = 0; v < nvec; ++v)
-			launch_split_f64_bf16x2(vin + v * ldin, xh + v * stats_.n, xl + v * stats_.n, stats_.n,
-			                        stream_);
-		const void* hi[BASE_COUNT] = { d_apack16_, d_bpack16_, xh, bh };
-		const void* lo[BASE_COUNT] = { d_apack16_ ? d_apack16_ + na : nullptr,
-			                       d_bpack16_ ? d_bpack16_ + nb : nullptr, xl, bl };
-		launch_grouped_gemm_bf16x3(st1_.bm, st1_.bn, st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
-		                           int(st1_.tiles.size()), hi, lo, stats_.n, bxs, vout, ldout,
-		                           nullptr, bh, bl, bxs, true, nvec, 0, stream_);
+		launch_prep_x_bf16x2(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xph, xpl, xth, xtl,
+		                     vs, nvec, stream_);
+		const void* hi[PBASE_COUNT] = { d_apack16_, d_bpack16_, xph, xth, bh };
+		const void* lo[PBASE_COUNT] = { d_apack16_ ? d_apack16_ + na : nullptr,
+			                        d_bpack16_ ? d_bpack16_ + nb : nullptr, xpl, xtl, bl };
+		launch_panel_bf16x3(st1_.bm, st1_.bn, v1, st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
+		                    int(st1_.tiles.size()), hi, lo, vs, vout, ldout, nullptr, bh, bl, bxs,
+		                    true, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
 		float* ws = static_cast<float*>(d_ws_);
-		launch_grouped_gemm_bf16x3(st2_.bm, st2_.bn, st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
-		                           int(st2_.tiles.size()), hi, lo, stats_.n, bxs, vout, ldout, ws,
-		                           nullptr, nullptr, wss, false, nvec, beta, stream_);
+		launch_panel_bf16x3(st2_.bm, st2_.bn, v1, st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
+		                    int(st2_.tiles.size()), hi, lo, vs, vout, ldout, ws, nullptr, nullptr,
+		                    wss, false, nvec, beta, stream_);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                    vout, ldout, nvec, beta, stream_);
 	} else {
-		for (int v = 0; v < nvec; ++v)
-			launch_cast_f64_to_f32(vin + v * ldin, d_x32_ + v * stats_.n, stats_.n, stream_);
-		float*       bx                = static_cast<float*>(d_bx_);
-		float*       ws                = static_cast<float*>(d_ws_);
-		const float* bases[BASE_COUNT] = { d_apack32_, d_bpack32_, d_x32_, bx };
-		launch_grouped_gemm<float, float>(st1_.bm, st1_.bn, st1_.d_tasks, st1_.d_segs,
-		                                  st1_.d_tiles, int(st1_.tiles.size()), bases, stats_.n,
-		                                  bxs, bx, bxs, bx, bxs, nvec, 0, stream_, tuning_.mfma32);
+		float* bx = static_cast<float*>(d_bx_);
+		float* ws = static_cast<float*>(d_ws_);
+		float* xp = static_cast<float*>(d_xin_);
+		float* xt = xp + int64_t(nvec) * xp_elems_;
+		launch_prep_x<float>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
+		                     stream_);
+		const float* bases[PBASE_COUNT] = { d_apack32_, d_bpack32_, xp, xt, bx };
+		launch_panel_gemm<float, float>(st1_.bm, st1_.bn, v1, st1_.d_tasks, st1_.d_segs,
+		                                st1_.d_tiles, int(st1_.tiles.size()), bases, vs, bx, bxs,
+		                                bx, bxs, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
-		launch_grouped_gemm<float, double>(st2_.bm, st2_.bn, st2_.d_tasks, st2_.d_segs,
-		                                   st2_.d_tiles, int(st2_.tiles.size()), bases, stats_.n,
-		                                   bxs, vout, ldout, ws, wss, nvec, beta, stream_,
-		                                   tuning_.mfma32);
+		launch_panel_gemm<float, double>(st2_.bm, st2_.bn, v1, st2_.d_tasks, st2_.d_segs,
+		                                 st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
+		                                 ldout, ws, wss, nvec, beta, stream_);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                    vout, ldout, nvec, beta, stream_);
 	}

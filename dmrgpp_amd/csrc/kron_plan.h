@@ -62,16 +62,16 @@ struct Tuning {
 };
 
 struct StageTables {
-	int               bm = 64; // output tile shape of this stage's launch
-	int               bn = 64;
-	std::vector<Task> tasks;
-	std::vector<Seg>  segs;
-	std::vector<Tile> tiles;
-	Task*             d_tasks = nullptr;
-	Seg*              d_segs  = nullptr;
-	Tile*             d_tiles = nullptr;
-	void              upload();
-	void              release();
+	int                bm = 64; // output tile shape of this stage's launch
+	int                bn = 64;
+	std::vector<PTask> tasks;
+	std::vector<PSeg>  segs;
+	std::vector<Tile>  tiles;
+	PTask*             d_tasks = nullptr;
+	PSeg*              d_segs  = nullptr;
+	Tile*              d_tiles = nullptr;
+	void               upload();
+	void               release();
 };
 
 // Process-wide autotuner cache (histogram signature -> tuning).
@@ -138,9 +138,17 @@ private:
 	hipStream_t stream_ = nullptr;
 
 	// Unsplit tasks of both stages as planned from the blocks (retune builds the tables).
-	std::vector<Task>   t1_tasks_, t2_tasks_;
-	std::vector<Seg>    t1_segs_, t2_segs_;
+	std::vector<PTask>  t1_tasks_, t2_tasks_;
+	std::vector<PSeg>   t1_segs_, t2_segs_;
 	std::vector<double> cost1_, cost2_;
+
+	// Per-call input preparation (XP / XT panels of the input vector).
+	std::vector<PrepTile>  prep_tiles_;
+	std::vector<PrepPatch> prep_patches_;
+	PrepTile*              d_prep_tiles_   = nullptr;
+	PrepPatch*             d_prep_patches_ = nullptr;
+	int64_t                xp_elems_       = 0;
+	int64_t                xt_elems_       = 0;
 
 	std::vector<Reduce>      reduce_;
 	std::vector<ReduceChunk> chunks_;
@@ -156,10 +164,11 @@ private:
 	uint16_t*            d_bpack16_ = nullptr;
 	void*                d_bx_      = nullptr; // stage-1 products, bx_bytes_ bytes
 	void*                d_ws_      = nullptr; // split-k partials, ws_bytes_ bytes
-	float*               d_x32_     = nullptr; // FP32 copy of the input, x32_cap_ vectors
-	int                  x32_cap_   = 0;       // vectors d_x32_ holds
+	void*                d_xin_     = nullptr; // XP and XT panels of the input, xin_bytes_
 	size_t               bx_bytes_  = 0;
 	size_t               ws_bytes_  = 0;
+	size_t               xin_bytes_ = 0;
+	int                  bx_prec_   = -1; // precision whose zero padding d_bx_ holds
 
 	hipEvent_t ev_[3] = { nullptr, nullptr, nullptr };
 

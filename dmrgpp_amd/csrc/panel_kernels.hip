@@ -1,0 +1,768 @@
+// Grouped multi-segment GEMM on K-major operand panels for CDNA4 (gfx950): the kernels of the
+// superblock matvec in all three precision modes, and the per-call preparation of the input.
+//
+// One launch computes a list of independent output matrices ("tasks"), each the sum of one or
+// more products ("segments") C += L * R. The matvec uses two launches (kron_plan.cpp):
+//   stage 1: for every (I,J) with general terms, [B_k1; B_k2; ...] * X_J (row-stacked B
+//            blocks), written into the stage-2 panel BX_I;
+//   stage 2: Y_I^T = A_I * BX_I^T + A * X_J^T (identity right factor) + X_J^T * B^T (identity
+//            left factor), i.e. the transpose of the output patch, so that consecutive lanes
+//            write consecutive elements of the column-major Y_I.
+// This is the two-stage evaluation of GPUPlugin/apply_Htarget_sparse_not_ready.cpp:199-449 with
+// the identity factors of H_L (x) I and I (x) H_R contracted directly.
+//
+// Every operand is K-major with rows padded to a multiple of 8 elements and zero padding:
+// element (m, k) of a left operand at loff + k + m * ldl, element (k, n) of a right operand at
+// roff + k + n * ldr. The planner packs the factors that way, the input vector is rewritten that
+// way on every call (prep_x_kernel: X_J column major with padded columns, and X_J transposed),
+// and stage 1 writes BX_I that way. So every global load is a 16-byte vector along k, every LDS
+// store and fragment read a 16-byte vector, and the only edge handling is a row clamp (rows past
+// the task edge are computed and dropped) and a vector mask at the end of a segment's K range.
+//
+// Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a BM x BN output tile and
+// each wave a (BM/2) x (BN/2) sub-tile of MFMA blocks. K is walked in chunks of BK staged
+// through LDS, double buffered: the global loads of chunk c+1 are issued before the MFMAs of
+// chunk c. LDS rows hold the BK elements of one operand row as 16-byte slots, XOR-swizzled by
+// row (slot kv of row r at kv ^ swz(r)) so that the 16 rows one 16-lane pass reads fall on
+// distinct banks, without padding.
+// Within a chunk the lanes' k order is permuted (lane group g takes k = g * NF + s at MFMA step
+// s) so that a lane's operands for all NF steps are NF consecutive elements of one LDS row; the
+// sum over k is the same. Every output tile is owned by one workgroup and its contraction order
+// is fixed: results are bitwise reproducible (no atomics; split-k partials are summed in a fixed
+// order by splitk_reduce_kernel).
+#include "dmk_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <type_traits>
+
+namespace dmk {
+
+namespace {
+
+constexpr int kPThreads = 256;
+
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float  f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+template <typename T> struct Vec16;
+template <> struct Vec16<double> {
+	using type = d2_t;
+};
+template <> struct Vec16<float> {
+	using type = f4_t;
+};
+template <> struct Vec16<__bf16> {
+	using type = bf8_t;
+};
+
+// MFMA forms. kM x kM output block, kK-deep contraction per instruction; lane l supplies
+// A[l % kM][g] and B[g][l % kM] for the k index g = l / kM (the bf16 forms: 8 consecutive k
+// from 8 * (l / kM)); register r of the result holds row row(l, r), column l % kM
+// (layouts checked on gfx950).
+struct OpF64 {
+	using T                    = double;
+	static constexpr int kM    = 16;
+	static constexpr int kK    = 4;
+	static constexpr int kRegs = 4;
+	using acc_t                = double __attribute__((ext_vector_type(4)));
+	__device__ static inline acc_t mma(double a, double b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+struct OpF32 {
+	using T                    = float;
+	static constexpr int kM    = 16;
+	static constexpr int kK    = 4;
+	static constexpr int kRegs = 4;
+	using acc_t                = float __attribute__((ext_vector_type(4)));
+	__device__ static inline acc_t mma(float a, float b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+struct OpF32x32 {
+	using T                    = float;
+	static constexpr int kM    = 32;
+	static constexpr int kK    = 2;
+	static constexpr int kRegs = 16;
+	using acc_t                = float __attribute__((ext_vector_type(16)));
+	__device__ static inline acc_t mma(float a, float b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r)
+	{
+		return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+	}
+};
+
+struct OpBf16 {
+	static constexpr int kM    = 16;
+	static constexpr int kK    = 32; // 8 consecutive k per lane, 4 lane groups
+	static constexpr int kRegs = 4;
+	using acc_t                = float __attribute__((ext_vector_type(4)));
+	__device__ static inline acc_t mma(bf8_t a, bf8_t b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+struct OpBf16x32 {
+	static constexpr int kM    = 32;
+	static constexpr int kK    = 16; // 8 consecutive k per lane, 2 lane groups
+	static constexpr int kRegs = 16;
+	using acc_t                = float __attribute__((ext_vector_type(16)));
+	__device__ static inline acc_t mma(bf8_t a, bf8_t b, acc_t c)
+	{
+		return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+	}
+	__device__ static inline int row(int lane, int r)
+	{
+		return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+	}
+};
+
+template <typename T> struct PBasesT {
+	const T* p[PBASE_COUNT];
+};
+
+// Address of element (r, c) of a task's output.
+__device__ inline int64_t out_addr(const PTask& tk, int r, int c)
+{
+	const int rr  = r % tk.rsplit;
+	const int blk = r / tk.rsplit;
+	return tk.coff + int64_t(rr) * tk.sm + int64_t(blk) * tk.n + c;
+}
+
+// Advances (seg, k0) to the next K chunk of the task; false past the last one.
+template <int BK>
+__device__ inline bool next_chunk(const PSeg* __restrict__ segs, int seg_end, int& seg, int& k0)
+{
+	k0 += BK;
+	while (seg < seg_end && k0 >= segs[seg].k) {
+		++seg;
+		k0 = 0;
+	}
+	return seg < seg_end;
+}
+
+// Per-thread load geometry of one operand tile (ROWS x BK elements, VW-element vectors): thread
+// t loads vectors t + 256 q, i.e. rows t / VPR + (256 / VPR) q at vector column t % VPR.
+template <typename E, int ROWS, int BK> struct TileLoad {
+	using vec_t                = typename Vec16<E>::type;
+	static constexpr int VW    = 16 / int(sizeof(E));
+	static constexpr int VPR   = BK / VW;
+	static constexpr int NV    = ROWS * VPR / kPThreads;
+	static constexpr int RSTEP = kPThreads / VPR;
+	static_assert(NV >= 1 && ROWS * VPR % kPThreads == 0, "tile / thread mismatch");
+	const E* p[NV]; // row pointers of the current segment (column kv * VW)
+
+	__device__ inline void setup(const E* base, int ld, int row0, int nrows, int tid)
+	{
+		const int r  = tid / VPR;
+		const int kv = tid % VPR;
+#pragma unroll
+		for (int q = 0; q < NV; ++q) {
+			const int row = min(row0 + r + RSTEP * q, nrows - 1); // clamp: edge rows are dropped
+			p[q]          = base + int64_t(row) * ld + kv * VW;
+		}
+	}
+	__device__ inline void load(vec_t (&v)[NV], int k0, int K, int tid) const
+	{
+		if (k0 + BK <= K) {
+#pragma unroll
+			for (int q = 0; q < NV; ++q)
+				v[q] = *reinterpret_cast<const vec_t*>(p[q] + k0);
+		} else {
+			const bool ok = k0 + (tid % VPR) * VW < K; // rows are zero padded past K
+#pragma unroll
+			for (int q = 0; q < NV; ++q)
+				v[q] = ok ? *reinterpret_cast<const vec_t*>(p[q] + k0) : vec_t {};
+		}
+	}
+	// Slot of vector kv of LDS row r: rows r and r + 16 / VPR start on the same bank, so they
+	// take different XOR masks.
+	__device__ static inline int slot(int r, int kv) { return kv ^ ((r * VPR / 16) & (VPR - 1)); }
+	__device__ inline void store(vec_t* __restrict__ s, const vec_t (&v)[NV], int tid) const
+	{
+		const int r  = tid / VPR;
+		const int kv = tid % VPR;
+#pragma unroll
+		for (int q = 0; q < NV; ++q) {
+			const int row          = r + RSTEP * q;
+			s[row * VPR + slot(row, kv)] = v[q];
+		}
+	}
+};
+
+// FP64 / FP32: T operands, TO output for tasks writing the caller's vector (the FP32 mode writes
+// its stage-2 output in FP64 for the FP64 Lanczos recurrence). Workspace tasks (stage-1 panels,
+// split-k partials) store T and always overwrite.
+template <typename TO, int BM, int BN, int BK, typename Op>
+__global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
+    const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
+    PBasesT<typename Op::T> bases, PStrides vs, TO* __restrict__ out_y, int64_t ystride,
+    typename Op::T* __restrict__ out_ws, int64_t wsstride, int beta)
+{
+	using T     = typename Op::T;
+	using acc_t = typename Op::acc_t;
+	using LA    = TileLoad<T, BM, BK>;
+	using LB    = TileLoad<T, BN, BK>;
+	using vec_t = typename LA::vec_t;
+	constexpr int VW  = LA::VW;
+	constexpr int VPR = LA::VPR; // 16-byte slots per LDS row
+	constexpr int WM = BM / 2, WN = BN / 2;
+	constexpr int FI = WM / Op::kM, FJ = WN / Op::kM;
+	constexpr int NF = BK / Op::kK; // consecutive k per lane per chunk
+	static_assert(NF % VW == 0, "fragment / vector mismatch");
+
+	__shared__ vec_t As[2][BM * VPR];
+	__shared__ vec_t Bs[2][BN * VPR];
+
+	const Tile tl = tiles[blockIdx.x];
+	if (tl.task < 0)
+		return; // padding slot of the XCD-aware order
+	const PTask tk  = tasks[tl.task];
+	const int   tid = threadIdx.x;
+	const int   vec = blockIdx.y;
+
+	const T* b0   = bases.p[PBASE_APACK];
+	const T* b1   = bases.p[PBASE_BPACK];
+	const T* b2   = bases.p[PBASE_XP] + vec * vs.xp;
+	const T* b3   = bases.p[PBASE_XT] + vec * vs.xt;
+	const T* b4   = bases.p[PBASE_BX] + vec * vs.bx;
+	auto     pick = [&](int id) -> const T* {
+                return id == PBASE_APACK ? b0 : id == PBASE_BPACK ? b1 : id == PBASE_XP ? b2 : id == PBASE_XT ? b3 : b4;
+	};
+	out_y += vec * ystride;
+	out_ws += vec * wsstride;
+
+	const int lane = tid & 63;
+	const int wave = tid >> 6;
+	const int wm   = (wave & 1) * WM;
+	const int wn   = (wave >> 1) * WN;
+	const int g    = lane / Op::kM; // lane group: k slice g * NF .. g * NF + NF - 1
+	const int lr   = lane % Op::kM;
+
+	acc_t acc[FI][FJ];
+#pragma unroll
+	for (int i = 0; i < FI; ++i)
+#pragma unroll
+		for (int j = 0; j < FJ; ++j)
+			acc[i][j] = acc_t {};
+
+	int seg = tk.seg_begin;
+	int k0  = 0;
+	while (seg < tk.seg_end && segs[seg].k <= 0)
+		++seg;
+
+	if (seg < tk.seg_end) {
+		LA    la;
+		LB    lb;
+		vec_t va[LA::NV], vb[LB::NV];
+		PSeg  s = segs[seg];
+		la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
+		lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
+		la.load(va, k0, s.k, tid);
+		lb.load(vb, k0, s.k, tid);
+		la.store(As[0], va, tid);
+		lb.store(Bs[0], vb, tid);
+		__syncthreads();
+		int buf = 0;
+		for (;;) {
+			int        nseg    = seg;
+			int        nk0     = k0;
+			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, nseg, nk0);
+			if (hasNext) {
+				if (nseg != seg) {
+					s = segs[nseg];
+					la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
+					lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
+				}
+				la.load(va, nk0, s.k, tid);
+				lb.load(vb, nk0, s.k, tid);
+			}
+			// Fragments: NF consecutive k of one LDS row per operand block.
+			T fa[FI][NF], fb[FJ][NF];
+#pragma unroll
+			for (int i = 0; i < FI; ++i) {
+				const int r = wm + Op::kM * i + lr;
+#pragma unroll
+				for (int v = 0; v < NF / VW; ++v) {
+					const vec_t x = As[buf][r * VPR + LA::slot(r, g * (NF / VW) + v)];
+#pragma unroll
+					for (int e = 0; e < VW; ++e)
+						fa[i][v * VW + e] = x[e];
+				}
+			}
+#pragma unroll
+			for (int j = 0; j < FJ; ++j) {
+				const int r = wn + Op::kM * j + lr;
+#pragma unroll
+				for (int v = 0; v < NF / VW; ++v) {
+					const vec_t x = Bs[buf][r * VPR + LB::slot(r, g * (NF / VW) + v)];
+#pragma unroll
+					for (int e = 0; e < VW; ++e)
+						fb[j][v * VW + e] = x[e];
+				}
+			}
+#pragma unroll
+			for (int st = 0; st < NF; ++st)
+#pragma unroll
+				for (int i = 0; i < FI; ++i)
+#pragma unroll
+					for (int j = 0; j < FJ; ++j)
+						acc[i][j] = Op::mma(fa[i][st], fb[j][st], acc[i][j]);
+			if (!hasNext)
+				break;
+			la.store(As[buf ^ 1], va, tid);
+			lb.store(Bs[buf ^ 1], vb, tid);
+			__syncthreads();
+			buf ^= 1;
+			seg = nseg;
+			k0  = nk0;
+		}
+	}
+
+	const bool to_y = tk.out_is_y != 0;
+#pragma unroll
+	for (int i = 0; i < FI; ++i)
+#pragma unroll
+		for (int j = 0; j < FJ; ++j)
+#pragma unroll
+			for (int r = 0; r < Op::kRegs; ++r) {
+				const int row = tl.m0 + wm + Op::kM * i + Op::row(lane, r);
+				const int col = tl.n0 + wn + Op::kM * j + lr;
+				if (row < tk.m && col < tk.n) {
+					const int64_t addr = out_addr(tk, row, col);
+					if (to_y) {
+						TO v = TO(acc[i][j][r]);
+						if (beta)
+							v += out_y[addr];
+						out_y[addr] = v;
+					} else {
+						out_ws[addr] = acc[i][j][r];
+					}
+				}
+			}
+}
+
+// BF16x3: every operand value v is held as hi = bf16(v), lo = bf16(v - hi) in two arrays of the
+// same layout, and a product is lo_a*hi_b + hi_a*lo_b + hi_a*hi_b on bf16 MFMAs accumulating in
+// FP32 (lo*lo, ~2^-16 relative, is dropped). PAIR: workspace tasks store a (hi, lo) pair
+// (stage-1 panels) instead of FP32 (split-k partials). Stage-2 output to the caller is FP64.
+template <int BM, int BN, int BK, bool PAIR, typename Op>
+__global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
+    const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
+    PBasesT<__bf16> hi, PBasesT<__bf16> lo, PStrides vs, double* __restrict__ out_y,
+    int64_t ystride, float* __restrict__ out_f, __bf16* __restrict__ out_h,
+    __bf16* __restrict__ out_l, int64_t wsstride, int beta)
+{
+	using acc_t = typename Op::acc_t;
+	using LA    = TileLoad<__bf16, BM, BK>;
+	using LB    = TileLoad<__bf16, BN, BK>;
+	constexpr int VPR = LA::VPR;
+	constexpr int WM = BM / 2, WN = BN / 2;
+	constexpr int FI = WM / Op::kM, FJ = WN / Op::kM;
+	constexpr int NST = BK / Op::kK;     // MFMA steps per chunk
+	constexpr int NG  = 64 / Op::kM;     // lane groups, 8 consecutive k each per step
+
+	__shared__ bf8_t Ah[2][BM * VPR];
+	__shared__ bf8_t Al[2][BM * VPR];
+	__shared__ bf8_t Bh[2][BN * VPR];
+	__shared__ bf8_t Bl[2][BN * VPR];
+
+	const Tile tl = tiles[blockIdx.x];
+	if (tl.task < 0)
+		return;
+	const PTask tk  = tasks[tl.task];
+	const int   tid = threadIdx.x;
+	const int   vec = blockIdx.y;
+
+	const int64_t vo[PBASE_COUNT] = { 0, 0, vec * vs.xp, vec * vs.xt, vec * vs.bx };
+	auto pick = [&](const PBasesT<__bf16>& b, int id) -> const __bf16* {
+		const __bf16* p = id == PBASE_APACK ? b.p[PBASE_APACK]
+		                  : id == PBASE_BPACK ? b.p[PBASE_BPACK]
+		                  : id == PBASE_XP    ? b.p[PBASE_XP] + vo[PBASE_XP]
+		                  : id == PBASE_XT    ? b.p[PBASE_XT] + vo[PBASE_XT]
+		                                      : b.p[PBASE_BX] + vo[PBASE_BX];
+		return p;
+	};
+	out_y += vec * ystride;
+	out_f += vec * wsstride;
+	out_h += vec * wsstride;
+	out_l += vec * wsstride;
+
+	const int lane = tid & 63;
+	const int wave = tid >> 6;
+	const int wm   = (wave & 1) * WM;
+	const int wn   = (wave >> 1) * WN;
+	const int g    = lane / Op::kM;
+	const int lr   = lane % Op::kM;
+
+	acc_t acc[FI][FJ];
+#pragma unroll
+	for (int i = 0; i < FI; ++i)
+#pragma unroll
+		for (int j = 0; j < FJ; ++j)
+			acc[i][j] = acc_t {};
+
+	int seg = tk.seg_begin;
+	int k0  = 0;
+	while (seg < tk.seg_end && segs[seg].k <= 0)
+		++seg;
+
+	if (seg < tk.seg_end) {
+		LA    lah, lal;
+		LB    lbh, lbl;
+		bf8_t vah[LA::NV], val[LA::NV], vbh[LB::NV], vbl[LB::NV];
+		PSeg  s     = segs[seg];
+		auto  setup = [&](const PSeg& sg) {
+                        lah.setup(pick(hi, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
+                        lal.setup(pick(lo, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
+                        lbh.setup(pick(hi, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
+                        lbl.setup(pick(lo, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
+		};
+		auto load = [&](int kk0, int K) {
+			lah.load(vah, kk0, K, tid);
+			lal.load(val, kk0, K, tid);
+			lbh.load(vbh, kk0, K, tid);
+			lbl.load(vbl, kk0, K, tid);
+		};
+		auto store = [&](int b) {
+			lah.store(Ah[b], vah, tid);
+			lal.store(Al[b], val, tid);
+			lbh.store(Bh[b], vbh, tid);
+			lbl.store(Bl[b], vbl, tid);
+		};
+		setup(s);
+		load(k0, s.k);
+		store(0);
+		__syncthreads();
+		int buf = 0;
+		for (;;) {
+			int        nseg    = seg;
+			int        nk0     = k0;
+			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, nseg, nk0);
+			if (hasNext) {
+				if (nseg != seg) {
+					s = segs[nseg];
+					setup(s);
+				}
+				load(nk0, s.k);
+			}
+#pragma unroll
+			for (int st = 0; st < NST; ++st) {
+				// Step st covers k in [st * kK, st * kK + kK): lane group g the 8 at 8 g.
+				const int kv = st * NG + g;
+				bf8_t     ah[FI], al[FI], bh[FJ], bl[FJ];
+#pragma unroll
+				for (int i = 0; i < FI; ++i) {
+					const int r = wm + Op::kM * i + lr;
+					ah[i]       = Ah[buf][r * VPR + LA::slot(r, kv)];
+					al[i]       = Al[buf][r * VPR + LA::slot(r, kv)];
+				}
+#pragma unroll
+				for (int j = 0; j < FJ; ++j) {
+					const int r = wn + Op::kM * j + lr;
+					bh[j]       = Bh[buf][r * VPR + LB::slot(r, kv)];
+					bl[j]       = Bl[buf][r * VPR + LB::slot(r, kv)];
+				}
+#pragma unroll
+				for (int i = 0; i < FI; ++i)
+#pragma unroll
+					for (int j = 0; j < FJ; ++j) {
+						acc[i][j] = Op::mma(al[i], bh[j], acc[i][j]);
+						acc[i][j] = Op::mma(ah[i], bl[j], acc[i][j]);
+						acc[i][j] = Op::mma(ah[i], bh[j], acc[i][j]);
+					}
+			}
+			if (!hasNext)
+				break;
+			store(buf ^ 1);
+			__syncthreads();
+			buf ^= 1;
+			seg = nseg;
+			k0  = nk0;
+		}
+	}
+
+	const bool to_y = tk.out_is_y != 0;
+#pragma unroll
+	for (int i = 0; i < FI; ++i)
+#pragma unroll
+		for (int j = 0; j < FJ; ++j)
+#pragma unroll
+			for (int r = 0; r < Op::kRegs; ++r) {
+				const int row = tl.m0 + wm + Op::kM * i + Op::row(lane, r);
+				const int col = tl.n0 + wn + Op::kM * j + lr;
+				if (row < tk.m && col < tk.n) {
+					const int64_t addr = out_addr(tk, row, col);
+					const float   v    = acc[i][j][r];
+					if (to_y) {
+						double y = double(v);
+						if (beta)
+							y += out_y[addr];
+						out_y[addr] = y;
+					} else if (PAIR) {
+						const __bf16 h = __bf16(v);
+						out_h[addr]    = h;
+						out_l[addr]    = __bf16(v - float(h));
+					} else {
+						out_f[addr] = v;
+					}
+				}
+			}
+}
+
+// Input preparation, one 64 x 64 tile of one patch per workgroup: X_J (R x L, column major at
+// xoff) is written column major with padded columns (XP: element (m, c) at pofs + m + c * ldp,
+// zero for R <= m < ldp) and transposed (XT: element (m, c) at tofs + c + m * ldt, zero for
+// L <= c < ldt), converted to the compute type.
+template <typename E> struct PrepOut {
+	E* xp;
+	E* xt;
+};
+
+__device__ inline void put(double* p, int64_t i, double v) { p[i] = v; }
+__device__ inline void put(float* p, int64_t i, double v) { p[i] = float(v); }
+
+template <typename E>
+__global__ __launch_bounds__(256) void prep_x_kernel(const PrepTile* __restrict__ tiles,
+                                                     const PrepPatch* __restrict__ patches,
+                                                     const double* __restrict__ x, int64_t ldx,
+                                                     E* __restrict__ xp, E* __restrict__ xt,
+                                                     PStrides vs)
+{
+	__shared__ double t[64][65];
+	const PrepTile  pt = tiles[blockIdx.x];
+	const PrepPatch pp = patches[pt.patch];
+	const int       v  = blockIdx.y;
+	x += v * ldx + pp.xoff;
+	xp += v * vs.xp + pp.pofs;
+	xt += v * vs.xt + pp.tofs;
+	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+	// Column-major read of the tile (m = m0 + tx fastest), XP written the same way.
+	for (int c = ty; c < 64; c += 4) {
+		const int    m   = pt.m0 + tx;
+		const int    col = pt.c0 + c;
+		const double val = (m < pp.R && col < pp.L) ? x[m + int64_t(col) * pp.R] : 0.0;
+		t[c][tx]         = val;
+		if (m < pp.ldp && col < pp.L)
+			put(xp, m + int64_t(col) * pp.ldp, val);
+	}
+	__syncthreads();
+	// XT: c fastest.
+	for (int r = ty; r < 64; r += 4) {
+		const int m   = pt.m0 + r;
+		const int col = pt.c0 + tx;
+		if (m < pp.R && col < pp.ldt)
+			put(xt, col + int64_t(m) * pp.ldt, t[tx][r]);
+	}
+}
+
+// The same into (hi, lo) bf16 pairs: hi = bf16(v), lo = bf16(v - hi), the residual in double.
+__global__ __launch_bounds__(256) void prep_x_bf16x2_kernel(
+    const PrepTile* __restrict__ tiles, const PrepPatch* __restrict__ patches,
+    const double* __restrict__ x, int64_t ldx, __bf16* __restrict__ xph, __bf16* __restrict__ xpl,
+    __bf16* __restrict__ xth, __bf16* __restrict__ xtl, PStrides vs)
+{
+	__shared__ double t[64][65];
+	const PrepTile  pt = tiles[blockIdx.x];
+	const PrepPatch pp = patches[pt.patch];
+	const int       v  = blockIdx.y;
+	x += v * ldx + pp.xoff;
+	const int64_t po = v * vs.xp + pp.pofs, to = v * vs.xt + pp.tofs;
+	const int     tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+	for (int c = ty; c < 64; c += 4) {
+		const int    m   = pt.m0 + tx;
+		const int    col = pt.c0 + c;
+		const double val = (m < pp.R && col < pp.L) ? x[m + int64_t(col) * pp.R] : 0.0;
+		t[c][tx]         = val;
+		if (m < pp.ldp && col < pp.L) {
+			const __bf16  h = __bf16(float(val));
+			const int64_t i = po + m + int64_t(col) * pp.ldp;
+			xph[i]          = h;
+			xpl[i]          = __bf16(float(val - double(float(h))));
+		}
+	}
+	__syncthreads();
+	for (int r = ty; r < 64; r += 4) {
+		const int m   = pt.m0 + r;
+		const int col = pt.c0 + tx;
+		if (m < pp.R && col < pp.ldt) {
+			const double  val = t[tx][r];
+			const __bf16  h   = __bf16(float(val));
+			const int64_t i   = to + col + int64_t(m) * pp.ldt;
+			xth[i]            = h;
+			xtl[i]            = __bf16(float(val - double(float(h))));
+		}
+	}
+}
+
+// hi = bf16(x), lo = bf16(x - hi), the residual taken in double (elementwise; the packed
+// factors keep their layout).
+__global__ void split_f64_bf16x2_kernel(const double* __restrict__ x, __bf16* __restrict__ hi,
+                                        __bf16* __restrict__ lo, int64_t n)
+{
+	int64_t stride = int64_t(gridDim.x) * blockDim.x;
+	for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const double v = x[i];
+		const __bf16 h = __bf16(float(v));
+		hi[i]          = h;
+		lo[i]          = __bf16(float(v - double(float(h))));
+	}
+}
+
+} // namespace
+
+void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hipStream_t stream)
+{
+	if (n <= 0)
+		return;
+	int64_t blocks = (n + 255) / 256;
+	if (blocks > 4096)
+		blocks = 4096;
+	hipLaunchKernelGGL(split_f64_bf16x2_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, x,
+	                   static_cast<__bf16*>(hi), static_cast<__bf16*>(lo), n);
+}
+
+// Host-side launchers -------------------------------------------------------------------------
+
+template <typename T, typename TO>
+void launch_panel_gemm(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
+                       const Tile* tiles, int ntiles, const T* const* bases, const PStrides& vs,
+                       TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec, int beta,
+                       hipStream_t stream)
+{
+	if (ntiles <= 0 || nvec <= 0)
+		return;
+	PBasesT<T> b;
+	for (int i = 0; i < PBASE_COUNT; ++i)
+		b.p[i] = bases[i];
+	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
+#define DMK_PLAUNCH(BM, BN, BK, OP)                                                                \
+	hipLaunchKernelGGL((panel_gemm_kernel<TO, BM, BN, BK, OP>), grid, dim3(kPThreads), 0, stream, \
+	                   tasks, segs, tiles, b, vs, out_y, ystride, out_ws, wsstride, beta)
+#define DMK_PSHAPES(BK, OP)                                                                        \
+	if (bm == 64 && bn == 64)                                                                  \
+		DMK_PLAUNCH(64, 64, BK, OP);                                                       \
+	else if (bm == 128 && bn == 64)                                                            \
+		DMK_PLAUNCH(128, 64, BK, OP);                                                      \
+	else if (bm == 64 && bn == 128)                                                            \
+		DMK_PLAUNCH(64, 128, BK, OP);                                                      \
+	else if (bm == 128 && bn == 128)                                                           \
+		DMK_PLAUNCH(128, 128, BK, OP);                                                     \
+	else                                                                                       \
+		throw std::runtime_error("panel gemm: unsupported tile shape");
+	if constexpr (std::is_same<T, double>::value) {
+		DMK_PSHAPES(16, OpF64)
+	} else {
+		if (variant == 1) {
+			DMK_PSHAPES(32, OpF32x32)
+		} else {
+			DMK_PSHAPES(32, OpF32)
+		}
+	}
+#undef DMK_PSHAPES
+#undef DMK_PLAUNCH
+}
+
+template void launch_panel_gemm<double, double>(int, int, int, const PTask*, const PSeg*,
+                                                const Tile*, int, const double* const*,
+                                                const PStrides&, double*, int64_t, double*,
+                                                int64_t, int, int, hipStream_t);
+template void launch_panel_gemm<float, float>(int, int, int, const PTask*, const PSeg*,
+                                              const Tile*, int, const float* const*,
+                                              const PStrides&, float*, int64_t, float*, int64_t,
+                                              int, int, hipStream_t);
+template void launch_panel_gemm<float, double>(int, int, int, const PTask*, const PSeg*,
+                                               const Tile*, int, const float* const*,
+                                               const PStrides&, double*, int64_t, float*, int64_t,
+                                               int, int, hipStream_t);
+
+// variant 1 selects the 32x32x16 bf16 MFMA form.
+void launch_panel_bf16x3(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
+                         const Tile* tiles, int ntiles, const void* const* bases_hi,
+                         const void* const* bases_lo, const PStrides& vs, double* out_y,
+                         int64_t ystride, float* out_f, void* out_h, void* out_l, int64_t wsstride,
+                         bool pair, int nvec, int beta, hipStream_t stream)
+{
+	if (ntiles <= 0 || nvec <= 0)
+		return;
+	PBasesT<__bf16> h, l;
+	for (int i = 0; i < PBASE_COUNT; ++i) {
+		h.p[i] = static_cast<const __bf16*>(bases_hi[i]);
+		l.p[i] = static_cast<const __bf16*>(bases_lo[i]);
+	}
+	__bf16*    oh = static_cast<__bf16*>(out_h);
+	__bf16*    ol = static_cast<__bf16*>(out_l);
+	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
+#define DMK_BLAUNCH(BM, BN, PAIR, OP)                                                              \
+	hipLaunchKernelGGL((panel_bf16x3_kernel<BM, BN, 32, PAIR, OP>), grid, dim3(kPThreads), 0,  \
+	                   stream, tasks, segs, tiles, h, l, vs, out_y, ystride, out_f, oh, ol,      \
+	                   wsstride, beta)
+#define DMK_BSHAPES(PAIR, OP)                                                                      \
+	if (bm == 64 && bn == 64)                                                                  \
+		DMK_BLAUNCH(64, 64, PAIR, OP);                                                     \
+	else if (bm == 128 && bn == 64)                                                            \
+		DMK_BLAUNCH(128, 64, PAIR, OP);                                                    \
+	else if (bm == 64 && bn == 128)                                                            \
+		DMK_BLAUNCH(64, 128, PAIR, OP);                                                    \
+	else if (bm == 128 && bn == 128)                                                           \
+		DMK_BLAUNCH(128, 128, PAIR, OP);                                                   \
+	else                                                                                       \
+		throw std::runtime_error("panel gemm bf16x3: unsupported tile shape");
+	if (pair) {
+		if (variant == 1) {
+			DMK_BSHAPES(true, OpBf16x32)
+		} else {
+			DMK_BSHAPES(true, OpBf16)
+		}
+	} else {
+		if (variant == 1) {
+			DMK_BSHAPES(false, OpBf16x32)
+		} else {
+			DMK_BSHAPES(false, OpBf16)
+		}
+	}
+#undef DMK_BSHAPES
+#undef DMK_BLAUNCH
+}
+
+template <typename E>
+void launch_prep_x(const PrepTile* tiles, int ntiles, const PrepPatch* patches, const double* x,
+                   int64_t ldx, E* xp, E* xt, const PStrides& vs, int nvec, hipStream_t stream)
+{
+	if (ntiles <= 0 || nvec <= 0)
+		return;
+	hipLaunchKernelGGL((prep_x_kernel<E>), dim3(unsigned(ntiles), unsigned(nvec)), dim3(256), 0,
+	                   stream, tiles, patches, x, ldx, xp, xt, vs);
+}
+template void launch_prep_x<double>(const PrepTile*, int, const PrepPatch*, const double*, int64_t,
+                                    double*, double*, const PStrides&, int, hipStream_t);
+template void launch_prep_x<float>(const PrepTile*, int, const PrepPatch*, const double*, int64_t,
+                                   float*, float*, const PStrides&, int, hipStream_t);
+
+void launch_prep_x_bf16x2(const PrepTile* tiles, int ntiles, const PrepPatch* patches,
+                          const double* x, int64_t ldx, void* xph, void* xpl, void* xth, void* xtl,
+                          const PStrides& vs, int nvec, hipStream_t stream)
+{
+	if (ntiles <= 0 || nvec <= 0)
+		return;
+	hipLaunchKernelGGL(prep_x_bf16x2_kernel, dim3(unsigned(ntiles), unsigned(nvec)), dim3(256), 0,
+	                   stream, tiles, patches, x, ldx, static_cast<__bf16*>(xph),
+	                   static_cast<__bf16*>(xpl), static_cast<__bf16*>(xth),
+	                   static_cast<__bf16*>(xtl), vs);
+}
+
+} // namespace dmk

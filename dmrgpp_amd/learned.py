@@ -5,9 +5,10 @@ correction at the same FP64 residual.
   python -m dmrgpp_amd.learned --config hubbard-ladder-8x2 [--m M] [--train-sweeps 1]
 
 Phase 1 runs the first sweeps with Davidson (diagonal correction) and harvests training pairs
-from every converged step; the MLP is then trained on the GPU. Validation: the last training
-superblock is solved again with the diagonal and with the learned correction; the learned one
-is kept only if it needs fewer matvecs there (otherwise the diagonal method is the fallback).
+from every converged step; the MLP is then trained on the GPU. Validation: the last three
+training superblocks are solved again with the diagonal and with the learned correction; the
+learned one is kept only if it needs at least 5% fewer matvecs over them (otherwise the
+diagonal method is the fallback).
 Phase 2 runs the remaining sweeps from the same block stores with plain Lanczos, Davidson +
 diagonal, Davidson + MLP and the validated choice, each step started from the same transformed
 wave function, and counts matvecs and time.
@@ -26,16 +27,20 @@ from . import models, sweep
 
 class DavidsonSolver:
     """sweep.run solver: Davidson with the given correction; harvests into mlp when asked and
-    keeps the last problem it solved (for validation)."""
+    keeps the last `keep` problems it solved (for validation)."""
 
     def __init__(self, precond="diag", mlp=None, harvest=False, tol=1e-9, max_basis=24,
-                 max_matvecs=600, seed=1234):
+                 max_matvecs=600, seed=1234, keep=3):
         self.precond, self.mlp, self.harvest = precond, mlp, harvest
         self.tol, self.max_basis, self.max_matvecs, self.seed = tol, max_basis, max_matvecs, seed
-        self.last = None
+        self.keep, self.recent = keep, []
+
+    @property
+    def last(self):
+        return self.recent[-1] if self.recent else None
 
     def __call__(self, prob, init=None):
-        self.last = prob
+        self.recent = (self.recent + [prob])[-self.keep:]
         from . import BatchedGemm, DeviceArray
 
         own = not isinstance(init, DeviceArray)
@@ -123,13 +128,15 @@ def compare(name: str, m=None, train_sweeps=1, test_sweeps=1, tol=1e-9, train_it
     t2 = time.perf_counter()
     log(f"harvested {nsamp} pairs in {t1 - t0:.1f} s; trained {train_iters} Adam steps in "
         f"{t2 - t1:.2f} s, loss {loss0:.4g} -> {loss_all:.4g}")
-    # Validation on the last training superblock: keep the learned correction only if it wins.
-    val = {}
-    for pc in ("diag", "mlp"):
-        e, _, r = DavidsonSolver(pc, mlp=mlp, tol=tol)(harvest.last)
-        r["z_dev"].free()
-        val[pc] = r["matvecs"]
-    chosen = "mlp" if val["mlp"] < val["diag"] else "diag"
+    # Validation on the last training superblocks: keep the learned correction only if it wins
+    # clearly.
+    val = {"diag": 0, "mlp": 0}
+    for prob in harvest.recent:
+        for pc in ("diag", "mlp"):
+            e, _, r = DavidsonSolver(pc, mlp=mlp, tol=tol)(prob)
+            r["z_dev"].free()
+            val[pc] += r["matvecs"]
+    chosen = "mlp" if val["mlp"] < 0.95 * val["diag"] else "diag"
     log(f"validation matvecs: diag {val['diag']}, mlp {val['mlp']} -> {chosen}")
     out = {"config": name, "m": m, "samples": nsamp, "train_s": t2 - t1, "loss_untrained": loss0,
            "loss": loss_all, "validation_matvecs": val, "chosen": chosen}

@@ -7,7 +7,7 @@
         X[i] = (i + 1) / n;
   (iii) a replay of a saved superblock (bench.py --save-problem), e.g. the m=2048 ladder.
 
-  python -m dmrgpp_amd.microbench [--replay F.npz] [--iters 30]
+  python -m dmrgpp_amd.microbench [--replay F.npz [--load-tuning T.json]] [--iters 30]
 
 Each case reports the average matvec time from HIP events (one warm-up, then iters calls) and
 the algorithmic TFLOP/s in FP64, FP32 and BF16x3.
@@ -23,7 +23,7 @@ import numpy as np
 from . import problems
 
 
-def run_case(name, prob, x, iters, log):
+def run_case(name, prob, x, iters, log, tunings=None):
     from . import BatchedGemm, DeviceArray
 
     op = BatchedGemm(prob, "fp64")
@@ -33,6 +33,8 @@ def run_case(name, prob, x, iters, log):
            "flops": info["flops_algorithmic"]}
     for prec in ("fp64", "fp32", "bf16x3"):
         op.set_precision(prec)
+        if tunings and prec in tunings:
+            op.set_tuning(**tunings[prec])
         op.profile(dx, dy, 1)
         _, _, ms = op.profile(dx, dy, iters)
         out[prec] = {"ms": round(ms, 4), "tflops": round(info["flops_algorithmic"] / (ms * 1e-3)
@@ -46,7 +48,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--replay", default=None, help="superblock .npz written by bench.py")
     ap.add_argument("--iters", type=int, default=30)
+    ap.add_argument("--load-tuning", default=None,
+                    help="per-precision tunings (JSON written by bench.py --save-tuning) for the "
+                         "replayed superblock")
     a = ap.parse_args(argv)
+    tunings = json.load(open(a.load_tuning)) if a.load_tuning else None
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     res = []
     p1 = problems.block_diagonal_problem()
@@ -59,7 +65,8 @@ def main(argv=None):
 
         p3 = load_npz(a.replay, name="superblock")
         res.append(run_case("replay:" + p3.name, p3,
-                            np.random.default_rng(1234).uniform(-1, 1, p3.n), a.iters, log))
+                            np.random.default_rng(1234).uniform(-1, 1, p3.n), a.iters, log,
+                            tunings))
     print(json.dumps(res), flush=True)
 
 
