@@ -33,13 +33,24 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 157.3}  # bf16x3: FP32-equ
 M = 2048
 LX = 32
 # HBM bytes per launch of the dominant kernel (stage 2) from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on this workload, FETCH scaled by the copy-kernel calibration factor 2.0 (the
-# profiles directory holds the summaries); None where not measured.
-TRAFFIC = {}
+# WRITE_SIZE on this workload (profiles/panel_gemm_ladder_m2048.md), FETCH scaled by the
+# copy-kernel calibration factor 2.0, WRITE by 1.0. Keyed by precision and the stage-2 launch
+# shape (tile m, tile n, splitk, xcd order, MFMA form) the counters were taken with; a run whose
+# autotuner picks another shape reports None.
+TRAFFIC = {
+    ("fp64", (64, 64, 2048, 0, 0)): 3.2517e9 + 0.0672e9,
+    ("fp32", (64, 64, 2048, 0, 1)): 1.6933e9 + 0.0343e9,
+    ("bf16x3", (128, 64, 1024, 1, 0)): 0.5742e9 + 0.0692e9,
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def shape_key(t):
+    """The stage-2 launch shape of a tuning (the key of TRAFFIC)."""
+    return (t["stage2_tile_m"], t["stage2_tile_n"], t["splitk"], t["xcd_order"], t["mfma32"])
 
 
 def cpu_baseline(prob, x, budget_s=15.0, max_calls=20):
@@ -174,7 +185,7 @@ def main():
             op.set_tuning(**loaded[prec])
         elif not args.no_autotune:
             t_at = time.perf_counter()
-            op.autotune(dx, dy, 3)
+            op.autotune(dx, dy, 5)
             log(f"{prec}: autotuned in {time.perf_counter() - t_at:.2f} s -> {op.tuning()}")
         tunings[prec] = op.tuning()
         for _ in range(args.warmup):
@@ -252,7 +263,8 @@ def main():
                 "bf16x3": "panel_bf16x3_kernel"}[prec]
         return {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[prec], 4),
-                "traffic": TRAFFIC.get(prec), "kernel": f"{kern} ({kname})",
+                "traffic": TRAFFIC.get((prec, shape_key(tunings[prec]))),
+                "kernel": f"{kern} ({kname})",
                 "ms_per_launch": round(kms, 4), "flops_per_launch": kflops}
 
     r = results["fp32"]

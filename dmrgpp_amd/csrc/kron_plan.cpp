@@ -1021,22 +1021,25 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		throw std::invalid_argument("dmk_autotune: iters must be >= 1");
 	const Tuning start = tuning_;
 	std::vector<Tuning> cands;
-	const int nvar = (precision_ == PREC_FP32) ? 2 : 1; // FP32: 16x16x4 and 32x32x2 MFMA
+	// MFMA form: FP32 16x16x4 or 32x32x2, BF16x3 16x16x32 or 32x32x16.
+	const int nvar = (precision_ == PREC_FP64) ? 1 : 2;
 	for (int var = 0; var < nvar; ++var)
-		for (int bm1 : { 64, 128 })
-			for (int bn1 : { 64, 128 })
-				for (int bm2 : { 64, 128 })
-					for (int bn2 : { 64, 128 })
-						for (int sk : { 0, 1024, 2048 }) {
-							Tuning t;
-							t.bm1    = bm1;
-							t.bn1    = bn1;
-							t.bm2    = bm2;
-							t.bn2    = bn2;
-							t.splitk = sk;
-							t.mfma32 = var;
-							cands.push_back(t);
-						}
+		for (int xcd : { 0, 1 })
+			for (int bm1 : { 64, 128 })
+				for (int bn1 : { 64, 128 })
+					for (int bm2 : { 64, 128 })
+						for (int bn2 : { 64, 128 })
+							for (int sk : { 0, 1024, 2048 }) {
+								Tuning t;
+								t.bm1    = bm1;
+								t.bn1    = bn1;
+								t.bm2    = bm2;
+								t.bn2    = bn2;
+								t.splitk = sk;
+								t.xcd    = xcd;
+								t.mfma32 = var;
+								cands.push_back(t);
+							}
 	Tuning best     = start;
 	double best_ms  = 1e300;
 	for (const Tuning& t : cands) {

@@ -52,7 +52,7 @@ struct Tuning {
 	int bn2    = 64;
 	int splitk = 1024;
 	int xcd    = 0; // 1: XCD-aware tile order
-	int mfma32 = 0; // 1: FP32 mode on 32x32x2 MFMA blocks
+	int mfma32 = 0; // 1: 32x32 MFMA blocks (FP32 32x32x2, BF16x3 32x32x16); FP64 ignores it
 
 	bool operator==(const Tuning& o) const
 	{

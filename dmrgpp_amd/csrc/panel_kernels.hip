@@ -143,15 +143,22 @@ __device__ inline int64_t out_addr(const PTask& tk, int r, int c)
 	return tk.coff + int64_t(rr) * tk.sm + int64_t(blk) * tk.n + c;
 }
 
-// Advances (seg, k0) to the next K chunk of the task; false past the last one.
+// Advances (seg, k0) to the next K chunk of the task, with K the current segment's length (held
+// in registers, so the common step reads no descriptor); false past the last chunk. newseg is
+// set when the chunk starts a new segment.
 template <int BK>
-__device__ inline bool next_chunk(const PSeg* __restrict__ segs, int seg_end, int& seg, int& k0)
+__device__ inline bool next_chunk(const PSeg* __restrict__ segs, int seg_end, int K, int& seg,
+                                  int& k0, bool& newseg)
 {
 	k0 += BK;
-	while (seg < seg_end && k0 >= segs[seg].k) {
+	newseg = false;
+	if (k0 < K)
+		return true;
+	k0     = 0;
+	newseg = true;
+	do
 		++seg;
-		k0 = 0;
-	}
+	while (seg < seg_end && segs[seg].k <= 0);
 	return seg < seg_end;
 }
 
@@ -189,9 +196,23 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 				v[q] = ok ? *reinterpret_cast<const vec_t*>(p[q] + k0) : vec_t {};
 		}
 	}
-	// Slot of vector kv of LDS row r: rows r and r + 16 / VPR start on the same bank, so they
-	// take different XOR masks.
-	__device__ static inline int slot(int r, int kv) { return kv ^ ((r * VPR / 16) & (VPR - 1)); }
+	// Slot of vector kv of LDS row r. A 16-byte LDS access is served 8 lanes (32 banks) or 16
+	// lanes (64 banks) per pass; the fragment reads put consecutive rows on consecutive lanes,
+	// so the XOR mask f(r) keeps any 8 consecutive rows, and any 16 consecutive rows that start
+	// on the same bank, on distinct slots: f is a permutation of 0..VPR-1 on rows 8a..8a+7, on
+	// the even and on the odd rows of 16a..16a+15 (VPR = 8), and on each residue class mod 2
+	// within 8 rows and mod 4 within 16 rows (VPR = 4).
+	__device__ static inline int slot(int r, int kv)
+	{
+		int f;
+		if constexpr (VPR == 8)
+			f = (r & 7) ^ ((r >> 3) & 1);
+		else if constexpr (VPR == 4)
+			f = ((r >> 1) & 3) ^ ((r >> 3) & 1);
+		else
+			f = r & (VPR - 1);
+		return kv ^ f;
+	}
 	__device__ inline void store(vec_t* __restrict__ s, const vec_t (&v)[NV], int tid) const
 	{
 		const int r  = tid / VPR;
@@ -279,11 +300,12 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 		__syncthreads();
 		int buf = 0;
 		for (;;) {
-			int        nseg    = seg;
-			int        nk0     = k0;
-			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, nseg, nk0);
+			int        nseg = seg;
+			int        nk0  = k0;
+			bool       newseg;
+			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, s.k, nseg, nk0, newseg);
 			if (hasNext) {
-				if (nseg != seg) {
+				if (newseg) {
 					s = segs[nseg];
 					la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
 					lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
@@ -450,11 +472,12 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 		__syncthreads();
 		int buf = 0;
 		for (;;) {
-			int        nseg    = seg;
-			int        nk0     = k0;
-			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, nseg, nk0);
+			int        nseg = seg;
+			int        nk0  = k0;
+			bool       newseg;
+			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, s.k, nseg, nk0, newseg);
 			if (hasNext) {
-				if (nseg != seg) {
+				if (newseg) {
 					s = segs[nseg];
 					setup(s);
 				}

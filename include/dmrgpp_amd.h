@@ -82,7 +82,7 @@ typedef struct dmk_tuning {
 	int32_t stage2_tile_m, stage2_tile_n;
 	int32_t splitk;
 	int32_t xcd_order;
-	int32_t mfma32;            /* FP32 mode: 1 = 32x32x2 MFMA blocks, 0 = 16x16x4 */
+	int32_t mfma32;            /* 1 = 32x32 MFMA blocks (FP32: 32x32x2, BF16x3: 32x32x16), 0 = 16x16 */
 	int32_t pad;
 } dmk_tuning;
 
