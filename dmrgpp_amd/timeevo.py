@@ -2,13 +2,14 @@
 together by the Krylov time step, all targets in one grouped Kron launch per Krylov step.
 
   python -m dmrgpp_amd.timeevo --config hubbard-ladder-24x2 [--m 2048] [--targets 4]
-                               [--dt 0.05] [--steps 40] [--precision fp64|fp32]
+                               [--dt 0.05] [--steps 40] [--precision fp64[,fp32,bf16x3]]
 
 The superblock is the middle of the lattice after infinite-size growth (sweep driver, GPU
 Lanczos). The targets are the ground state and seeded random vectors of the same sector
 (complex), normalized. Each time step calls dmk_krylov_evolve_z once for all targets; the run
 reports the time per step, Krylov dimensions, and the drift of each target's norm and energy
-(both conserved by exact evolution).
+(both conserved by exact evolution). Several precisions (comma separated) evolve the same
+superblock and start vectors one after the other.
 """
 from __future__ import annotations
 
@@ -115,8 +116,12 @@ def main(argv=None):
     prob, z, e = superblock(a.config, a.m, log=log)
     log(f"superblock n={prob.n} patches={prob.npatches} terms={len(prob.blocks)} E0={e:.12f} "
         f"({time.perf_counter() - t0:.1f} s)")
-    out = evolve(prob, z, a.targets, a.dt, a.steps, a.precision, log=log)
-    out["config"] = a.config
+    out = {"config": a.config, "E0": e, "npatches": prob.npatches, "nterms": len(prob.blocks),
+           "runs": {}}
+    for prec in a.precision.split(","):
+        log(f"-- {prec}")
+        out["runs"][prec] = evolve(prob, z, a.targets, a.dt, a.steps, prec, log=log)
+        log(json.dumps(out["runs"][prec]))
     print(json.dumps(out), flush=True)
     if a.json:
         with open(a.json, "w") as fh:
