@@ -35,8 +35,9 @@ LX = 32
 # HBM bytes per launch of the dominant kernel (stage 2) from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload (profiles/panel_gemm_ladder_m2048.md), FETCH scaled by the
 # copy-kernel calibration factor 2.0, WRITE by 1.0. Keyed by precision and the stage-2 launch
-# shape (tile m, tile n, splitk, xcd order, MFMA form) the counters were taken with; a run whose
-# autotuner picks another shape reports None.
+# shape (tile m, tile n, splitk, xcd order, MFMA form) the counters were taken with (load
+# pipeline depth 1; the depth moves the loads in time, not the bytes); a run whose autotuner
+# picks another shape reports None.
 TRAFFIC = {
     ("fp64", (64, 64, 2048, 0, 0)): 3.2517e9 + 0.0672e9,
     ("fp32", (64, 64, 2048, 0, 1)): 1.6933e9 + 0.0343e9,

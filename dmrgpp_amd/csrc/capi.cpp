@@ -312,7 +312,7 @@ void to_c(const dmk::Tuning& t, dmk_tuning* c)
 	c->splitk        = t.splitk;
 	c->xcd_order     = t.xcd;
 	c->mfma32        = t.mfma32;
-	c->pad           = 0;
+	c->prefetch      = t.depth;
 }
 } // namespace
 
@@ -339,6 +339,7 @@ int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning)
 		t.splitk = tuning->splitk;
 		t.xcd    = tuning->xcd_order;
 		t.mfma32 = tuning->mfma32;
+		t.depth  = tuning->prefetch == 0 ? 1 : tuning->prefetch;
 		plan_of(handle).retune(t);
 		return DMK_OK;
 	});

@@ -32,10 +32,10 @@
 namespace dmk {
 
 template <typename T, typename TO>
-void launch_panel_gemm(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
-                       const Tile* tiles, int ntiles, const T* const* bases, const PStrides& vs,
-                       TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec, int beta,
-                       hipStream_t stream);
+void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* tasks,
+                       const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
+                       const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
+                       int64_t wsstride, int nvec, int beta, hipStream_t stream);
 void launch_panel_bf16x3(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
                          const Tile* tiles, int ntiles, const void* const* bases_hi,
                          const void* const* bases_lo, const PStrides& vs, double* out_y,
@@ -695,6 +695,8 @@ void Plan::retune(const Tuning& t)
 		throw std::invalid_argument("dmk tuning: tile shapes must be 64 or 128, splitk >= 0");
 	if (t.mfma32 != 0 && t.mfma32 != 1)
 		throw std::invalid_argument("dmk tuning: mfma32 must be 0 or 1");
+	if (t.depth != 1 && t.depth != 2)
+		throw std::invalid_argument("dmk tuning: prefetch must be 1 or 2");
 	if (stream_)
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
 	tuning_                 = t;
@@ -910,6 +912,7 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 	const int      npt = int(prep_tiles_.size());
 	const PStrides vs { xp_elems_, xt_elems_, bxs };
 	const int      v1  = tuning_.mfma32;
+	const int      dp  = tuning_.depth;
 	if (precision_ == PREC_FP64) {
 		double* bx = static_cast<double*>(d_bx_);
 		double* ws = static_cast<double*>(d_ws_);
@@ -918,12 +921,12 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		launch_prep_x<double>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
 		                      stream_);
 		const double* bases[PBASE_COUNT] = { d_apack_, d_bpack_, xp, xt, bx };
-		launch_panel_gemm<double, double>(st1_.bm, st1_.bn, 0, st1_.d_tasks, st1_.d_segs,
+		launch_panel_gemm<double, double>(st1_.bm, st1_.bn, 0, dp, st1_.d_tasks, st1_.d_segs,
 		                                  st1_.d_tiles, int(st1_.tiles.size()), bases, vs, vout,
 		                                  ldout, bx, bxs, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
-		launch_panel_gemm<double, double>(st2_.bm, st2_.bn, 0, st2_.d_tasks, st2_.d_segs,
+		launch_panel_gemm<double, double>(st2_.bm, st2_.bn, 0, dp, st2_.d_tasks, st2_.d_segs,
 		                                  st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
 		                                  ldout, ws, wss, nvec, beta, stream_);
 		launch_splitk_reduce<double, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
@@ -961,12 +964,12 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		launch_prep_x<float>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
 		                     stream_);
 		const float* bases[PBASE_COUNT] = { d_apack32_, d_bpack32_, xp, xt, bx };
-		launch_panel_gemm<float, float>(st1_.bm, st1_.bn, v1, st1_.d_tasks, st1_.d_segs,
+		launch_panel_gemm<float, float>(st1_.bm, st1_.bn, v1, dp, st1_.d_tasks, st1_.d_segs,
 		                                st1_.d_tiles, int(st1_.tiles.size()), bases, vs, bx, bxs,
 		                                bx, bxs, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
-		launch_panel_gemm<float, double>(st2_.bm, st2_.bn, v1, st2_.d_tasks, st2_.d_segs,
+		launch_panel_gemm<float, double>(st2_.bm, st2_.bn, v1, dp, st2_.d_tasks, st2_.d_segs,
 		                                 st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
 		                                 ldout, ws, wss, nvec, beta, stream_);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
@@ -1023,7 +1026,9 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 	std::vector<Tuning> cands;
 	// MFMA form: FP32 16x16x4 or 32x32x2, BF16x3 16x16x32 or 32x32x16.
 	const int nvar = (precision_ == PREC_FP64) ? 1 : 2;
+	const int ndepth = (precision_ == PREC_BF16X3) ? 1 : 2; // the BF16x3 kernel prefetches 1
 	for (int var = 0; var < nvar; ++var)
+	for (int depth = 1; depth <= ndepth; ++depth)
 		for (int xcd : { 0, 1 })
 			for (int bm1 : { 64, 128 })
 				for (int bn1 : { 64, 128 })
@@ -1038,6 +1043,7 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 								t.splitk = sk;
 								t.xcd    = xcd;
 								t.mfma32 = var;
+								t.depth  = depth;
 								cands.push_back(t);
 							}
 	Tuning best     = start;

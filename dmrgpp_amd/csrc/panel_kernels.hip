@@ -228,7 +228,7 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 // FP64 / FP32: T operands, TO output for tasks writing the caller's vector (the FP32 mode writes
 // its stage-2 output in FP64 for the FP64 Lanczos recurrence). Workspace tasks (stage-1 panels,
 // split-k partials) store T and always overwrite.
-template <typename TO, int BM, int BN, int BK, typename Op>
+template <typename TO, int BM, int BN, int BK, int PF, typename Op>
 __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
     PBasesT<typename Op::T> bases, PStrides vs, TO* __restrict__ out_y, int64_t ystride,
@@ -282,37 +282,30 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 			acc[i][j] = acc_t {};
 
 	int seg = tk.seg_begin;
-	int k0  = 0;
 	while (seg < tk.seg_end && segs[seg].k <= 0)
 		++seg;
 
 	if (seg < tk.seg_end) {
-		LA    la;
-		LB    lb;
-		vec_t va[LA::NV], vb[LB::NV];
-		PSeg  s = segs[seg];
+		// Load cursor: the next chunk whose global loads are issued (segment seg, offset ck0).
+		LA   la;
+		LB   lb;
+		PSeg s      = segs[seg];
+		int  ck0    = 0;
+		bool cvalid = true;
 		la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
 		lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
-		la.load(va, k0, s.k, tid);
-		lb.load(vb, k0, s.k, tid);
-		la.store(As[0], va, tid);
-		lb.store(Bs[0], vb, tid);
-		__syncthreads();
-		int buf = 0;
-		for (;;) {
-			int        nseg = seg;
-			int        nk0  = k0;
-			bool       newseg;
-			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, s.k, nseg, nk0, newseg);
-			if (hasNext) {
-				if (newseg) {
-					s = segs[nseg];
-					la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
-					lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
-				}
-				la.load(va, nk0, s.k, tid);
-				lb.load(vb, nk0, s.k, tid);
+		auto issue = [&](vec_t (&xa)[LA::NV], vec_t (&xb)[LB::NV]) {
+			la.load(xa, ck0, s.k, tid);
+			lb.load(xb, ck0, s.k, tid);
+			bool newseg;
+			cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
+			if (cvalid && newseg) {
+				s = segs[seg];
+				la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
+				lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
 			}
+		};
+		auto compute = [&](int b) {
 			// Fragments: NF consecutive k of one LDS row per operand block.
 			T fa[FI][NF], fb[FJ][NF];
 #pragma unroll
@@ -320,7 +313,7 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 				const int r = wm + Op::kM * i + lr;
 #pragma unroll
 				for (int v = 0; v < NF / VW; ++v) {
-					const vec_t x = As[buf][r * VPR + LA::slot(r, g * (NF / VW) + v)];
+					const vec_t x = As[b][r * VPR + LA::slot(r, g * (NF / VW) + v)];
 #pragma unroll
 					for (int e = 0; e < VW; ++e)
 						fa[i][v * VW + e] = x[e];
@@ -331,7 +324,7 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 				const int r = wn + Op::kM * j + lr;
 #pragma unroll
 				for (int v = 0; v < NF / VW; ++v) {
-					const vec_t x = Bs[buf][r * VPR + LB::slot(r, g * (NF / VW) + v)];
+					const vec_t x = Bs[b][r * VPR + LB::slot(r, g * (NF / VW) + v)];
 #pragma unroll
 					for (int e = 0; e < VW; ++e)
 						fb[j][v * VW + e] = x[e];
@@ -344,14 +337,62 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 #pragma unroll
 					for (int j = 0; j < FJ; ++j)
 						acc[i][j] = Op::mma(fa[i][st], fb[j][st], acc[i][j]);
-			if (!hasNext)
-				break;
-			la.store(As[buf ^ 1], va, tid);
-			lb.store(Bs[buf ^ 1], vb, tid);
+		};
+		vec_t a0[LA::NV], b0[LB::NV];
+		int   buf = 0;
+		if constexpr (PF == 1) {
+			// One chunk ahead: the loads of chunk c+1 fly during the MFMAs of chunk c.
+			issue(a0, b0);
+			la.store(As[0], a0, tid);
+			lb.store(Bs[0], b0, tid);
 			__syncthreads();
-			buf ^= 1;
-			seg = nseg;
-			k0  = nk0;
+			for (;;) {
+				const bool have1 = cvalid;
+				if (have1)
+					issue(a0, b0);
+				compute(buf);
+				if (!have1)
+					break;
+				la.store(As[buf ^ 1], a0, tid);
+				lb.store(Bs[buf ^ 1], b0, tid);
+				__syncthreads();
+				buf ^= 1;
+			}
+		} else {
+			// Two chunks ahead, two register sets: at the top of each half, LDS[buf] holds chunk
+			// c and one set holds chunk c+1 (when it exists); the loads of chunk c+2 go to the
+			// other set and are stored one chunk later.
+			vec_t a1[LA::NV], b1[LB::NV];
+			issue(a0, b0);
+			bool have1 = cvalid;
+			if (have1)
+				issue(a1, b1);
+			la.store(As[0], a0, tid);
+			lb.store(Bs[0], b0, tid);
+			__syncthreads();
+			for (;;) {
+				const bool have2 = cvalid;
+				if (have2)
+					issue(a0, b0);
+				compute(buf);
+				if (!have1)
+					break;
+				la.store(As[buf ^ 1], a1, tid);
+				lb.store(Bs[buf ^ 1], b1, tid);
+				__syncthreads();
+				buf ^= 1;
+				const bool have3 = cvalid;
+				if (have3)
+					issue(a1, b1);
+				compute(buf);
+				if (!have2)
+					break;
+				la.store(As[buf ^ 1], a0, tid);
+				lb.store(Bs[buf ^ 1], b0, tid);
+				__syncthreads();
+				buf ^= 1;
+				have1 = have3;
+			}
 		}
 	}
 
@@ -662,10 +703,10 @@ void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hip
 // Host-side launchers -------------------------------------------------------------------------
 
 template <typename T, typename TO>
-void launch_panel_gemm(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
-                       const Tile* tiles, int ntiles, const T* const* bases, const PStrides& vs,
-                       TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec, int beta,
-                       hipStream_t stream)
+void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* tasks,
+                       const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
+                       const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
+                       int64_t wsstride, int nvec, int beta, hipStream_t stream)
 {
 	if (ntiles <= 0 || nvec <= 0)
 		return;
@@ -674,8 +715,16 @@ void launch_panel_gemm(int bm, int bn, int variant, const PTask* tasks, const PS
 		b.p[i] = bases[i];
 	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
 #define DMK_PLAUNCH(BM, BN, BK, OP)                                                                \
-	hipLaunchKernelGGL((panel_gemm_kernel<TO, BM, BN, BK, OP>), grid, dim3(kPThreads), 0, stream, \
-	                   tasks, segs, tiles, b, vs, out_y, ystride, out_ws, wsstride, beta)
+	do {                                                                                       \
+		if (depth == 2)                                                                    \
+			hipLaunchKernelGGL((panel_gemm_kernel<TO, BM, BN, BK, 2, OP>), grid,         \
+			                   dim3(kPThreads), 0, stream, tasks, segs, tiles, b, vs,    \
+			                   out_y, ystride, out_ws, wsstride, beta);                  \
+		else                                                                               \
+			hipLaunchKernelGGL((panel_gemm_kernel<TO, BM, BN, BK, 1, OP>), grid,         \
+			                   dim3(kPThreads), 0, stream, tasks, segs, tiles, b, vs,    \
+			                   out_y, ystride, out_ws, wsstride, beta);                  \
+	} while (0)
 #define DMK_PSHAPES(BK, OP)                                                                        \
 	if (bm == 64 && bn == 64)                                                                  \
 		DMK_PLAUNCH(64, 64, BK, OP);                                                       \
@@ -700,15 +749,15 @@ void launch_panel_gemm(int bm, int bn, int variant, const PTask* tasks, const PS
 #undef DMK_PLAUNCH
 }
 
-template void launch_panel_gemm<double, double>(int, int, int, const PTask*, const PSeg*,
+template void launch_panel_gemm<double, double>(int, int, int, int, const PTask*, const PSeg*,
                                                 const Tile*, int, const double* const*,
                                                 const PStrides&, double*, int64_t, double*,
                                                 int64_t, int, int, hipStream_t);
-template void launch_panel_gemm<float, float>(int, int, int, const PTask*, const PSeg*,
+template void launch_panel_gemm<float, float>(int, int, int, int, const PTask*, const PSeg*,
                                               const Tile*, int, const float* const*,
                                               const PStrides&, float*, int64_t, float*, int64_t,
                                               int, int, hipStream_t);
-template void launch_panel_gemm<float, double>(int, int, int, const PTask*, const PSeg*,
+template void launch_panel_gemm<float, double>(int, int, int, int, const PTask*, const PSeg*,
                                                const Tile*, int, const float* const*,
                                                const PStrides&, double*, int64_t, float*, int64_t,
                                                int, int, hipStream_t);

@@ -76,14 +76,15 @@ typedef struct dmk_info {
 /* Launch shape of a handle: output tile (64 or 128 each way) of the stage-1 and stage-2
  * grouped-GEMM launches, the split-k granularity of stage 2 (target contraction length per
  * piece, 0 = never split; pieces are summed in a fixed order, so results stay deterministic),
- * and the workgroup order (0 = by decreasing task cost, 1 = XCD-aware regions). */
+ * the workgroup order (0 = by decreasing task cost, 1 = XCD-aware regions), the MFMA block form
+ * and the load pipeline depth (global loads issued 1 or 2 K chunks ahead; 0 means 1). */
 typedef struct dmk_tuning {
 	int32_t stage1_tile_m, stage1_tile_n;
 	int32_t stage2_tile_m, stage2_tile_n;
 	int32_t splitk;
 	int32_t xcd_order;
 	int32_t mfma32;            /* 1 = 32x32 MFMA blocks (FP32: 32x32x2, BF16x3: 32x32x16), 0 = 16x16 */
-	int32_t pad;
+	int32_t prefetch;          /* 1 or 2 K chunks ahead (0 = 1) */
 } dmk_tuning;
 
 /* Replaces BatchedGemm<double>::BatchedGemm(noperator, npatches, left_patch_size,

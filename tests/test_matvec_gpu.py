@@ -211,15 +211,15 @@ def test_gpu_multi_vector_and_accumulate():
 
 TUNINGS = [
     dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=0,
-         xcd_order=0, mfma32=0),
+         xcd_order=0, mfma32=0, prefetch=1),
     dict(stage1_tile_m=128, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=128, splitk=16,
-         xcd_order=0, mfma32=1),
+         xcd_order=0, mfma32=1, prefetch=2),
     dict(stage1_tile_m=64, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=128, splitk=32,
-         xcd_order=1, mfma32=1),
+         xcd_order=1, mfma32=1, prefetch=1),
     dict(stage1_tile_m=128, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=64, splitk=48,
-         xcd_order=1, mfma32=0),
+         xcd_order=1, mfma32=0, prefetch=2),
     dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=64,
-         xcd_order=0, mfma32=1),
+         xcd_order=0, mfma32=1, prefetch=2),
 ]
 
 
