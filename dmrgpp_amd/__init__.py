@@ -44,6 +44,8 @@ class DmkInfo(ctypes.Structure):
         ("bytes_bx", ctypes.c_double), ("precision", ctypes.c_int32), ("pad", ctypes.c_int32),
         ("split_tasks", ctypes.c_int64), ("ws_elems", ctypes.c_int64),
         ("is_complex", ctypes.c_int32), ("pad2", ctypes.c_int32), ("n_complex", ctypes.c_int64),
+        ("ms_scan", ctypes.c_double), ("ms_stage", ctypes.c_double),
+        ("ms_upload", ctypes.c_double), ("raw_bytes", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -298,6 +298,10 @@ int dmk_get_info(const dmk_handle* handle, dmk_info* info)
 		info->ws_elems          = s.ws_elems;
 		info->is_complex        = handle->is_complex ? 1 : 0;
 		info->n_complex         = handle->nc;
+		info->ms_scan           = s.ms_scan;
+		info->ms_stage          = s.ms_stage;
+		info->ms_upload         = s.ms_upload;
+		info->raw_bytes         = s.raw_bytes;
 		return DMK_OK;
 	});
 }

@@ -140,6 +140,26 @@ struct PrepPatch {
 	int32_t ldt;
 };
 
+// Device-side packing of the factors (initkron_kernels.hip): block (rows x cols, compact column
+// major at raw_off in the uploaded blocks) written as panel rows at dst with row stride lddst,
+// then `tail` (< 64) zeros after each row.
+struct PackJob {
+	int64_t raw_off;
+	int64_t dst;
+	int32_t rows;
+	int32_t cols;
+	int32_t lddst;
+	int32_t tail;
+	int32_t to_b; // 0: A pack, 1: B pack
+	int32_t pad;
+};
+struct PackTile {
+	int32_t job;
+	int32_t i0;
+	int32_t j0;
+	int32_t pad;
+};
+
 // Rows of K-major panels are padded to this many elements (16 bytes of bf16).
 constexpr int kPanelPad = 8;
 inline int64_t panel_ld(int64_t k) { return (k + kPanelPad - 1) / kPanelPad * kPanelPad; }

@@ -20,7 +20,10 @@
 #include "pool.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -52,23 +55,32 @@ void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchu
                           int64_t wsstride, TO* out, int64_t ystride, int nvec, int beta,
                           hipStream_t stream);
 void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream);
+void launch_pack_panels(const PackJob* jobs, const PackTile* tiles, int ntiles, const double* raw,
+                        double* apack, double* bpack, hipStream_t stream);
 void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hipStream_t stream);
 
 namespace {
 
 constexpr int kXcds = 8; // MI355X: 8 accelerator complex dies, 32 CUs each
 
-int64_t count_nnz(const double* p, int64_t rows, int64_t cols, int64_t ld)
+// setup_nC.cpp:47-124 counts the nonzeros of each block; only "any nonzero" matters, so the
+// scan stops at the first one.
+bool has_nonzero(const double* p, int64_t rows, int64_t cols, int64_t ld)
 {
 	if (!p)
-		return 0;
-	int64_t nnz = 0;
+		return false;
 	for (int64_t j = 0; j < cols; ++j) {
 		const double* c = p + j * ld;
 		for (int64_t i = 0; i < rows; ++i)
-			nnz += (c[i] != 0.0);
+			if (c[i] != 0.0)
+				return true;
 	}
-	return nnz;
+	return false;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 bool is_identity(const double* p, int64_t rows, int64_t cols, int64_t ld)
@@ -280,28 +292,69 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 				    && (int64_t(ldA[id]) < L[I] || int64_t(ldB[id]) < R[I]))
 					throw std::runtime_error("dmk_create: leading dimension smaller than rows");
 			}
-	// Scan every present block once, in parallel (nonzero test and identity test).
+	// Scan every distinct present block once, in parallel: a nonzero test that stops at the
+	// first nonzero, and an identity test (blocks shared by several terms, such as the
+	// identity factors, are one distinct block).
+	const auto t_scan0 = std::chrono::steady_clock::now();
 	std::vector<size_t> present;
 	for (size_t id = 0; id < kind.size(); ++id)
 		if (Amatrix[id] && Bmatrix[id])
 			present.push_back(id);
-	std::vector<char> scan(present.size(), NONE);
-	parallel_for(present.size(), [&](size_t t) {
+	struct Distinct {
+		const double* p;
+		int64_t       rows, cols, ld;
+		char          nonzero = 0, identity = 0;
+		int64_t       raw_off = -1; // offset among the uploaded blocks, -1 if not uploaded
+	};
+	std::vector<Distinct> blocks;
+	std::map<std::tuple<const double*, int64_t, int64_t, int64_t>, int32_t> block_id;
+	auto distinct = [&](const double* p, int64_t rows, int64_t cols, int64_t ld) {
+		auto key = std::make_tuple(p, rows, cols, ld);
+		auto it  = block_id.find(key);
+		if (it != block_id.end())
+			return it->second;
+		const int32_t d = int32_t(blocks.size());
+		Distinct      b;
+		b.p    = p;
+		b.rows = rows;
+		b.cols = cols;
+		b.ld   = ld;
+		blocks.push_back(b);
+		block_id.emplace(key, d);
+		return d;
+	};
+	std::vector<int32_t> a_of(present.size()), b_of(present.size());
+	for (size_t t = 0; t < present.size(); ++t) {
 		const size_t  id = present[t];
 		const int64_t I  = int64_t(id % size_t(np));
 		const int64_t J  = int64_t((id / size_t(np)) % size_t(np));
-		const double* a  = Amatrix[id];
-		const double* b  = Bmatrix[id];
-		if (count_nnz(a, L[I], L[J], int64_t(ldA[id])) == 0
-		    || count_nnz(b, R[I], R[J], int64_t(ldB[id])) == 0)
+		a_of[t]          = distinct(Amatrix[id], L[I], L[J], int64_t(ldA[id]));
+		b_of[t]          = distinct(Bmatrix[id], R[I], R[J], int64_t(ldB[id]));
+	}
+	parallel_for(blocks.size(), [&](size_t d) {
+		Distinct& b = blocks[d];
+		b.nonzero   = has_nonzero(b.p, b.rows, b.cols, b.ld) ? 1 : 0;
+		b.identity  = (b.nonzero && is_identity(b.p, b.rows, b.cols, b.ld)) ? 1 : 0;
+	});
+	std::vector<char> scan(present.size(), NONE);
+	for (size_t t = 0; t < present.size(); ++t) {
+		const Distinct& a = blocks[size_t(a_of[t])];
+		const Distinct& b = blocks[size_t(b_of[t])];
+		if (!a.nonzero || !b.nonzero)
 			scan[t] = NONE;
-		else if (is_identity(b, R[I], R[J], int64_t(ldB[id])))
+		else if (b.identity)
 			scan[t] = IDENT_B;
-		else if (is_identity(a, L[I], L[J], int64_t(ldA[id])))
+		else if (a.identity)
 			scan[t] = IDENT_A;
 		else
 			scan[t] = GENERAL;
-	});
+	}
+	std::vector<int32_t> a_block(kind.size(), -1), b_block(kind.size(), -1);
+	for (size_t t = 0; t < present.size(); ++t) {
+		a_block[present[t]] = a_of[t];
+		b_block[present[t]] = b_of[t];
+	}
+	stats_.ms_scan = ms_since(t_scan0);
 	for (size_t t = 0; t < present.size(); ++t) {
 		const size_t  id = present[t];
 		const int64_t I  = int64_t(id % size_t(np));
@@ -360,11 +413,11 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	// Packing is planned first (offsets and copy jobs) and done afterwards in parallel into a
 	// pinned staging buffer, then uploaded with one asynchronous copy per pack.
 	struct Copy {
-		int           to_b;  // 0: A pack, 1: B pack
-		int64_t       dst;   // element offset of row 0 in the pack
-		const double* src;   // column-major rows x cols block
-		int64_t       rows, cols, ldsrc, lddst;
-		int64_t       tail;  // elements zeroed after each copied row (row padding)
+		int     to_b;  // 0: A pack, 1: B pack
+		int64_t dst;   // element offset of row 0 in the pack
+		int32_t src;   // distinct block (rows x cols, column major)
+		int64_t rows, cols, lddst;
+		int64_t tail;  // elements zeroed after each copied row (row padding)
 	};
 	std::vector<Copy> copies;
 	int64_t           asize = 0, bsize = 0;
@@ -430,10 +483,10 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 				const size_t  id   = idx(I, J, ks[t]);
 				const int64_t c0   = col + t * L[J];
 				const bool    last = (c0 + L[J] == K[I]);
-				copies.push_back(Copy { 0, apanel_off[I] + c0, Amatrix[id], L[I], L[J],
-				                        int64_t(ldA[id]), ldap[I], last ? ldap[I] - K[I] : 0 });
-				copies.push_back(Copy { 1, boff + t * R[I] * ldbs, Bmatrix[id], R[I], R[J],
-				                        int64_t(ldB[id]), ldbs, ldbs - R[J] });
+				copies.push_back(Copy { 0, apanel_off[I] + c0, a_block[id], L[I], L[J], ldap[I],
+				                        last ? ldap[I] - K[I] : 0 });
+				copies.push_back(Copy { 1, boff + t * R[I] * ldbs, b_block[id], R[I], R[J], ldbs,
+				                        ldbs - R[J] });
 			}
 			PTask tk {};
 			tk.coff      = bx_off[I] + col;
@@ -494,8 +547,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 					const int64_t lda  = panel_ld(L[J]);
 					const int64_t aoff = asize;
 					asize += L[I] * lda;
-					copies.push_back(Copy { 0, aoff, Amatrix[id], L[I], L[J], int64_t(ldA[id]),
-					                        lda, lda - L[J] });
+					copies.push_back(Copy { 0, aoff, a_block[id], L[I], L[J], lda, lda - L[J] });
 					PSeg s {};
 					s.lbase = PBASE_APACK;
 					s.loff  = aoff;
@@ -511,8 +563,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 					const int64_t ldb  = panel_ld(R[J]);
 					const int64_t boff = bsize;
 					bsize += R[I] * ldb;
-					copies.push_back(Copy { 1, boff, Bmatrix[id], R[I], R[J], int64_t(ldB[id]),
-					                        ldb, ldb - R[J] });
+					copies.push_back(Copy { 1, boff, b_block[id], R[I], R[J], ldb, ldb - R[J] });
 					PSeg s {};
 					s.lbase = PBASE_XP;
 					s.loff  = prep_patches_[size_t(J)].pofs;
@@ -542,39 +593,73 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	DMK_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 	for (auto& e : ev_)
 		DMK_HIP_CHECK(hipEventCreate(&e));
-	// Pack into pinned staging memory in parallel, then upload both packs asynchronously.
+	// Upload every distinct block a copy job reads, compact and column major, through pinned
+	// staging (filled in parallel), then build the panels on the device.
+	const auto t_stage0 = std::chrono::steady_clock::now();
+	int64_t    raw_elems = 0;
+	for (const Copy& c : copies) {
+		Distinct& b = blocks[size_t(c.src)];
+		if (b.raw_off < 0) {
+			b.raw_off = raw_elems;
+			raw_elems += b.rows * b.cols;
+		}
+	}
+	std::vector<int32_t> uploaded;
+	for (size_t d = 0; d < blocks.size(); ++d)
+		if (blocks[d].raw_off >= 0)
+			uploaded.push_back(int32_t(d));
+	const size_t rbytes = size_t(std::max<int64_t>(raw_elems, 1)) * sizeof(double);
+	double*      hraw   = static_cast<double*>(pool_host_alloc(rbytes));
+	parallel_for(uploaded.size(), [&](size_t u) {
+		const Distinct& b = blocks[size_t(uploaded[u])];
+		double*         d = hraw + b.raw_off;
+		if (b.ld == b.rows)
+			std::memcpy(d, b.p, size_t(b.rows * b.cols) * sizeof(double));
+		else
+			for (int64_t j = 0; j < b.cols; ++j)
+				std::memcpy(d + j * b.rows, b.p + j * b.ld, size_t(b.rows) * sizeof(double));
+	});
+	std::vector<PackJob>  jobs;
+	std::vector<PackTile> ptiles;
+	jobs.reserve(copies.size());
+	for (const Copy& c : copies) {
+		PackJob j {};
+		j.raw_off = blocks[size_t(c.src)].raw_off;
+		j.dst     = c.dst;
+		j.rows    = int32_t(c.rows);
+		j.cols    = int32_t(c.cols);
+		j.lddst   = int32_t(c.lddst);
+		j.tail    = int32_t(c.tail);
+		j.to_b    = c.to_b;
+		const int32_t jid = int32_t(jobs.size());
+		jobs.push_back(j);
+		for (int32_t i0 = 0; i0 < j.rows; i0 += 64)
+			for (int32_t j0 = 0; j0 < std::max(j.cols, 1); j0 += 64)
+				ptiles.push_back(PackTile { jid, i0, j0, 0 });
+	}
+	stats_.ms_stage = ms_since(t_stage0);
+	const auto t_up0 = std::chrono::steady_clock::now();
 	const size_t abytes = size_t(std::max<int64_t>(asize, 1)) * sizeof(double);
 	const size_t bbytes = size_t(std::max<int64_t>(bsize, 1)) * sizeof(double);
-	double*      ha     = static_cast<double*>(pool_host_alloc(abytes + bbytes));
-	double*      hb     = ha + std::max<int64_t>(asize, 1);
-	parallel_for(copies.size(), [&](size_t t) {
-		// Row-major destination from a column-major source, in 32 x 32 blocks.
-		const Copy& c = copies[t];
-		double*     d = (c.to_b ? hb : ha) + c.dst;
-		for (int64_t i0 = 0; i0 < c.rows; i0 += 32) {
-			const int64_t i1 = std::min(c.rows, i0 + 32);
-			for (int64_t j0 = 0; j0 < c.cols; j0 += 32) {
-				const int64_t j1 = std::min(c.cols, j0 + 32);
-				for (int64_t j = j0; j < j1; ++j) {
-					const double* sc = c.src + j * c.ldsrc;
-					for (int64_t i = i0; i < i1; ++i)
-						d[i * c.lddst + j] = sc[i];
-				}
-			}
-			if (c.tail > 0)
-				for (int64_t i = i0; i < i1; ++i)
-					std::memset(d + i * c.lddst + c.cols, 0, size_t(c.tail) * sizeof(double));
-		}
-	});
-	d_apack_ = static_cast<double*>(pool_device_alloc(abytes));
-	d_bpack_ = static_cast<double*>(pool_device_alloc(bbytes));
-	hipError_t e1 = hipMemcpyAsync(d_apack_, ha, abytes, hipMemcpyHostToDevice, stream_);
-	hipError_t e2 = hipMemcpyAsync(d_bpack_, hb, bbytes, hipMemcpyHostToDevice, stream_);
+	d_apack_            = static_cast<double*>(pool_device_alloc(abytes));
+	d_bpack_            = static_cast<double*>(pool_device_alloc(bbytes));
+	double*   draw      = static_cast<double*>(pool_device_alloc(rbytes));
+	PackJob*  djobs     = dev_alloc_copy(jobs);
+	PackTile* dtiles    = dev_alloc_copy(ptiles);
+	hipError_t e1 = hipMemcpyAsync(draw, hraw, rbytes, hipMemcpyHostToDevice, stream_);
+	if (e1 == hipSuccess)
+		launch_pack_panels(djobs, dtiles, int(ptiles.size()), draw, d_apack_, d_bpack_, stream_);
+	hipError_t e2 = hipGetLastError();
 	hipError_t e3 = hipStreamSynchronize(stream_);
-	pool_host_free(ha);
+	pool_host_free(hraw);
+	pool_device_free(draw);
+	pool_device_free(djobs);
+	pool_device_free(dtiles);
 	DMK_HIP_CHECK(e1);
 	DMK_HIP_CHECK(e2);
 	DMK_HIP_CHECK(e3);
+	stats_.ms_upload = ms_since(t_up0);
+	stats_.raw_bytes = double(rbytes);
 	d_prep_tiles_   = dev_alloc_copy(prep_tiles_);
 	d_prep_patches_ = dev_alloc_copy(prep_patches_);
 	// Per precision mode: a tuning the autotuner chose for this histogram earlier in the

@@ -41,6 +41,12 @@ struct PlanStats {
 	int64_t ws_elems          = 0; // split-k partials
 	int64_t max_patch_left    = 0;
 	int64_t max_patch_right   = 0;
+	// Plan construction (host wall time): block scan, staging of the uploaded blocks, upload
+	// plus device packing; bytes uploaded.
+	double ms_scan   = 0;
+	double ms_stage  = 0;
+	double ms_upload = 0;
+	double raw_bytes = 0;
 };
 
 // Launch shape of a plan: output tile per stage, split-k granularity of stage 2 (target

@@ -1,5 +1,7 @@
 #include "pool.h"
 
+#include <atomic>
+
 #include "dmk_internal.h"
 
 #include <algorithm>
@@ -141,11 +143,13 @@ void parallel_for(size_t n, const std::function<void(size_t)>& fn, unsigned thre
 			fn(i);
 		return;
 	}
+	// Dynamic scheduling: the items differ in cost (blocks of very different sizes).
+	std::atomic<size_t>      next { 0 };
 	std::vector<std::thread> pool;
 	pool.reserve(threads);
 	for (unsigned t = 0; t < threads; ++t)
-		pool.emplace_back([&, t] {
-			for (size_t i = t; i < n; i += threads)
+		pool.emplace_back([&] {
+			for (size_t i = next.fetch_add(1); i < n; i = next.fetch_add(1))
 				fn(i);
 		});
 	for (auto& th : pool)

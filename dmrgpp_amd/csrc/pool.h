@@ -21,7 +21,7 @@ bool  pool_device_owns(void* p);
 void* pool_host_alloc(size_t bytes); // pinned
 void  pool_host_free(void* p);
 
-// Runs fn(i) for i in [0, n) on up to `threads` host threads (static blocks of indices).
+// Runs fn(i) for i in [0, n) on up to `threads` host threads (items handed out one at a time).
 void parallel_for(size_t n, const std::function<void(size_t)>& fn, unsigned threads = 0);
 
 } // namespace dmk

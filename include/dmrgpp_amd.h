@@ -71,6 +71,10 @@ typedef struct dmk_info {
 	int32_t is_complex;        /* 1: created by dmk_create_z (n above is 2 n_complex) */
 	int32_t pad2;
 	int64_t n_complex;         /* complex vector length (equals n for a real handle) */
+	double  ms_scan;           /* handle creation, host wall ms: block scan (nonzero, identity) */
+	double  ms_stage;          /*   copy of the uploaded blocks into pinned staging */
+	double  ms_upload;         /*   upload and panel packing on the device */
+	double  raw_bytes;         /*   bytes uploaded */
 } dmk_info;
 
 /* Launch shape of a handle: output tile (64 or 128 each way) of the stage-1 and stage-2
