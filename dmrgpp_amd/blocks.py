@@ -41,14 +41,17 @@ class Basis:
 
 @dataclass
 class BSOp:
-    """Block-sparse operator: blocks[(go, gi)] is a dense (size_out[go] x size_in[gi]) array."""
+    """Block-sparse operator: blocks[(go, gi)] is a dense (size_out[go] x size_in[gi]) array,
+    kept column major (Fortran order), the layout the Kron plugin takes, so that building a
+    superblock problem copies nothing."""
     bout: Basis
     bin: Basis
     blocks: dict = field(default_factory=dict)
     fermionic: bool = False
 
     def transpose(self) -> "BSOp":
-        return BSOp(self.bin, self.bout, {(gi, go): b.T.copy() for (go, gi), b in self.blocks.items()},
+        return BSOp(self.bin, self.bout,
+                    {(gi, go): np.asfortranarray(b.T) for (go, gi), b in self.blocks.items()},
                     self.fermionic)
 
     def scaled(self, c: float) -> "BSOp":
@@ -83,12 +86,13 @@ class BSOp:
                 continue
             wo, (ngo, _) = wout[go]
             wi, (ngi, _) = win[gi]
-            out[(ngo, ngi)] = wo.T @ b @ wi
+            # (wi^T b^T wo)^T: the C-ordered product's transpose is the Fortran-ordered result.
+            out[(ngo, ngi)] = ((wi.T @ b.T) @ wo).T
         return BSOp(bout, bin_, out, self.fermionic)
 
 
 def identity(b: Basis) -> BSOp:
-    return BSOp(b, b, {(g, g): np.eye(s) for g, s in enumerate(b.sizes)})
+    return BSOp(b, b, {(g, g): np.eye(s, order="F") for g, s in enumerate(b.sizes)})
 
 
 def add_qn(a: tuple, b: tuple) -> tuple:
@@ -137,7 +141,7 @@ class Product:
                 bm = np.eye(nb_o) if b is None else b
                 blk = out.get((Go, Gi))
                 if blk is None:
-                    blk = np.zeros((self.basis.sizes[Go], self.basis.sizes[Gi]))
+                    blk = np.zeros((self.basis.sizes[Go], self.basis.sizes[Gi]), order="F")
                     out[(Go, Gi)] = blk
                 blk[oo:oo + na_o * nb_o, oi:oi + na_i * nb_i] += coef * np.kron(bm, am)
         ferm = (opA is not None and opA.fermionic) != (opB is not None and opB.fermionic)

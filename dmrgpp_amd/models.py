@@ -34,7 +34,7 @@ def _site_op(basis: Basis, dense: np.ndarray, fermionic=False) -> BSOp:
         for gi in range(basis.ngroups):
             b = dense[basis.offsets[go]:basis.offsets[go + 1], basis.offsets[gi]:basis.offsets[gi + 1]]
             if np.any(b != 0):
-                blocks[(go, gi)] = b.copy()
+                blocks[(go, gi)] = np.asfortranarray(b)
     return BSOp(basis, basis, blocks, fermionic)
 
 

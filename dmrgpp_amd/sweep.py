@@ -78,7 +78,8 @@ def grow(A: Block, B: Block, model: SiteModel, lattice: Lattice, keep) -> Block:
                 a = a.times_parity_in(A.parity)
             H.add_(P.kron(a, B.ops[(j, oj)], c))
     ops = {}
-    pA = BSOp(A.basis, A.basis, {(g, g): np.eye(s) * A.parity[g] for g, s in enumerate(A.basis.sizes)})
+    pA = BSOp(A.basis, A.basis,
+              {(g, g): np.eye(s, order="F") * A.parity[g] for g, s in enumerate(A.basis.sizes)})
     for (s, name), op in A.ops.items():
         if keep(s):
             ops[(s, name)] = P.kron(op, None)
