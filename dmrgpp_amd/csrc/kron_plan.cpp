@@ -663,14 +663,35 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	d_prep_tiles_   = dev_alloc_copy(prep_tiles_);
 	d_prep_patches_ = dev_alloc_copy(prep_patches_);
 	// Per precision mode: a tuning the autotuner chose for this histogram earlier in the
-	// process, else the default.
+	// process, else the mode's default.
 	const int p0 = precision_;
 	for (int p = 0; p < 3; ++p) {
-		precision_ = p;
+		precision_     = p;
+		tuning_of_[p] = default_tuning(p);
 		cached_tuning(tuning_key(), tuning_of_[p]);
 	}
 	precision_ = p0;
 	retune(tuning_of_[p0]);
+}
+
+// The autotuner's usual choice per mode on the m = 2048 ladder superblocks: 64 x 64 tiles with
+// split-k 2048 and loads two chunks ahead (FP32 on 32 x 32 MFMA blocks); BF16x3 with 128 x 64
+// stage-2 tiles in the XCD-aware order, split-k 1024.
+Tuning default_tuning(int precision)
+{
+	Tuning t;
+	t.splitk = 2048;
+	t.depth  = 2;
+	if (precision == PREC_FP32)
+		t.mfma32 = 1;
+	if (precision == PREC_BF16X3) {
+		t.bm2    = 128;
+		t.splitk = 1024;
+		t.xcd    = 1;
+		t.mfma32 = 1;
+		t.depth  = 1;
+	}
+	return t;
 }
 
 std::string Plan::tuning_key() const

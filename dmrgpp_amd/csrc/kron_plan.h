@@ -81,6 +81,9 @@ struct StageTables {
 	void               release();
 };
 
+// The starting tuning of a precision mode (before any autotuning).
+Tuning default_tuning(int precision);
+
 // Process-wide autotuner cache (histogram signature -> tuning).
 bool cached_tuning(const std::string& key, Tuning& t);
 void store_tuning(const std::string& key, const Tuning& t);
