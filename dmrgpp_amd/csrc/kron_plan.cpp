@@ -1128,41 +1128,79 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 {
 	if (iters < 1)
 		throw std::invalid_argument("dmk_autotune: iters must be >= 1");
-	const Tuning start = tuning_;
-	std::vector<Tuning> cands;
-	// MFMA form: FP32 16x16x4 or 32x32x2, BF16x3 16x16x32 or 32x32x16.
-	const int nvar = (precision_ == PREC_FP64) ? 1 : 2;
-	const int ndepth = (precision_ == PREC_BF16X3) ? 1 : 2; // the BF16x3 kernel prefetches 1
-	for (int var = 0; var < nvar; ++var)
-	for (int depth = 1; depth <= ndepth; ++depth)
-		for (int xcd : { 0, 1 })
-			for (int bm1 : { 64, 128 })
-				for (int bn1 : { 64, 128 })
-					for (int bm2 : { 64, 128 })
-						for (int bn2 : { 64, 128 })
-							for (int sk : { 0, 1024, 2048 }) {
-								Tuning t;
-								t.bm1    = bm1;
-								t.bn1    = bn1;
-								t.bm2    = bm2;
-								t.bn2    = bn2;
-								t.splitk = sk;
-								t.xcd    = xcd;
-								t.mfma32 = var;
-								t.depth  = depth;
-								cands.push_back(t);
-							}
-	Tuning best     = start;
-	double best_ms  = 1e300;
-	for (const Tuning& t : cands) {
+	// Coordinate descent from the mode's default: each launch parameter in turn takes every
+	// value with the others fixed, keeping the fastest; two passes.
+	auto time_of = [&](const Tuning& t) {
 		retune(t);
 		double ms = 0;
 		profile(vin, vout, 1, nullptr, nullptr, &ms); // warm-up
 		profile(vin, vout, iters, nullptr, nullptr, &ms);
-		if (ms < best_ms) {
-			best_ms = ms;
-			best    = t;
+		return ms;
+	};
+	Tuning best    = default_tuning(precision_);
+	double best_ms = time_of(best);
+	auto   try_set = [&](const std::vector<Tuning>& cands) {
+		for (const Tuning& t : cands) {
+			if (t == best)
+				continue;
+			const double ms = time_of(t);
+			if (ms < best_ms) {
+				best_ms = ms;
+				best    = t;
+			}
 		}
+	};
+	for (int pass = 0; pass < 2; ++pass) {
+		std::vector<Tuning> c;
+		// MFMA form (FP32 16x16x4 or 32x32x2, BF16x3 16x16x32 or 32x32x16).
+		if (precision_ != PREC_FP64)
+			for (int v : { 0, 1 }) {
+				Tuning t = best;
+				t.mfma32 = v;
+				c.push_back(t);
+			}
+		try_set(c);
+		c.clear();
+		// Load pipeline depth (the BF16x3 kernel runs one chunk ahead).
+		if (precision_ != PREC_BF16X3)
+			for (int d : { 1, 2 }) {
+				Tuning t = best;
+				t.depth  = d;
+				c.push_back(t);
+			}
+		try_set(c);
+		c.clear();
+		for (int bm : { 64, 128 })
+			for (int bn : { 64, 128 }) {
+				Tuning t = best;
+				t.bm2    = bm;
+				t.bn2    = bn;
+				c.push_back(t);
+			}
+		try_set(c);
+		c.clear();
+		for (int sk : { 0, 768, 1024, 1536, 2048, 3072, 4096 }) {
+			Tuning t = best;
+			t.splitk = sk;
+			c.push_back(t);
+		}
+		try_set(c);
+		c.clear();
+		for (int x : { 0, 1 }) {
+			Tuning t = best;
+			t.xcd    = x;
+			c.push_back(t);
+		}
+		try_set(c);
+		c.clear();
+		for (int bm : { 64, 128 })
+			for (int bn : { 64, 128 }) {
+				Tuning t = best;
+				t.bm1    = bm;
+				t.bn1    = bn;
+				c.push_back(t);
+			}
+		try_set(c);
 	}
 	retune(best);
 	store_tuning(tuning_key(), best);

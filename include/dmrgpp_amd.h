@@ -157,8 +157,10 @@ int dmk_get_info(const dmk_handle* handle, dmk_info* info);
 int dmk_get_tuning(const dmk_handle* handle, dmk_tuning* tuning);
 int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning);
 
-/* Times the candidate tunings on this handle's problem (iters matvecs each, in the current
- * precision, on the device vectors vin/vout) and keeps the fastest. The choice is cached in the
+/* Tunes the launch shape on this handle's problem (iters matvecs per candidate, in the current
+ * precision, on the device vectors vin/vout): coordinate descent from the mode's default over the
+ * MFMA form, load depth, stage-2 tile, split-k, tile order and stage-1 tile, two passes, keeping
+ * the fastest. The choice is cached in the
  * process under the block-size histogram of the problem, and handles created later for a
  * problem with the same histogram start from it. chosen may be NULL. */
 int dmk_autotune_d(dmk_handle* handle, const double* vin, double* vout, int iters,
