@@ -215,7 +215,9 @@ def main():
     op.set_precision("fp64")
     lz = {}
     modes = (("fp64", {}), ("fp32+refine", {"precision": "fp32", "refine": 1}),
-             ("bf16x3+refine", {"precision": "bf16x3", "refine": 1}))
+             ("bf16x3+refine", {"precision": "bf16x3", "refine": 1}),
+             ("fp32+davidson", {"precision": "fp32", "refine": 2}),
+             ("bf16x3+davidson", {"precision": "bf16x3", "refine": 2}))
     for mode, kw in modes:
         r = op.lanczos_ground(dx, dz, **kw)
         lz[mode] = r
@@ -232,7 +234,8 @@ def main():
         steps_per_sweep = 2 * (lattice.nsites - 3)
         for mode, solver in (("fp64", sweep.GpuSolver(precision="fp64")),
                              ("fp32+refine", sweep.GpuSolver(precision="fp32", refine=1)),
-                             ("bf16x3+refine", sweep.GpuSolver(precision="bf16x3", refine=1))):
+                             ("bf16x3+refine", sweep.GpuSolver(precision="bf16x3", refine=1)),
+                             ("fp32+davidson", sweep.GpuSolver(precision="fp32", refine=2))):
             st = sweep.clone_state(state)
             t = time.perf_counter()
             e, recs = sweep.run(model, lattice, lambda N: (0,), M, [M], solver, log=log,
@@ -247,7 +250,7 @@ def main():
                                "energy_last": e}
             log(f"sweep segment {mode}: {len(recs)} steps, {per:.3f} s/step "
                 f"(eigensolve {lz_ms:.1f} ms/step), full sweep ~{per * steps_per_sweep:.1f} s")
-        for mode in ("fp32+refine", "bf16x3+refine"):
+        for mode in ("fp32+refine", "bf16x3+refine", "fp32+davidson"):
             sweep_out[mode]["eigensolve_speedup_vs_fp64"] = round(
                 sweep_out["fp64"]["eigensolve_ms_per_step"]
                 / max(1e-9, sweep_out[mode]["eigensolve_ms_per_step"]), 3)

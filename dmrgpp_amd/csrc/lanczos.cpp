@@ -11,6 +11,8 @@
 // FP64 matvecs because that vector is already close to the FP64 eigenvector.
 #include "lanczos.h"
 
+#include "davidson.h"
+
 #include "pool.h"
 #include "tridiag.h"
 
@@ -335,7 +337,25 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 		r.energy          = d.energy;
 		r.steps           = d.m;
 		r.matvecs         = d.matvecs;
-		if (p.precision != PREC_FP64 && p.refine) {
+		bool residual_done = false;
+		if (p.precision != PREC_FP64 && p.refine == 2) {
+			// FP64 Davidson (Olsen's diagonal correction) from the mixed-precision Ritz vector,
+			// to a residual of sqrt(refine_tolerance): the Rayleigh quotient's error is of the
+			// order of the squared residual over the gap, the size of the Lanczos |dE| test.
+			DavidsonParams dp;
+			dp.tolerance   = std::sqrt(p.refine_tolerance);
+			dp.max_basis   = 12;
+			dp.max_matvecs = std::max(p.max_steps, 8);
+			dp.precond     = PRECOND_DIAG;
+			DMK_HIP_CHECK(hipMemcpyAsync(ws.tmp2(), z, size_t(n) * sizeof(double),
+			                             hipMemcpyDeviceToDevice, plan.stream()));
+			const DavidsonReport dr = davidson_ground(plan, ws.tmp2(), z, dp, nullptr);
+			r.energy   = dr.energy;
+			r.steps    = dr.iterations;
+			r.matvecs += dr.matvecs;
+			r.residual = dr.residual;
+			residual_done = true;
+		} else if (p.precision != PREC_FP64 && p.refine) {
 			plan.set_precision(PREC_FP64);
 			// Restart from the mixed-precision Ritz vector (copied: z is overwritten below).
 			DMK_HIP_CHECK(hipMemcpyAsync(ws.tmp2(), z, size_t(n) * sizeof(double),
@@ -348,16 +368,18 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 			r.steps  = d2.m;
 			r.matvecs += d2.matvecs;
 		}
-		// FP64 residual ||H z - E z|| / ||z||.
+		// FP64 residual ||H z - E z|| / ||z|| (the Davidson refinement reports its own).
 		plan.set_precision(PREC_FP64);
-		double* NRM = ws.scal() + 2 * (ws.columns() - 1);
-		plan.apply(z, n, ws.tmp(), n, 1, 0);
-		launch_residual(ws.tmp(), z, r.energy, n, ws.partial(), NRM, plan.stream());
-		launch_dot(z, z, n, ws.partial(), NRM + 1, 1, plan.stream());
-		DMK_HIP_CHECK(hipMemcpyAsync(ws.host(), NRM, 2 * sizeof(double), hipMemcpyDeviceToHost,
-		                             plan.stream()));
-		DMK_HIP_CHECK(hipStreamSynchronize(plan.stream()));
-		r.residual = ws.host()[0] / (ws.host()[1] > 0 ? ws.host()[1] : 1.0);
+		if (!residual_done) {
+			double* NRM = ws.scal() + 2 * (ws.columns() - 1);
+			plan.apply(z, n, ws.tmp(), n, 1, 0);
+			launch_residual(ws.tmp(), z, r.energy, n, ws.partial(), NRM, plan.stream());
+			launch_dot(z, z, n, ws.partial(), NRM + 1, 1, plan.stream());
+			DMK_HIP_CHECK(hipMemcpyAsync(ws.host(), NRM, 2 * sizeof(double),
+			                             hipMemcpyDeviceToHost, plan.stream()));
+			DMK_HIP_CHECK(hipStreamSynchronize(plan.stream()));
+			r.residual = ws.host()[0] / (ws.host()[1] > 0 ? ws.host()[1] : 1.0);
+		}
 		plan.set_precision(saved_prec);
 	} catch (...) {
 		(void)hipEventDestroy(ev0);

@@ -184,7 +184,9 @@ typedef struct dmk_lanczos_params {
 	double  tolerance;        /* stop when the lowest Ritz value moves less than this */
 	int32_t reortho;          /* 1: full Gram-Schmidt each step (LanczosVectors.h:243-271) */
 	int32_t precision;        /* matvec precision of the first run (enum dmk_precision) */
-	int32_t refine;           /* 1: after a non-FP64 run, restart in FP64 from its Ritz vector */
+	int32_t refine;           /* after a non-FP64 run: 1 = restart Lanczos in FP64 from its Ritz
+	                             vector; 2 = FP64 Davidson (diagonal correction) from it to a
+	                             residual of sqrt(refine_tolerance) */
 	int32_t refine_min_steps;
 	double  refine_tolerance;
 	int32_t graph;            /* 1: each step is one replay of a captured HIP graph (default) */

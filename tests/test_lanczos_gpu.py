@@ -89,6 +89,25 @@ def test_gpu_lanczos_fp32_with_fp64_refinement():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_gpu_lanczos_mixed_with_davidson_refinement(precision):
+    # refine = 2: the FP64 refinement is a Davidson run (diagonal correction) from the mixed Ritz
+    # vector to a residual of sqrt(refine_tolerance) = 1e-6; its energy is the FP64 Lanczos
+    # energy within 1e-10 relative, and the residual it reports is below the target.
+    prob = _qn(3, {-4: 10, -2: 60, 0: 100, 2: 60, 4: 10})
+    init = np.random.default_rng(1234).uniform(-1, 1, prob.n)
+    r64, _ = gpu_lanczos(prob, init)
+    rd, z = gpu_lanczos(prob, init, precision=precision, refine=2)
+    assert abs(rd["energy"] - r64["energy"]) < 1e-10 * abs(r64["energy"]), (rd, r64)
+    assert rd["residual"] <= 1e-6 * (1 + 1e-9)
+    assert rd["matvecs"] > rd["matvecs_first"]
+    # The returned vector is the unit Ritz vector of that energy.
+    zn = z / np.linalg.norm(z)
+    hz = oracle.plugin_apply(prob, zn)
+    assert abs(zn @ hz - rd["energy"]) < 1e-10 * abs(rd["energy"])
+
+
+@pytest.mark.gpu
 def test_gpu_lanczos_reortho():
     prob = _qn(2, {-2: 30, 0: 50, 2: 30})
     init = np.random.default_rng(1234).uniform(-1, 1, prob.n)
