@@ -1,7 +1,8 @@
 """The benchmark workloads as sweep-driver settings, and a command-line runner.
 
   python -m dmrgpp_amd.configs --config heisenberg-chain-32 [--precision fp64|fp32|bf16x3]
-                               [--sweeps N] [--stop-after K] [--wft] [--json out.json]
+                               [--refine 1|2] [--sweeps N] [--stop-after K] [--wft]
+                               [--json out.json]
 
 Each run prints one line per DMRG step and, with --json, writes the step records and the total
 sweep wall time.
@@ -40,11 +41,11 @@ CONFIGS = {
 
 
 def run_config(name: str, precision="fp64", m=None, sweeps=None, stop_after=None, log=print,
-               solver=None, wft=None):
+               solver=None, wft=None, refine=2):
     c = CONFIGS[name]
     m = m or c["m"]
     nsw = c["sweeps"] if sweeps is None else sweeps
-    solver = solver or sweep.GpuSolver(precision=precision)
+    solver = solver or sweep.GpuSolver(precision=precision, refine=refine)
     t0 = time.perf_counter()
     e, recs = sweep.run(c["model"](), c["lattice"](), c["target"], m, [m] * nsw, solver, log=log,
                         stop_after=stop_after, wft=wft)
@@ -61,10 +62,14 @@ def main(argv=None):
     ap.add_argument("--stop-after", type=int, default=None)
     ap.add_argument("--wft", action="store_true",
                     help="start finite-sweep Lanczos runs from the transformed previous state")
+    ap.add_argument("--refine", type=int, default=2,
+                    help="FP64 refinement of a mixed-precision solve: 1 = Lanczos restart, "
+                         "2 = Davidson (default)")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     e, recs, wall = run_config(a.config, a.precision, a.m, a.sweeps, a.stop_after,
-                               log=lambda s: print(s, flush=True), wft=True if a.wft else None)
+                               log=lambda s: print(s, flush=True), wft=True if a.wft else None,
+                               refine=a.refine)
     sweep_steps = [r for r in recs if r.phase != "infinite"]
     out = {"config": a.config, "precision": a.precision, "energy": e, "wall_s": wall,
            "steps": len(recs), "matvecs": sum(r.matvecs for r in recs),
