@@ -124,66 +124,78 @@ void build_tiles_plain(StageTables& st, const std::vector<double>& cost)
 }
 
 // XCD-aware order. Workgroup b runs on XCD b % kXcds, and the XCDs do not share L2. A task with
-// at least kXcds tiles is cut into kXcds rectangular regions of its tile grid, one per XCD, so
-// the tiles that read the same row panel (and column panel) of the operands share one L2;
-// smaller tasks go whole to the least loaded XCD. Each XCD's list stays in decreasing task
-// cost; slot i of XCD x becomes blockIdx kXcds * i + x, and holes are empty tiles (task -1).
+// at least kXcds tiles is cut into kXcds rectangular regions of its tile grid, so the tiles that
+// read the same row panel (and column panel) of the operands share one L2; smaller tasks are one
+// region. Regions go to XCDs longest first, each to the least loaded XCD (the uneven cuts of
+// similar tile grids would otherwise pile onto the same XCDs); each XCD's list keeps the
+// regions in decreasing task cost. Slot i of XCD x becomes blockIdx kXcds * i + x, and holes
+// are empty tiles (task -1).
 void build_tiles_xcd(StageTables& st, const std::vector<double>& cost)
 {
 	const int kTileM = st.bm, kTileN = st.bn;
-	std::vector<int> order(st.tasks.size());
-	std::iota(order.begin(), order.end(), 0);
-	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-	std::vector<std::vector<Tile>> lists(kXcds);
-	std::vector<double>            load(kXcds, 0.0);
-	for (int t : order) {
+	struct Region {
+		int    task, r0, r1, c0, c1;
+		double cost;
+	};
+	std::vector<Region> regions;
+	for (size_t t = 0; t < st.tasks.size(); ++t) {
 		const PTask& tk = st.tasks[t];
-		const int   tr = std::max(1, (tk.m + kTileM - 1) / kTileM);
-		const int   tc = std::max(1, (tk.n + kTileN - 1) / kTileN);
+		const int    tr = std::max(1, (tk.m + kTileM - 1) / kTileM);
+		const int    tc = std::max(1, (tk.n + kTileN - 1) / kTileN);
 		const double per_tile = cost[t] / double(tr * tc);
-		if (tr * tc >= kXcds) {
-			// Region grid pr x pc = kXcds with regions as square as the tile grid allows.
-			int    best_pr = 1;
-			double best    = 1e300;
-			for (int pr = 1; pr <= kXcds; pr *= 2) {
-				const int pc = kXcds / pr;
-				if (pr > tr || pc > tc)
-					continue;
-				const double rh = double(tr) / pr, rw = double(tc) / pc;
-				const double score = std::max(rh, rw) / std::min(rh, rw);
-				if (score < best) {
-					best    = score;
-					best_pr = pr;
-				}
+		if (tr * tc < kXcds) {
+			regions.push_back(Region { int(t), 0, tr, 0, tc, cost[t] });
+			continue;
+		}
+		// Region grid pr x pc = kXcds with regions as square as the tile grid allows.
+		int    best_pr = 1;
+		double best    = 1e300;
+		for (int pr = 1; pr <= kXcds; pr *= 2) {
+			const int pc = kXcds / pr;
+			if (pr > tr || pc > tc)
+				continue;
+			const double rh = double(tr) / pr, rw = double(tc) / pc;
+			const double score = std::max(rh, rw) / std::min(rh, rw);
+			if (score < best) {
+				best    = score;
+				best_pr = pr;
 			}
-			int pr = best_pr, pc = kXcds / best_pr;
-			if (pr > tr || pc > tc) {
-				// Degenerate grid (e.g. 1 x 9): cut the longer side only.
-				pr = (tr >= tc) ? std::min(tr, kXcds) : 1;
-				pc = (tr >= tc) ? 1 : std::min(tc, kXcds);
+		}
+		int pr = best_pr, pc = kXcds / best_pr;
+		if (pr > tr || pc > tc) {
+			// Degenerate grid (e.g. 1 x 9): cut the longer side only.
+			pr = (tr >= tc) ? std::min(tr, kXcds) : 1;
+			pc = (tr >= tc) ? 1 : std::min(tc, kXcds);
+		}
+		for (int ri = 0; ri < pr; ++ri)
+			for (int ci = 0; ci < pc; ++ci) {
+				Region g { int(t), (tr * ri) / pr, (tr * (ri + 1)) / pr, (tc * ci) / pc,
+					   (tc * (ci + 1)) / pc, 0.0 };
+				g.cost = per_tile * double((g.r1 - g.r0) * (g.c1 - g.c0));
+				regions.push_back(g);
 			}
-			// XCDs ordered by current load so uneven regions even out across tasks.
-			std::vector<int> xs(kXcds);
-			std::iota(xs.begin(), xs.end(), 0);
-			std::stable_sort(xs.begin(), xs.end(), [&](int a, int b) { return load[a] < load[b]; });
-			int r = 0;
-			for (int ri = 0; ri < pr; ++ri) {
-				const int r0 = (tr * ri) / pr, r1 = (tr * (ri + 1)) / pr;
-				for (int ci = 0; ci < pc; ++ci, ++r) {
-					const int c0 = (tc * ci) / pc, c1 = (tc * (ci + 1)) / pc;
-					const int x  = xs[r % kXcds];
-					for (int i = r0; i < r1; ++i)
-						for (int j = c0; j < c1; ++j)
-							lists[x].push_back(Tile { t, i * kTileM, j * kTileN, 0 });
-					load[x] += per_tile * double((r1 - r0) * (c1 - c0));
-				}
-			}
-		} else {
-			const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
-			for (int i = 0; i < tr; ++i)
-				for (int j = 0; j < tc; ++j)
-					lists[x].push_back(Tile { t, i * kTileM, j * kTileN, 0 });
-			load[x] += cost[t];
+	}
+	std::vector<int> by_cost(regions.size());
+	std::iota(by_cost.begin(), by_cost.end(), 0);
+	std::stable_sort(by_cost.begin(), by_cost.end(),
+	                 [&](int a, int b) { return regions[a].cost > regions[b].cost; });
+	std::vector<std::vector<int>> owned(kXcds);
+	std::vector<double>           load(kXcds, 0.0);
+	for (int g : by_cost) {
+		const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+		owned[x].push_back(g);
+		load[x] += regions[g].cost;
+	}
+	std::vector<std::vector<Tile>> lists(kXcds);
+	for (int x = 0; x < kXcds; ++x) {
+		std::stable_sort(owned[x].begin(), owned[x].end(), [&](int a, int b) {
+			return cost[regions[a].task] > cost[regions[b].task];
+		});
+		for (int g : owned[x]) {
+			const Region& r = regions[g];
+			for (int i = r.r0; i < r.r1; ++i)
+				for (int j = r.c0; j < r.c1; ++j)
+					lists[x].push_back(Tile { r.task, i * kTileM, j * kTileN, 0 });
 		}
 	}
 	size_t longest = 0;
@@ -674,14 +686,15 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	retune(tuning_of_[p0]);
 }
 
-// The autotuner's usual choice per mode on the m = 2048 ladder superblocks: 64 x 64 tiles with
-// split-k 2048 and loads two chunks ahead (FP32 on 32 x 32 MFMA blocks); BF16x3 with 128 x 64
-// stage-2 tiles in the XCD-aware order, split-k 1024.
+// The autotuner's usual choice per mode on the m = 2048 ladder superblocks: 64 x 64 tiles in the
+// XCD-aware order with split-k 2048 and loads two chunks ahead (FP32 on 32 x 32 MFMA blocks);
+// BF16x3 with 128 x 64 stage-2 tiles, split-k 1024.
 Tuning default_tuning(int precision)
 {
 	Tuning t;
 	t.splitk = 2048;
 	t.depth  = 2;
+	t.xcd    = 1;
 	if (precision == PREC_FP32)
 		t.mfma32 = 1;
 	if (precision == PREC_BF16X3) {
