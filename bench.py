@@ -39,6 +39,9 @@ LX = 32
 # pipeline depth 1; the depth moves the loads in time, not the bytes); a run whose autotuner
 # picks another shape reports None.
 TRAFFIC = {
+    ("fp64", (64, 64, 2048, 1, 0)): 1.2210e9 + 0.0646e9,
+    ("fp32", (64, 64, 2048, 1, 1)): 0.5814e9 + 0.0332e9,
+    ("bf16x3", (128, 64, 1024, 1, 1)): 0.5472e9 + 0.0675e9,
     ("fp64", (64, 64, 2048, 0, 0)): 3.3655e9 + 0.0706e9,
     ("fp32", (64, 64, 2048, 0, 1)): 1.6939e9 + 0.0341e9,
     ("bf16x3", (128, 64, 1024, 1, 0)): 0.5651e9 + 0.0685e9,
