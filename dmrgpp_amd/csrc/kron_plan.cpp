@@ -1192,7 +1192,7 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 			}
 		try_set(c);
 		c.clear();
-		for (int sk : { 0, 768, 1024, 1536, 2048, 3072, 4096 }) {
+		for (int sk : { 0, 512, 768, 1024, 1280, 1536, 1792, 2048, 2560, 3072, 4096 }) {
 			Tuning t = best;
 			t.splitk = sk;
 			c.push_back(t);

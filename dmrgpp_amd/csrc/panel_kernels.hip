@@ -162,6 +162,16 @@ __device__ inline bool next_chunk(const PSeg* __restrict__ segs, int seg_end, in
 	return seg < seg_end;
 }
 
+// Number of BK chunks of a task (segments with k <= 0 have none).
+template <int BK>
+__device__ inline int chunk_count(const PSeg* __restrict__ segs, int seg_begin, int seg_end)
+{
+	int n = 0;
+	for (int q = seg_begin; q < seg_end; ++q)
+		n += max(0, (segs[q].k + BK - 1) / BK);
+	return n;
+}
+
 // Per-thread load geometry of one operand tile (ROWS x BK elements, VW-element vectors): thread
 // t loads vectors t + 256 q, i.e. rows t / VPR + (256 / VPR) q at vector column t % VPR.
 template <typename E, int ROWS, int BK> struct TileLoad {
@@ -183,18 +193,18 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 			p[q]          = base + int64_t(row) * ld + kv * VW;
 		}
 	}
+	// Issues the loads of the chunk at k0 of a segment K elements long, without branches (so the
+	// compiler can count the loads in flight and wait only for the set a store needs). A vector
+	// past K is replaced by the chunk's first vector of the same row, which is inside the row, and
+	// zeroed when stored (rows are zero padded past K, so a vector that starts before K needs no
+	// mask).
 	__device__ inline void load(vec_t (&v)[NV], int k0, int K, int tid) const
 	{
-		if (k0 + BK <= K) {
+		const int kvw = (tid % VPR) * VW;
+		const int off = k0 + kvw < K ? k0 : k0 - kvw;
 #pragma unroll
-			for (int q = 0; q < NV; ++q)
-				v[q] = *reinterpret_cast<const vec_t*>(p[q] + k0);
-		} else {
-			const bool ok = k0 + (tid % VPR) * VW < K; // rows are zero padded past K
-#pragma unroll
-			for (int q = 0; q < NV; ++q)
-				v[q] = ok ? *reinterpret_cast<const vec_t*>(p[q] + k0) : vec_t {};
-		}
+		for (int q = 0; q < NV; ++q)
+			v[q] = *reinterpret_cast<const vec_t*>(p[q] + off);
 	}
 	// Slot of vector kv of LDS row r. A 16-byte LDS access is served 8 lanes (32 banks) or 16
 	// lanes (64 banks) per pass; the fragment reads put consecutive rows on consecutive lanes,
@@ -213,14 +223,25 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 			f = r & (VPR - 1);
 		return kv ^ f;
 	}
-	__device__ inline void store(vec_t* __restrict__ s, const vec_t (&v)[NV], int tid) const
+	// Stores a chunk whose first lim elements lie inside its segment (lim >= BK: all of it).
+	__device__ inline void store(vec_t* __restrict__ s, const vec_t (&v)[NV], int tid,
+	                             int lim) const
 	{
 		const int r  = tid / VPR;
 		const int kv = tid % VPR;
+		if (lim >= BK) {
 #pragma unroll
-		for (int q = 0; q < NV; ++q) {
-			const int row          = r + RSTEP * q;
-			s[row * VPR + slot(row, kv)] = v[q];
+			for (int q = 0; q < NV; ++q) {
+				const int row               = r + RSTEP * q;
+				s[row * VPR + slot(row, kv)] = v[q];
+			}
+		} else {
+			const bool ok = kv * VW < lim;
+#pragma unroll
+			for (int q = 0; q < NV; ++q) {
+				const int row               = r + RSTEP * q;
+				s[row * VPR + slot(row, kv)] = ok ? v[q] : vec_t {};
+			}
 		}
 	}
 };
@@ -287,22 +308,30 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 
 	if (seg < tk.seg_end) {
 		// Load cursor: the next chunk whose global loads are issued (segment seg, offset ck0).
+		// Loads are issued unconditionally: past the tile's last chunk (cvalid false) the last
+		// chunk is loaded again and never stored.
 		LA   la;
 		LB   lb;
 		PSeg s      = segs[seg];
 		int  ck0    = 0;
+		int  lastk  = 0;
 		bool cvalid = true;
 		la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
 		lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
-		auto issue = [&](vec_t (&xa)[LA::NV], vec_t (&xb)[LB::NV]) {
-			la.load(xa, ck0, s.k, tid);
-			lb.load(xb, ck0, s.k, tid);
-			bool newseg;
-			cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
-			if (cvalid && newseg) {
-				s = segs[seg];
-				la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
-				lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
+		auto issue = [&](vec_t (&xa)[LA::NV], vec_t (&xb)[LB::NV], int& lim) {
+			const int kk = cvalid ? ck0 : lastk;
+			la.load(xa, kk, s.k, tid);
+			lb.load(xb, kk, s.k, tid);
+			lim   = s.k - kk;
+			lastk = kk;
+			if (cvalid) {
+				bool newseg;
+				cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
+				if (cvalid && newseg) {
+					s = segs[seg];
+					la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
+					lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
+				}
 			}
 		};
 		auto compute = [&](int b) {
@@ -339,59 +368,58 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 						acc[i][j] = Op::mma(fa[i][st], fb[j][st], acc[i][j]);
 		};
 		vec_t a0[LA::NV], b0[LB::NV];
-		int   buf = 0;
+		int   lim0, buf = 0;
+		// The loops are counted and leave only at the top, so that every path into the loop
+		// header has the same loads in flight and the compiler waits for one register set only.
+		const int nch = chunk_count<BK>(segs, seg, tk.seg_end);
 		if constexpr (PF == 1) {
 			// One chunk ahead: the loads of chunk c+1 fly during the MFMAs of chunk c.
-			issue(a0, b0);
-			la.store(As[0], a0, tid);
-			lb.store(Bs[0], b0, tid);
+			issue(a0, b0, lim0);
+			la.store(As[0], a0, tid, lim0);
+			lb.store(Bs[0], b0, tid, lim0);
 			__syncthreads();
-			for (;;) {
-				const bool have1 = cvalid;
-				if (have1)
-					issue(a0, b0);
+			for (int c = 0; c + 1 < nch; ++c) {
+				issue(a0, b0, lim0);
 				compute(buf);
-				if (!have1)
-					break;
-				la.store(As[buf ^ 1], a0, tid);
-				lb.store(Bs[buf ^ 1], b0, tid);
+				la.store(As[buf ^ 1], a0, tid, lim0);
+				lb.store(Bs[buf ^ 1], b0, tid, lim0);
 				__syncthreads();
 				buf ^= 1;
 			}
+			compute(buf);
 		} else {
-			// Two chunks ahead, two register sets: at the top of each half, LDS[buf] holds chunk
-			// c and one set holds chunk c+1 (when it exists); the loads of chunk c+2 go to the
-			// other set and are stored one chunk later.
+			// Two chunks ahead, two register sets: at the top of each iteration, LDS[buf] holds
+			// chunk c and set 1 holds chunk c+1 (or a repeat of the last chunk); the loads of
+			// chunk c+2 go to set 0 and are stored one chunk later.
 			vec_t a1[LA::NV], b1[LB::NV];
-			issue(a0, b0);
-			bool have1 = cvalid;
-			if (have1)
-				issue(a1, b1);
-			la.store(As[0], a0, tid);
-			lb.store(Bs[0], b0, tid);
+			int   lim1;
+			issue(a0, b0, lim0);
+			issue(a1, b1, lim1);
+			la.store(As[0], a0, tid, lim0);
+			lb.store(Bs[0], b0, tid, lim0);
 			__syncthreads();
-			for (;;) {
-				const bool have2 = cvalid;
-				if (have2)
-					issue(a0, b0);
+			int c = 0;
+			for (; c + 2 < nch; c += 2) {
+				issue(a0, b0, lim0);
 				compute(buf);
-				if (!have1)
-					break;
-				la.store(As[buf ^ 1], a1, tid);
-				lb.store(Bs[buf ^ 1], b1, tid);
+				la.store(As[buf ^ 1], a1, tid, lim1);
+				lb.store(Bs[buf ^ 1], b1, tid, lim1);
 				__syncthreads();
 				buf ^= 1;
-				const bool have3 = cvalid;
-				if (have3)
-					issue(a1, b1);
+				issue(a1, b1, lim1);
 				compute(buf);
-				if (!have2)
-					break;
-				la.store(As[buf ^ 1], a0, tid);
-				lb.store(Bs[buf ^ 1], b0, tid);
+				la.store(As[buf ^ 1], a0, tid, lim0);
+				lb.store(Bs[buf ^ 1], b0, tid, lim0);
 				__syncthreads();
 				buf ^= 1;
-				have1 = have3;
+			}
+			// One or two chunks remain; the second one is in set 1.
+			compute(buf);
+			if (c + 1 < nch) {
+				la.store(As[buf ^ 1], a1, tid, lim1);
+				lb.store(Bs[buf ^ 1], b1, tid, lim1);
+				__syncthreads();
+				compute(buf ^ 1);
 			}
 		}
 	}
@@ -501,29 +529,13 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 			lbh.load(vbh, kk0, K, tid);
 			lbl.load(vbl, kk0, K, tid);
 		};
-		auto store = [&](int b) {
-			lah.store(Ah[b], vah, tid);
-			lal.store(Al[b], val, tid);
-			lbh.store(Bh[b], vbh, tid);
-			lbl.store(Bl[b], vbl, tid);
+		auto store = [&](int b, int lim) {
+			lah.store(Ah[b], vah, tid, lim);
+			lal.store(Al[b], val, tid, lim);
+			lbh.store(Bh[b], vbh, tid, lim);
+			lbl.store(Bl[b], vbl, tid, lim);
 		};
-		setup(s);
-		load(k0, s.k);
-		store(0);
-		__syncthreads();
-		int buf = 0;
-		for (;;) {
-			int        nseg = seg;
-			int        nk0  = k0;
-			bool       newseg;
-			const bool hasNext = next_chunk<BK>(segs, tk.seg_end, s.k, nseg, nk0, newseg);
-			if (hasNext) {
-				if (newseg) {
-					s = segs[nseg];
-					setup(s);
-				}
-				load(nk0, s.k);
-			}
+		auto compute = [&](int b) {
 #pragma unroll
 			for (int st = 0; st < NST; ++st) {
 				// Step st covers k in [st * kK, st * kK + kK): lane group g the 8 at 8 g.
@@ -532,14 +544,14 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 #pragma unroll
 				for (int i = 0; i < FI; ++i) {
 					const int r = wm + Op::kM * i + lr;
-					ah[i]       = Ah[buf][r * VPR + LA::slot(r, kv)];
-					al[i]       = Al[buf][r * VPR + LA::slot(r, kv)];
+					ah[i]       = Ah[b][r * VPR + LA::slot(r, kv)];
+					al[i]       = Al[b][r * VPR + LA::slot(r, kv)];
 				}
 #pragma unroll
 				for (int j = 0; j < FJ; ++j) {
 					const int r = wn + Op::kM * j + lr;
-					bh[j]       = Bh[buf][r * VPR + LB::slot(r, kv)];
-					bl[j]       = Bl[buf][r * VPR + LB::slot(r, kv)];
+					bh[j]       = Bh[b][r * VPR + LB::slot(r, kv)];
+					bl[j]       = Bl[b][r * VPR + LB::slot(r, kv)];
 				}
 #pragma unroll
 				for (int i = 0; i < FI; ++i)
@@ -550,14 +562,29 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 						acc[i][j] = Op::mma(ah[i], bh[j], acc[i][j]);
 					}
 			}
-			if (!hasNext)
-				break;
-			store(buf ^ 1);
+		};
+		setup(s);
+		load(k0, s.k);
+		store(0, s.k - k0);
+		__syncthreads();
+		// Counted loop, left only at the top: the loads of chunk c+1 fly during the MFMAs of
+		// chunk c.
+		const int nch = chunk_count<BK>(segs, seg, tk.seg_end);
+		int       buf = 0;
+		for (int c = 0; c + 1 < nch; ++c) {
+			bool newseg;
+			next_chunk<BK>(segs, tk.seg_end, s.k, seg, k0, newseg);
+			if (newseg) {
+				s = segs[seg];
+				setup(s);
+			}
+			load(k0, s.k);
+			compute(buf);
+			store(buf ^ 1, s.k - k0);
 			__syncthreads();
 			buf ^= 1;
-			seg = nseg;
-			k0  = nk0;
 		}
+		compute(buf);
 	}
 
 	const bool to_y = tk.out_is_y != 0;
