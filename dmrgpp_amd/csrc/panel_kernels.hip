@@ -250,7 +250,7 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 // its stage-2 output in FP64 for the FP64 Lanczos recurrence). Workspace tasks (stage-1 panels,
 // split-k partials) store T and always overwrite.
 template <typename TO, int BM, int BN, int BK, int PF, typename Op>
-__global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
+__global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 ? 5 : 1)) void panel_gemm_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
     PBasesT<typename Op::T> bases, PStrides vs, TO* __restrict__ out_y, int64_t ystride,
     typename Op::T* __restrict__ out_ws, int64_t wsstride, int beta)
