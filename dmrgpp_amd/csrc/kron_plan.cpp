@@ -39,7 +39,7 @@ void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* task
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
                        const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
                        int64_t wsstride, int nvec, int beta, hipStream_t stream);
-void launch_panel_bf16x3(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
+void launch_panel_bf16x3(int bm, int bn, int variant, int depth, const PTask* tasks, const PSeg* segs,
                          const Tile* tiles, int ntiles, const void* const* bases_hi,
                          const void* const* bases_lo, const PStrides& vs, double* out_y,
                          int64_t ystride, float* out_f, void* out_h, void* out_l, int64_t wsstride,
@@ -1064,13 +1064,13 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		const void* hi[PBASE_COUNT] = { d_apack16_, d_bpack16_, xph, xth, bh };
 		const void* lo[PBASE_COUNT] = { d_apack16_ ? d_apack16_ + na : nullptr,
 			                        d_bpack16_ ? d_bpack16_ + nb : nullptr, xpl, xtl, bl };
-		launch_panel_bf16x3(st1_.bm, st1_.bn, v1, st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
+		launch_panel_bf16x3(st1_.bm, st1_.bn, v1, dp, st1_.d_tasks, st1_.d_segs, st1_.d_tiles,
 		                    int(st1_.tiles.size()), hi, lo, vs, vout, ldout, nullptr, bh, bl, bxs,
 		                    true, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
 		float* ws = static_cast<float*>(d_ws_);
-		launch_panel_bf16x3(st2_.bm, st2_.bn, v1, st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
+		launch_panel_bf16x3(st2_.bm, st2_.bn, v1, dp, st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
 		                    int(st2_.tiles.size()), hi, lo, vs, vout, ldout, ws, nullptr, nullptr,
 		                    wss, false, nvec, beta, stream_);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
@@ -1174,13 +1174,12 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 			}
 		try_set(c);
 		c.clear();
-		// Load pipeline depth (the BF16x3 kernel runs one chunk ahead).
-		if (precision_ != PREC_BF16X3)
-			for (int d : { 1, 2 }) {
-				Tuning t = best;
-				t.depth  = d;
-				c.push_back(t);
-			}
+		// Load pipeline depth.
+		for (int d : { 1, 2 }) {
+			Tuning t = best;
+			t.depth  = d;
+			c.push_back(t);
+		}
 		try_set(c);
 		c.clear();
 		for (int bm : { 64, 128 })

@@ -451,7 +451,8 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 ? 5 : 1
 // same layout, and a product is lo_a*hi_b + hi_a*lo_b + hi_a*hi_b on bf16 MFMAs accumulating in
 // FP32 (lo*lo, ~2^-16 relative, is dropped). PAIR: workspace tasks store a (hi, lo) pair
 // (stage-1 panels) instead of FP32 (split-k partials). Stage-2 output to the caller is FP64.
-template <int BM, int BN, int BK, bool PAIR, typename Op>
+// PF: chunks loaded ahead (1 or 2), as in panel_gemm_kernel.
+template <int BM, int BN, int BK, int PF, bool PAIR, typename Op>
 __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
     PBasesT<__bf16> hi, PBasesT<__bf16> lo, PStrides vs, double* __restrict__ out_y,
@@ -508,32 +509,52 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 			acc[i][j] = acc_t {};
 
 	int seg = tk.seg_begin;
-	int k0  = 0;
 	while (seg < tk.seg_end && segs[seg].k <= 0)
 		++seg;
 
 	if (seg < tk.seg_end) {
-		LA    lah, lal;
-		LB    lbh, lbl;
-		bf8_t vah[LA::NV], val[LA::NV], vbh[LB::NV], vbl[LB::NV];
-		PSeg  s     = segs[seg];
-		auto  setup = [&](const PSeg& sg) {
+		// One register set: the hi and lo vectors of both operands of one chunk, and the
+		// chunk's extent inside its segment.
+		struct Set {
+			bf8_t ah[LA::NV], al[LA::NV], bh[LB::NV], bl[LB::NV];
+			int   lim;
+		};
+		LA   lah, lal;
+		LB   lbh, lbl;
+		PSeg s      = segs[seg];
+		int  ck0    = 0;
+		int  lastk  = 0;
+		bool cvalid = true;
+		auto setup  = [&](const PSeg& sg) {
                         lah.setup(pick(hi, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
                         lal.setup(pick(lo, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
                         lbh.setup(pick(hi, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
                         lbl.setup(pick(lo, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
 		};
-		auto load = [&](int kk0, int K) {
-			lah.load(vah, kk0, K, tid);
-			lal.load(val, kk0, K, tid);
-			lbh.load(vbh, kk0, K, tid);
-			lbl.load(vbl, kk0, K, tid);
+		// Unconditional loads of the cursor's chunk (past the last chunk the last one again,
+		// never stored), then the cursor moves on.
+		auto issue = [&](Set& x) {
+			const int kk = cvalid ? ck0 : lastk;
+			lah.load(x.ah, kk, s.k, tid);
+			lal.load(x.al, kk, s.k, tid);
+			lbh.load(x.bh, kk, s.k, tid);
+			lbl.load(x.bl, kk, s.k, tid);
+			x.lim = s.k - kk;
+			lastk = kk;
+			if (cvalid) {
+				bool newseg;
+				cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
+				if (cvalid && newseg) {
+					s = segs[seg];
+					setup(s);
+				}
+			}
 		};
-		auto store = [&](int b, int lim) {
-			lah.store(Ah[b], vah, tid, lim);
-			lal.store(Al[b], val, tid, lim);
-			lbh.store(Bh[b], vbh, tid, lim);
-			lbl.store(Bl[b], vbl, tid, lim);
+		auto store = [&](int b, const Set& x) {
+			lah.store(Ah[b], x.ah, tid, x.lim);
+			lal.store(Al[b], x.al, tid, x.lim);
+			lbh.store(Bh[b], x.bh, tid, x.lim);
+			lbl.store(Bl[b], x.bl, tid, x.lim);
 		};
 		auto compute = [&](int b) {
 #pragma unroll
@@ -564,27 +585,48 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 			}
 		};
 		setup(s);
-		load(k0, s.k);
-		store(0, s.k - k0);
-		__syncthreads();
-		// Counted loop, left only at the top: the loads of chunk c+1 fly during the MFMAs of
-		// chunk c.
+		// Counted loops, left only at the top (see panel_gemm_kernel).
 		const int nch = chunk_count<BK>(segs, seg, tk.seg_end);
+		Set       x0;
 		int       buf = 0;
-		for (int c = 0; c + 1 < nch; ++c) {
-			bool newseg;
-			next_chunk<BK>(segs, tk.seg_end, s.k, seg, k0, newseg);
-			if (newseg) {
-				s = segs[seg];
-				setup(s);
-			}
-			load(k0, s.k);
-			compute(buf);
-			store(buf ^ 1, s.k - k0);
+		if constexpr (PF == 1) {
+			issue(x0);
+			store(0, x0);
 			__syncthreads();
-			buf ^= 1;
+			for (int c = 0; c + 1 < nch; ++c) {
+				issue(x0);
+				compute(buf);
+				store(buf ^ 1, x0);
+				__syncthreads();
+				buf ^= 1;
+			}
+			compute(buf);
+		} else {
+			Set x1;
+			issue(x0);
+			issue(x1);
+			store(0, x0);
+			__syncthreads();
+			int c = 0;
+			for (; c + 2 < nch; c += 2) {
+				issue(x0);
+				compute(buf);
+				store(buf ^ 1, x1);
+				__syncthreads();
+				buf ^= 1;
+				issue(x1);
+				compute(buf);
+				store(buf ^ 1, x0);
+				__syncthreads();
+				buf ^= 1;
+			}
+			compute(buf);
+			if (c + 1 < nch) {
+				store(buf ^ 1, x1);
+				__syncthreads();
+				compute(buf ^ 1);
+			}
 		}
-		compute(buf);
 	}
 
 	const bool to_y = tk.out_is_y != 0;
@@ -789,8 +831,10 @@ template void launch_panel_gemm<float, double>(int, int, int, int, const PTask*,
                                                const PStrides&, double*, int64_t, float*, int64_t,
                                                int, int, hipStream_t);
 
-// variant 1 selects the 32x32x16 bf16 MFMA form.
-void launch_panel_bf16x3(int bm, int bn, int variant, const PTask* tasks, const PSeg* segs,
+// variant 1 selects the 32x32x16 bf16 MFMA form; depth 2 loads two chunks ahead (32x32x16
+// form only, the 16x16x32 form runs one ahead).
+void launch_panel_bf16x3(int bm, int bn, int variant, int depth, const PTask* tasks,
+                         const PSeg* segs,
                          const Tile* tiles, int ntiles, const void* const* bases_hi,
                          const void* const* bases_lo, const PStrides& vs, double* out_y,
                          int64_t ystride, float* out_f, void* out_h, void* out_l, int64_t wsstride,
@@ -806,10 +850,17 @@ void launch_panel_bf16x3(int bm, int bn, int variant, const PTask* tasks, const 
 	__bf16*    oh = static_cast<__bf16*>(out_h);
 	__bf16*    ol = static_cast<__bf16*>(out_l);
 	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
-#define DMK_BLAUNCH(BM, BN, PAIR, OP)                                                              \
-	hipLaunchKernelGGL((panel_bf16x3_kernel<BM, BN, 32, PAIR, OP>), grid, dim3(kPThreads), 0,  \
-	                   stream, tasks, segs, tiles, h, l, vs, out_y, ystride, out_f, oh, ol,      \
+#define DMK_BLAUNCH_PF(BM, BN, PF, PAIR, OP)                                                       \
+	hipLaunchKernelGGL((panel_bf16x3_kernel<BM, BN, 32, PF, PAIR, OP>), grid, dim3(kPThreads),  \
+	                   0, stream, tasks, segs, tiles, h, l, vs, out_y, ystride, out_f, oh, ol,   \
 	                   wsstride, beta)
+#define DMK_BLAUNCH(BM, BN, PAIR, OP)                                                              \
+	do {                                                                                       \
+		if (depth == 2 && OP::kM == 32)                                                    \
+			DMK_BLAUNCH_PF(BM, BN, 2, PAIR, OpBf16x32);                                \
+		else                                                                               \
+			DMK_BLAUNCH_PF(BM, BN, 1, PAIR, OP);                                       \
+	} while (0)
 #define DMK_BSHAPES(PAIR, OP)                                                                      \
 	if (bm == 64 && bn == 64)                                                                  \
 		DMK_BLAUNCH(64, 64, PAIR, OP);                                                     \
@@ -836,6 +887,7 @@ void launch_panel_bf16x3(int bm, int bn, int variant, const PTask* tasks, const 
 	}
 #undef DMK_BSHAPES
 #undef DMK_BLAUNCH
+#undef DMK_BLAUNCH_PF
 }
 
 template <typename E>
