@@ -88,7 +88,8 @@ typedef struct dmk_tuning {
 	int32_t splitk;
 	int32_t xcd_order;
 	int32_t mfma32;            /* 1 = 32x32 MFMA blocks (FP32: 32x32x2, BF16x3: 32x32x16), 0 = 16x16 */
-	int32_t prefetch;          /* 1 or 2 K chunks ahead (0 = 1) */
+	int32_t prefetch;          /* 1 or 2 K chunks ahead (0 = 1); BF16x3 honours 2 in the 32x32x16 form
+	                              only */
 } dmk_tuning;
 
 /* Replaces BatchedGemm<double>::BatchedGemm(noperator, npatches, left_patch_size,
