@@ -239,26 +239,37 @@ def main():
         steps_per_sweep = 2 * (lattice.nsites - 3)
         # Every step of these segments starts from the transformed wave function; the mixed
         # modes run those solves in their own precision here (mixed_warm), to time them.
+        # The truncation diagonalises density-matrix blocks of 512 rows or more on the GPU; the
+        # "fp64 host-trunc" segment keeps every block on the host LAPACK for comparison.
         mk = lambda p, r: sweep.GpuSolver(precision=p, refine=r, mixed_warm=True)  # noqa: E731
-        for mode, solver in (("fp64", sweep.GpuSolver(precision="fp64")),
-                             ("fp32+refine", mk("fp32", 1)),
-                             ("bf16x3+refine", mk("bf16x3", 1)),
-                             ("fp32+davidson", mk("fp32", 2)),
-                             ("bf16x3+davidson", mk("bf16x3", 2))):
+        gpu_eig = sweep.GpuDensityEig(512)
+        for mode, solver, eig in (("fp64", sweep.GpuSolver(precision="fp64"), gpu_eig),
+                                  ("fp64 host-trunc", sweep.GpuSolver(precision="fp64"), None),
+                                  ("fp32+refine", mk("fp32", 1), gpu_eig),
+                                  ("bf16x3+refine", mk("bf16x3", 1), gpu_eig),
+                                  ("fp32+davidson", mk("fp32", 2), gpu_eig),
+                                  ("bf16x3+davidson", mk("bf16x3", 2), gpu_eig)):
             st = sweep.clone_state(state)
             t = time.perf_counter()
             e, recs = sweep.run(model, lattice, lambda N: (0,), M, [M], solver, log=log,
-                                state=st, resume=True, stop_after=args.sweep_steps, wft=True)
+                                state=st, resume=True, stop_after=args.sweep_steps, wft=True,
+                                eig=eig)
             dt = time.perf_counter() - t
             per = dt / max(1, len(recs))
             lz_ms = sum(r.ms_lanczos for r in recs) / max(1, len(recs))
+            tr_ms = sum(r.info["timing"]["truncate_ms"] for r in recs) / max(1, len(recs))
+            parts = {k: round(sum(r.info["timing"].get("truncate_" + k, 0.0) for r in recs)
+                              / max(1, len(recs)), 1) for k in ("rho_ms", "eig_ms", "basis_ms")}
             sweep_out[mode] = {"steps": len(recs), "s_per_step": round(per, 4),
                                "eigensolve_ms_per_step": round(lz_ms, 1),
+                               "truncate_ms_per_step": round(tr_ms, 1),
+                               "truncate_parts_ms_per_step": parts,
                                "sweep_s_est": round(per * steps_per_sweep, 2),
                                "lanczos_matvecs": int(sum(r.matvecs for r in recs)),
                                "energy_last": e}
             log(f"sweep segment {mode}: {len(recs)} steps, {per:.3f} s/step "
-                f"(eigensolve {lz_ms:.1f} ms/step), full sweep ~{per * steps_per_sweep:.1f} s")
+                f"(eigensolve {lz_ms:.1f} ms/step, truncation {tr_ms:.1f} ms/step {parts}), "
+                f"full sweep ~{per * steps_per_sweep:.1f} s")
         for mode in ("fp32+refine", "bf16x3+refine", "fp32+davidson", "bf16x3+davidson"):
             sweep_out[mode]["eigensolve_speedup_vs_fp64"] = round(
                 sweep_out["fp64"]["eigensolve_ms_per_step"]

@@ -171,6 +171,7 @@ SIGNATURES = {
     "dmk_krylov_evolve_z": (_I, [_VP, _I, _VP, _SZ, _I, _VP, _VP, _SZ,
                                  ctypes.POINTER(KrylovParams), _VP, ctypes.POINTER(KrylovStats)]),
     "dmk_diagonal_d": (_I, [_VP, _VP]),
+    "dmk_density_diag_d": (_I, [_I, _VP, _VP, _VP]),
     "dmk_mlp_create": (_I, [ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_VP)]),
     "dmk_mlp_destroy": (None, [_VP]),
     "dmk_mlp_get_params": (_I, [_VP, _VP]),
@@ -248,6 +249,26 @@ def tridiag_eigen(a, b):
     _check(lib().dmk_tridiag_eigen(m, a.ctypes.data, b.ctypes.data, ev.ctypes.data,
                                    vec.ctypes.data), "dmk_tridiag_eigen")
     return ev, vec.reshape((m, m), order="F")
+
+
+def density_diag(blocks):
+    """Eigenpairs of symmetric blocks on the GPU (dmk_density_diag_d, the truncation's
+    DiagBlockDiagMatrix::diagonalise): returns [(w ascending, v with eigenvectors in columns)]."""
+    mats = [np.array(b, dtype=np.float64, order="F", copy=True) for b in blocks]
+    for a in mats:
+        assert a.ndim == 2 and a.shape[0] == a.shape[1]
+    nb = len(mats)
+    sizes = (ctypes.c_int64 * max(nb, 1))(*[a.shape[0] for a in mats])
+    ptrs = (ctypes.c_void_p * max(nb, 1))(*[a.ctypes.data for a in mats])
+    eigs = np.zeros(max(1, sum(a.shape[0] for a in mats)))
+    _check(lib().dmk_density_diag_d(nb, ctypes.addressof(sizes), ctypes.addressof(ptrs),
+                                    eigs.ctypes.data), "dmk_density_diag_d")
+    out, off = [], 0
+    for a in mats:
+        k = a.shape[0]
+        out.append((eigs[off:off + k].copy(), a))
+        off += k
+    return out
 
 
 def gemm_vbatch(transa: str, transb: str, groups, alpha=1.0, beta=0.0, cplx: bool = False):

@@ -42,6 +42,7 @@ struct dmk_handle {
 
 namespace dmk {
 
+void density_diag(int nblocks, const int64_t* n, double* const* blocks, double* eigs);
 void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
 void launch_sector_gather(const double* v, const int64_t* map, int64_t n, double* y, hipStream_t s);
 void launch_sector_scatter_add(const double* x, const int64_t* map, int64_t n, double* v,
@@ -549,6 +550,14 @@ int dmk_gemm_vbatch_z(char transa, char transb, const int* m_array, const int* n
 		dmk::gemm_vbatch(true, transa, transb, m_array, n_array, k_array, alpha, a_array,
 		                 lda_array, b_array, ldb_array, beta, c_array, ldc_array, group_count,
 		                 group_size);
+		return DMK_OK;
+	});
+}
+
+int dmk_density_diag_d(int nblocks, const int64_t* n, double* const* blocks, double* eigs)
+{
+	return guarded([&] {
+		dmk::density_diag(nblocks, n, blocks, eigs);
 		return DMK_OK;
 	});
 }

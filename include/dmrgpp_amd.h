@@ -278,6 +278,18 @@ int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t 
                         const dmk_krylov_params* params, dmk_krylov_result* results,
                         dmk_krylov_stats* stats);
 
+/* Eigen-decomposition of a block-diagonal symmetric matrix on the GPU: replaces
+ * DiagBlockDiagMatrix::diagonalise (DiagBlockDiagMatrix.h:73-88), i.e. PsimagLite::diag(block,
+ * eigs, 'V') on every symmetry block (BlockDiagonalMatrix.h:317-322), which the truncation calls
+ * on the reduced density matrix (DensityMatrixLocal.h:201-204). Block b is n[b] x n[b], column
+ * major, in host memory at blocks[b] with leading dimension n[b] (0 <= n[b] <= 32768); only its
+ * upper triangle is read. It is overwritten by its orthonormal eigenvectors (column j belongs to
+ * eigenvalue j), and its eigenvalues, ascending, go to eigs[o_b .. o_b + n[b]) with
+ * o_b = n[0] + ... + n[b-1]. rocSOLVER dsyevd on the device; synchronous. The reference's
+ * EnforcePhase (EnforcePhase.h:23-39) multiplies each eigenvector by the sign of a norm, which is
+ * +1 for real entries, so no sign convention is imposed here either. */
+int dmk_density_diag_d(int nblocks, const int64_t* n, double* const* blocks, double* eigs);
+
 /* Variable-size batched GEMM: replaces dmrg_Xgemm_vbatch<T> (GPUPlugin/dmrg_vbatch.h:118-133,
  * dmrg_vbatch.cpp:129-652), the free function the reference plugin and its test_vbatch driver
  * call. For every entry e of every group g: C_e = alpha op(A_e) op(B_e) + beta C_e, column major,

@@ -1,0 +1,81 @@
+"""The truncation's density-matrix eigensolver on the GPU (dmk_density_diag_d, the replacement of
+DiagBlockDiagMatrix::diagonalise): eigenpairs against the host LAPACK, degenerate spectra, and a
+sweep whose truncation runs on the GPU against the same sweep on the host."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import dmrgpp_amd as dm
+from dmrgpp_amd import configs
+
+
+def _density_like(n, rank, seed):
+    # rho = X^T X of a random R x n segment: symmetric, positive semidefinite, rank min(R, n)
+    # (zero eigenvalues repeated when R < n), scaled to unit trace as a density matrix is.
+    x = np.random.default_rng(seed).standard_normal((rank, n))
+    rho = x.T @ x
+    return rho / np.trace(rho)
+
+
+def test_density_diag_rejects_bad_arguments():
+    # Argument checks run before any device call.
+    lib = dm.lib()
+    sizes = (ctypes.c_int64 * 1)(-3)
+    ptrs = (ctypes.c_void_p * 1)(None)
+    eigs = (ctypes.c_double * 1)()
+    rc = lib.dmk_density_diag_d(1, ctypes.addressof(sizes), ctypes.addressof(ptrs),
+                                ctypes.addressof(eigs))
+    assert rc != 0 and b"out of range" in lib.dmk_last_error()
+    assert lib.dmk_density_diag_d(-1, None, None, None) != 0
+    # No blocks, or only empty ones: nothing to do.
+    assert lib.dmk_density_diag_d(0, None, None, None) == 0
+    sizes[0] = 0
+    assert lib.dmk_density_diag_d(1, ctypes.addressof(sizes), ctypes.addressof(ptrs),
+                                  ctypes.addressof(eigs)) == 0
+
+
+@pytest.mark.gpu
+def test_gpu_density_diag_matches_lapack():
+    # Sizes from 1 to 700, full-rank and rank-deficient blocks (a zero eigenvalue repeated
+    # hundreds of times), in one call.
+    specs = [(1, 1), (5, 5), (64, 64), (300, 40), (700, 700), (513, 100)]
+    blocks = [_density_like(n, r, seed=i) for i, (n, r) in enumerate(specs)]
+    out = dm.density_diag(blocks)
+    for a, (w, v) in zip(blocks, out):
+        n = a.shape[0]
+        want = np.linalg.eigvalsh(a)
+        assert np.all(np.diff(w) >= -1e-15)                      # ascending
+        assert np.max(np.abs(w - want)) < 1e-13                  # unit trace: absolute = relative
+        assert np.linalg.norm(a @ v - v * w) < 1e-12
+        assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-11 * np.sqrt(n)
+
+
+@pytest.mark.gpu
+def test_gpu_density_diag_degenerate_spectrum():
+    # Exactly degenerate eigenvalues (a Kronecker structure gives each one several times): the
+    # eigenspace projectors agree with LAPACK's even though the vectors inside them may not.
+    d = np.diag([0.4, 0.3, 0.2, 0.1])
+    q, _ = np.linalg.qr(np.random.default_rng(3).standard_normal((16, 16)))
+    a = q @ np.kron(d, d) @ q.T
+    (w, v), = dm.density_diag([a])
+    w0, v0 = np.linalg.eigh(a)
+    assert np.max(np.abs(w - w0)) < 1e-14
+    for lam in np.unique(np.round(w0, 12)):
+        s = np.abs(w0 - lam) < 1e-10
+        assert np.linalg.norm(v[:, s] @ v[:, s].T - v0[:, s] @ v0[:, s].T) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_truncation_on_gpu_matches_host():
+    # A Hubbard chain sweep with every density-matrix block on the GPU gives the host-truncation
+    # energies and discarded weights.
+    runs = {}
+    for key, min_rows in (("host", 0), ("gpu", 1)):
+        e, recs, _ = configs.run_config("hubbard-chain-8-U0", log=lambda s: None,
+                                        gpu_trunc_min=min_rows)
+        runs[key] = (e, [r.energy for r in recs], [r.discarded for r in recs])
+    (eh, sh, dh), (eg, sg, dg) = runs["host"], runs["gpu"]
+    assert abs(eh - eg) < 1e-10 * abs(eh)
+    assert np.max(np.abs(np.array(sh) - np.array(sg))) < 1e-9
+    assert np.max(np.abs(np.array(dh) - np.array(dg))) < 1e-10
