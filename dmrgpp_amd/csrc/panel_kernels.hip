@@ -21,8 +21,11 @@
 //
 // Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a BM x BN output tile and
 // each wave a (BM/2) x (BN/2) sub-tile of MFMA blocks. K is walked in chunks of BK staged
-// through LDS, double buffered: the global loads of chunk c+1 are issued before the MFMAs of
-// chunk c. LDS rows hold the BK elements of one operand row as 16-byte slots, XOR-swizzled by
+// through LDS, double buffered: the global loads of chunk c+1 (PF = 1), or of c+1 and c+2 into
+// two register sets (PF = 2), are in flight during the MFMAs of chunk c. The loads carry no
+// branches and the chunk loops are counted and leave only at the top, so that the compiler can
+// count the loads in flight and wait only for the register set an LDS store needs.
+// LDS rows hold the BK elements of one operand row as 16-byte slots, XOR-swizzled by
 // row (slot kv of row r at kv ^ swz(r)) so that the 16 rows one 16-lane pass reads fall on
 // distinct banks, without padding.
 // Within a chunk the lanes' k order is permuted (lane group g takes k = g * NF + s at MFMA step
