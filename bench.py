@@ -59,7 +59,18 @@ def shape_key(t):
     return (t["stage2_tile_m"], t["stage2_tile_n"], t["splitk"], t["xcd_order"])
 
 
-def cpu_baseline(prob, x, budget_s=15.0, max_calls=20):
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prob, x, budget_s=12.0, max_calls=200):
     from oracle import kron_numpy
 
     try:
@@ -79,7 +90,7 @@ def cpu_baseline(prob, x, budget_s=15.0, max_calls=20):
             break
     dt = time.perf_counter() - t0
     return {"value": round(calls / dt, 4), "unit": "matvecs/s", "cores": int(threads),
-            "kind": "port",
+            "kind": "port", "host": f"{host_cpu_model()}, {os.cpu_count()} logical CPUs",
             "sample": f"{calls} matvecs of the same m=2048 ladder superblock (KronConnections "
                       f"restatement, numpy BLAS GEMMs, {threads} BLAS threads), {dt:.1f} s"}
 
