@@ -807,11 +807,16 @@ std::string Plan::histogram_key() const
 
 void Plan::retune(const Tuning& t)
 {
-	auto valid = [](int bm, int bn) {
-		return (bm == 64 || bm == 128) && (bn == 64 || bn == 128);
+	// 64 / 128 in every mode; FP64 and FP32 also take 96 x 96, 96 x 64 and 64 x 96.
+	const bool allow96 = precision_ != PREC_BF16X3;
+	auto valid = [&](int bm, int bn) {
+		if ((bm == 64 || bm == 128) && (bn == 64 || bn == 128))
+			return true;
+		return allow96 && ((bm == 96 && (bn == 96 || bn == 64)) || (bm == 64 && bn == 96));
 	};
 	if (!valid(t.bm1, t.bn1) || !valid(t.bm2, t.bn2) || t.splitk < 0)
-		throw std::invalid_argument("dmk tuning: tile shapes must be 64 or 128, splitk >= 0");
+		throw std::invalid_argument("dmk tuning: tile shapes must be 64 or 128 (FP64 / FP32 "
+		                            "also 96x96, 96x64, 64x96), splitk >= 0");
 	if (t.mfma32 != 0 && t.mfma32 != 1)
 		throw std::invalid_argument("dmk tuning: mfma32 must be 0 or 1");
 	if (t.depth != 1 && t.depth != 2)
@@ -1163,6 +1168,12 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 			}
 		}
 	};
+	std::vector<std::pair<int, int>> tile_shapes { { 64, 64 }, { 128, 64 }, { 64, 128 },
+		                                       { 128, 128 } };
+	if (precision_ != PREC_BF16X3)
+		for (const auto& sh : { std::make_pair(96, 96), std::make_pair(96, 64),
+		                        std::make_pair(64, 96) })
+			tile_shapes.push_back(sh);
 	for (int pass = 0; pass < 2; ++pass) {
 		std::vector<Tuning> c;
 		// MFMA form (FP32 16x16x4 or 32x32x2, BF16x3 16x16x32 or 32x32x16).
@@ -1182,13 +1193,12 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		}
 		try_set(c);
 		c.clear();
-		for (int bm : { 64, 128 })
-			for (int bn : { 64, 128 }) {
-				Tuning t = best;
-				t.bm2    = bm;
-				t.bn2    = bn;
-				c.push_back(t);
-			}
+		for (const auto& sh : tile_shapes) {
+			Tuning t = best;
+			t.bm2    = sh.first;
+			t.bn2    = sh.second;
+			c.push_back(t);
+		}
 		try_set(c);
 		c.clear();
 		for (int sk : { 0, 512, 768, 1024, 1280, 1536, 1792, 2048, 2560, 3072, 4096 }) {
@@ -1205,13 +1215,12 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		}
 		try_set(c);
 		c.clear();
-		for (int bm : { 64, 128 })
-			for (int bn : { 64, 128 }) {
-				Tuning t = best;
-				t.bm1    = bm;
-				t.bn1    = bn;
-				c.push_back(t);
-			}
+		for (const auto& sh : tile_shapes) {
+			Tuning t = best;
+			t.bm1    = sh.first;
+			t.bn1    = sh.second;
+			c.push_back(t);
+		}
 		try_set(c);
 	}
 	retune(best);

@@ -808,15 +808,34 @@ void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* task
 		DMK_PLAUNCH(128, 128, BK, OP);                                                     \
 	else                                                                                       \
 		throw std::runtime_error("panel gemm: unsupported tile shape");
+	// 96-wide tiles (48-wide wave tiles, so 16x16 MFMA blocks in either MFMA setting): they fit
+	// patch sizes just above a multiple of 64, such as 1040, with less edge waste.
+#define DMK_PSHAPES96(BK, OP)                                                                      \
+	if (bm == 96 && bn == 96)                                                                  \
+		DMK_PLAUNCH(96, 96, BK, OP);                                                       \
+	else if (bm == 96 && bn == 64)                                                             \
+		DMK_PLAUNCH(96, 64, BK, OP);                                                       \
+	else if (bm == 64 && bn == 96)                                                             \
+		DMK_PLAUNCH(64, 96, BK, OP);                                                       \
+	else                                                                                       \
+		throw std::runtime_error("panel gemm: unsupported tile shape");
+	const bool s96 = bm == 96 || bn == 96;
 	if constexpr (std::is_same<T, double>::value) {
-		DMK_PSHAPES(16, OpF64)
+		if (s96) {
+			DMK_PSHAPES96(16, OpF64)
+		} else {
+			DMK_PSHAPES(16, OpF64)
+		}
 	} else {
-		if (variant == 1) {
+		if (s96) {
+			DMK_PSHAPES96(32, OpF32)
+		} else if (variant == 1) {
 			DMK_PSHAPES(32, OpF32x32)
 		} else {
 			DMK_PSHAPES(32, OpF32)
 		}
 	}
+#undef DMK_PSHAPES96
 #undef DMK_PSHAPES
 #undef DMK_PLAUNCH
 }

@@ -220,6 +220,13 @@ TUNINGS = [
          xcd_order=1, mfma32=0, prefetch=2),
     dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=64,
          xcd_order=0, mfma32=1, prefetch=2),
+    # 96-wide tiles (FP64 / FP32; BF16x3 rejects them), in both MFMA settings.
+    dict(stage1_tile_m=96, stage1_tile_n=64, stage2_tile_m=96, stage2_tile_n=96, splitk=32,
+         xcd_order=1, mfma32=1, prefetch=2),
+    dict(stage1_tile_m=64, stage1_tile_n=96, stage2_tile_m=64, stage2_tile_n=96, splitk=0,
+         xcd_order=0, mfma32=0, prefetch=1),
+    dict(stage1_tile_m=96, stage1_tile_n=96, stage2_tile_m=96, stage2_tile_n=64, splitk=48,
+         xcd_order=1, mfma32=0, prefetch=2),
 ]
 
 
@@ -230,6 +237,14 @@ def test_gpu_tunings_match_oracle(tuning, prec, tol):
     # Every tile shape, forced split-k (pieces of 16-48 along K, ragged last piece), both
     # workgroup orders and both FP32 MFMA forms give the oracle's result; the same tuning twice
     # is bitwise identical.
+    uses96 = 96 in (tuning["stage1_tile_m"], tuning["stage1_tile_n"], tuning["stage2_tile_m"],
+                    tuning["stage2_tile_n"])
+    if prec == "bf16x3" and uses96:
+        op = dm.BatchedGemm(problems.random_problem([5, 7], [6, 3], 2, seed=1), prec)
+        with pytest.raises(dm.DmkError):
+            op.set_tuning(**tuning)
+        op.close()
+        return
     left = {-2: 70, 0: 130, 2: 70}
     prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(2), seed=21)
     rand = problems.random_problem([3, 70, 5, 130], [65, 2, 64, 9], 3, density=0.7, seed=2)
