@@ -77,9 +77,10 @@ typedef struct dmk_info {
 	double  raw_bytes;         /*   bytes uploaded */
 } dmk_info;
 
-/* Launch shape of a handle: output tile (64 or 128 each way) of the stage-1 and stage-2
- * grouped-GEMM launches, the split-k granularity of stage 2 (target contraction length per
- * piece, 0 = never split; pieces are summed in a fixed order, so results stay deterministic),
+/* Launch shape of a handle: output tile (64 or 128 each way; FP64 and FP32 also 96 x 96,
+ * 96 x 64 and 64 x 96) of the stage-1 and stage-2 grouped-GEMM launches, the split-k
+ * granularity of stage 2 (target contraction length per piece, 0 = never split; pieces are
+ * summed in a fixed order, so results stay deterministic),
  * the workgroup order (0 = by decreasing task cost, 1 = XCD-aware regions), the MFMA block form
  * and the load pipeline depth (global loads issued 1 or 2 K chunks ahead; 0 means 1). */
 typedef struct dmk_tuning {
