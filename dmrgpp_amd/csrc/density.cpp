@@ -107,7 +107,9 @@ void density_diag(int nblocks, const int64_t* n, double* const* blocks, double* 
 			off += m;
 		}
 	} catch (...) {
-		hipStreamSynchronize(ctx.stream);
+		// Drain the stream before the buffer goes back to the pool; the first error is the
+		// one reported.
+		(void)hipStreamSynchronize(ctx.stream);
 		pool_device_free(buf);
 		throw;
 	}
