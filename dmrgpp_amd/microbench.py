@@ -8,9 +8,13 @@
   (iii) a replay of a saved superblock (bench.py --save-problem), e.g. the m=2048 ladder.
 
   python -m dmrgpp_amd.microbench [--replay F.npz [--load-tuning T.json]] [--iters 30]
+  python -m dmrgpp_amd.microbench --replay F.npz --multi 1,2,4,8 [--iters 12]
+
 
 Each case reports the average matvec time from HIP events (one warm-up, then iters calls) and
-the algorithmic TFLOP/s in FP64, FP32 and BF16x3.
+the algorithmic TFLOP/s in FP64, FP32 and BF16x3. With --multi, the replayed superblock is
+multiplied into several vectors per launch (dmk_apply_multi_d, the batched form of the Krylov
+time step) for each listed count, in FP64 and FP32, timed by wall clock around iters calls.
 """
 from __future__ import annotations
 
@@ -44,6 +48,33 @@ def run_case(name, prob, x, iters, log, tunings=None):
     return out
 
 
+def run_multi(prob, counts, iters, log):
+    import time
+
+    from . import BatchedGemm, DeviceArray
+
+    out = []
+    for prec in ("fp64", "fp32"):
+        op = BatchedGemm(prob, prec)
+        for nvec in counts:
+            X = np.random.default_rng(3).uniform(-1, 1, nvec * prob.n)
+            dX, dY = DeviceArray.from_host(X), DeviceArray(nvec * prob.n)
+            op.apply_multi(dX, dY, nvec)
+            op.sync()
+            t = time.perf_counter()
+            for _ in range(iters):
+                op.apply_multi(dX, dY, nvec)
+            op.sync()
+            ms = (time.perf_counter() - t) / iters * 1e3
+            out.append({"prec": prec, "nvec": nvec, "ms": round(ms, 3),
+                        "ms_per_vec": round(ms / nvec, 3)})
+            log(f"multi {prec:5s} nvec={nvec}: {ms:8.3f} ms, {ms / nvec:7.3f} ms per vector")
+            dX.free()
+            dY.free()
+        op.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--replay", default=None, help="superblock .npz written by bench.py")
@@ -51,9 +82,21 @@ def main(argv=None):
     ap.add_argument("--load-tuning", default=None,
                     help="per-precision tunings (JSON written by bench.py --save-tuning) for the "
                          "replayed superblock")
+    ap.add_argument("--multi", default=None,
+                    help="comma-separated vector counts: time dmk_apply_multi_d on the replayed "
+                         "superblock instead of the single-vector cases")
     a = ap.parse_args(argv)
     tunings = json.load(open(a.load_tuning)) if a.load_tuning else None
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    if a.multi:
+        from .problem import load_npz
+
+        if not a.replay:
+            ap.error("--multi needs --replay")
+        counts = [int(c) for c in a.multi.split(",")]
+        print(json.dumps(run_multi(load_npz(a.replay, name="superblock"), counts, a.iters, log)),
+              flush=True)
+        return 0
     res = []
     p1 = problems.block_diagonal_problem()
     res.append(run_case("bench_BatchedGemm-40-patches", p1,
