@@ -69,7 +69,7 @@ class LanczosParams(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("refine", ctypes.c_int32),
                 ("refine_min_steps", ctypes.c_int32), ("refine_tolerance", ctypes.c_double),
                 ("graph", ctypes.c_int32), ("pad", ctypes.c_int32),
-                ("stop_residual", ctypes.c_double)]
+                ("stop_residual", ctypes.c_double), ("switch_noise", ctypes.c_double)]
 
 
 class LanczosResult(ctypes.Structure):

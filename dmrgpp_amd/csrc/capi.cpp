@@ -397,6 +397,7 @@ int dmk_lanczos_default_params(dmk_lanczos_params* params)
 		params->refine_tolerance = d.refine_tolerance;
 		params->graph            = d.graph;
 		params->stop_residual    = d.stop_residual;
+		params->switch_noise     = d.switch_noise;
 		return DMK_OK;
 	});
 }
@@ -419,6 +420,9 @@ int dmk_lanczos_ground_d(dmk_handle* handle, const double* init, double* z,
 		lp.refine_tolerance = params->refine_tolerance;
 		lp.graph            = params->graph;
 		lp.stop_residual    = params->stop_residual;
+		lp.switch_noise     = params->switch_noise;
+		if (!(lp.switch_noise >= 0))
+			throw std::invalid_argument("dmk_lanczos_ground_d: switch_noise must be >= 0");
 		if (!handle->lanczos)
 			handle->lanczos = std::make_unique<dmk::LanczosWorkspace>();
 		dmk::LanczosReport r = dmk::lanczos_ground(p, *handle->lanczos, init, z, lp);

@@ -322,9 +322,16 @@ LanczosReport lanczos_ground(Plan& plan, LanczosWorkspace& ws, const double* ini
 	LanczosReport r;
 	try {
 		plan.set_precision(p.precision);
-		// Relative matvec noise of the reduced-precision modes (about 16 units of roundoff of the
-		// FP32 accumulation; the (hi, lo) bf16 pair representation for BF16x3).
-		const double noise = (p.precision == PREC_FP32) ? 1e-6 : (p.precision == PREC_BF16X3) ? 1e-4 : 0.0;
+		// Hand-over level of the reduced-precision first run, relative to ||H||: near the
+		// matvec noise (about 16 units of roundoff of the FP32 accumulation; the (hi, lo) bf16
+		// pair representation for BF16x3) before an FP64 Lanczos restart, which starts from
+		// scratch; earlier (3e-5 for FP32) before the FP64 Davidson refinement, which gains
+		// the remaining digits in a few matvecs (ladder superblock at m = 2048: FP32 70 -> 66
+		// ms, the restart slower with any larger value, BF16x3 flat between 3e-5 and 1e-3).
+		const double noise = p.precision == PREC_FP64 ? 0.0
+		                     : p.switch_noise > 0     ? p.switch_noise
+		                     : p.precision == PREC_FP32 ? (p.refine == 2 ? 3e-5 : 1e-6)
+		                                                : 1e-4;
 		const bool mixed_first = (p.precision != PREC_FP64 && p.refine);
 		Decomp d = decompose(plan, ws, init, p.max_steps, p.min_steps, p.tolerance,
 		                     (p.refine ? noise : 0.0), mixed_first ? 0.0 : p.stop_residual,

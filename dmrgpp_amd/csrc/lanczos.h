@@ -18,6 +18,7 @@ struct LanczosParams {
 	double refine_tolerance = 1e-12;
 	int    graph            = 1;     // capture each step in a HIP graph (not with reortho)
 	double stop_residual    = 0;     // > 0: stop once |beta_{j+1} s_j| < stop_residual
+	double switch_noise     = 0;     // > 0: hand-over level of a refined mixed first run
 };
 
 struct LanczosReport {

@@ -195,6 +195,10 @@ typedef struct dmk_lanczos_params {
 	int32_t pad;
 	double  stop_residual;    /* > 0: also stop once the Ritz residual estimate |beta_{j+1} s_j|
 	                             of the unit lowest Ritz vector drops below it (0 = off) */
+	double  switch_noise;     /* a refined non-FP64 first run hands over to the FP64 refinement
+	                             once its residual estimate drops below switch_noise * ||H||
+	                             (0 = the mode's default: FP32 3e-5 before the Davidson
+	                             refinement, 1e-6 before the Lanczos restart; BF16x3 1e-4) */
 } dmk_lanczos_params;
 
 typedef struct dmk_lanczos_result {

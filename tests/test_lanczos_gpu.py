@@ -108,6 +108,22 @@ def test_gpu_lanczos_mixed_with_davidson_refinement(precision):
 
 
 @pytest.mark.gpu
+def test_gpu_lanczos_switch_noise_moves_the_hand_over():
+    # A larger hand-over level stops the FP32 first run earlier (no more first-run matvecs) and
+    # the FP64 Davidson refinement still reaches the FP64 energy; a negative level is rejected.
+    prob = _qn(3, {-4: 10, -2: 60, 0: 100, 2: 60, 4: 10})
+    init = np.random.default_rng(1234).uniform(-1, 1, prob.n)
+    r64, _ = gpu_lanczos(prob, init)
+    lo, _ = gpu_lanczos(prob, init, precision="fp32", refine=2, switch_noise=1e-7)
+    hi, _ = gpu_lanczos(prob, init, precision="fp32", refine=2, switch_noise=1e-4)
+    assert hi["matvecs_first"] <= lo["matvecs_first"]
+    for r in (lo, hi):
+        assert abs(r["energy"] - r64["energy"]) < 1e-10 * abs(r64["energy"]), (r, r64)
+    with pytest.raises(dm.DmkError):
+        gpu_lanczos(prob, init, precision="fp32", refine=2, switch_noise=-1.0)
+
+
+@pytest.mark.gpu
 def test_gpu_lanczos_reortho():
     prob = _qn(2, {-2: 30, 0: 50, 2: 30})
     init = np.random.default_rng(1234).uniform(-1, 1, prob.n)
