@@ -254,7 +254,9 @@ def main():
         # "fp64 host-trunc" segment keeps every block on the host LAPACK for comparison.
         mk = lambda p, r: sweep.GpuSolver(precision=p, refine=r, mixed_warm=True)  # noqa: E731
         gpu_eig = sweep.GpuDensityEig(512)
-        dm.density_diag([np.eye(4)])  # rocSOLVER's first call loads its kernels: not timed
+        # rocSOLVER loads its kernels on the first call of each size class: not timed.
+        w = np.random.default_rng(0).standard_normal((640, 640))
+        dm.density_diag([w + w.T, np.eye(64) + 0.5])
         for mode, solver, eig in (("fp64", sweep.GpuSolver(precision="fp64"), gpu_eig),
                                   ("fp64 host-trunc", sweep.GpuSolver(precision="fp64"), None),
                                   ("fp32+refine", mk("fp32", 1), gpu_eig),
