@@ -250,13 +250,15 @@ def main():
         steps_per_sweep = 2 * (lattice.nsites - 3)
         # Every step of these segments starts from the transformed wave function; the mixed
         # modes run those solves in their own precision here (mixed_warm), to time them.
-        # The truncation diagonalises density-matrix blocks of 512 rows or more on the GPU; the
-        # "fp64 host-trunc" segment keeps every block on the host LAPACK for comparison.
+        # The truncation diagonalises density-matrix blocks of 512 rows or more on the GPU
+        # while the host LAPACK does the smaller ones; the "fp64 host-trunc" segment keeps
+        # every block on the host for comparison.
         mk = lambda p, r: sweep.GpuSolver(precision=p, refine=r, mixed_warm=True)  # noqa: E731
         gpu_eig = sweep.GpuDensityEig(512)
-        # rocSOLVER loads its kernels on the first call of each size class: not timed.
-        w = np.random.default_rng(0).standard_normal((640, 640))
-        dm.density_diag([w + w.T, np.eye(64) + 0.5])
+        # rocSOLVER loads its kernels on the first call of each size class, and each of the
+        # solver's four streams sizes its workspace on its first block: not timed.
+        w = np.random.default_rng(0).standard_normal((1280, 1280))
+        dm.density_diag([w + w.T] * 4 + [np.eye(64) + 0.5])
         for mode, solver, eig in (("fp64", sweep.GpuSolver(precision="fp64"), gpu_eig),
                                   ("fp64 host-trunc", sweep.GpuSolver(precision="fp64"), None),
                                   ("fp32+refine", mk("fp32", 1), gpu_eig),

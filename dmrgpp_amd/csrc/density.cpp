@@ -4,8 +4,11 @@
 //
 // Every block is copied to the device, diagonalised by rocSOLVER's divide-and-conquer dsyevd
 // (eigenvalues ascending, orthonormal eigenvectors in the columns, as LAPACK dsyev returns
-// them) and copied back. The blocks share one device buffer sized for the largest; a rocBLAS
-// handle and a stream are kept per device for the life of the process.
+// them) and copied back. A dsyevd of n ~ 1000 is a long chain of small, latency-bound kernels
+// (the tridiagonal reduction) that fills a fraction of the chip, so the blocks run
+// concurrently: up to kSolverStreams host threads, each with its own rocBLAS handle and stream,
+// take the blocks largest first. Handles and streams are kept per device for the life of the
+// process; every block has its own region of one device buffer.
 #include "dmk_internal.h"
 #include "pool.h"
 
@@ -14,22 +17,28 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <exception>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace dmk {
 
 namespace {
 
+constexpr int kSolverStreams = 4;
+
 struct SolverContext {
 	rocblas_handle handle = nullptr;
 	hipStream_t    stream = nullptr;
 };
 
-std::mutex                 g_mutex;
-std::vector<SolverContext> g_contexts; // per device, created on first use
+std::mutex                                                g_mutex;
+std::vector<std::array<SolverContext, kSolverStreams>> g_contexts; // per device, on first use
 
 void check_rocblas(rocblas_status st, const char* what)
 {
@@ -38,19 +47,48 @@ void check_rocblas(rocblas_status st, const char* what)
 		                         + std::to_string(int(st)) + ")");
 }
 
-SolverContext& context_for_current_device()
+// The first `count` contexts of the current device, created on first use.
+SolverContext* contexts_for_current_device(int count)
 {
 	int dev = 0;
 	DMK_HIP_CHECK(hipGetDevice(&dev));
 	if (int(g_contexts.size()) <= dev)
 		g_contexts.resize(dev + 1);
-	SolverContext& c = g_contexts[dev];
-	if (!c.handle) {
-		DMK_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-		check_rocblas(rocblas_create_handle(&c.handle), "rocblas_create_handle");
-		check_rocblas(rocblas_set_stream(c.handle, c.stream), "rocblas_set_stream");
+	for (int w = 0; w < count; ++w) {
+		SolverContext& c = g_contexts[dev][w];
+		if (!c.stream)
+			DMK_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+		if (!c.handle) {
+			check_rocblas(rocblas_create_handle(&c.handle), "rocblas_create_handle");
+			check_rocblas(rocblas_set_stream(c.handle, c.stream), "rocblas_set_stream");
+		}
 	}
-	return c;
+	return g_contexts[dev].data();
+}
+
+// One block: upload, dsyevd, download of the eigenvectors, eigenvalues and status.
+void diag_one(const SolverContext& ctx, int b, int64_t m, double* host, double* eigs, char* dev)
+{
+	const size_t mat   = sizeof(double) * size_t(m) * size_t(m);
+	double*      dA    = reinterpret_cast<double*>(dev);
+	double*      dW    = dA + m * m;
+	double*      dE    = dW + m;
+	rocblas_int* dInfo = reinterpret_cast<rocblas_int*>(dE + m);
+	DMK_HIP_CHECK(hipMemcpyAsync(dA, host, mat, hipMemcpyHostToDevice, ctx.stream));
+	check_rocblas(rocsolver_dsyevd(ctx.handle, rocblas_evect_original, rocblas_fill_upper,
+	                               rocblas_int(m), dA, rocblas_int(m), dW, dE, dInfo),
+	              "rocsolver_dsyevd");
+	rocblas_int info = 0;
+	DMK_HIP_CHECK(hipMemcpyAsync(host, dA, mat, hipMemcpyDeviceToHost, ctx.stream));
+	DMK_HIP_CHECK(hipMemcpyAsync(eigs, dW, sizeof(double) * size_t(m), hipMemcpyDeviceToHost,
+	                             ctx.stream));
+	DMK_HIP_CHECK(hipMemcpyAsync(&info, dInfo, sizeof(rocblas_int), hipMemcpyDeviceToHost,
+	                             ctx.stream));
+	DMK_HIP_CHECK(hipStreamSynchronize(ctx.stream));
+	if (info != 0)
+		throw std::runtime_error("dmk_density_diag_d: dsyevd did not converge on block "
+		                         + std::to_string(b) + " (info " + std::to_string(int(info))
+		                         + ")");
 }
 
 } // namespace
@@ -63,57 +101,68 @@ void density_diag(int nblocks, const int64_t* n, double* const* blocks, double* 
 		return;
 	if (!n || !blocks || !eigs)
 		throw std::invalid_argument("dmk_density_diag_d: null argument");
-	int64_t nmax = 0;
+	std::vector<int64_t> eoff(size_t(nblocks) + 1, 0); // offsets of the blocks' eigenvalues
+	std::vector<size_t>  doff(size_t(nblocks), 0);     // offsets of the blocks' device regions
+	std::vector<int>     order;
+	size_t               total = 0;
 	for (int b = 0; b < nblocks; ++b) {
 		if (n[b] < 0 || n[b] > (int64_t(1) << 15))
 			throw std::invalid_argument("dmk_density_diag_d: block size out of range (0..32768)");
 		if (n[b] > 0 && !blocks[b])
 			throw std::invalid_argument("dmk_density_diag_d: null block");
-		nmax = std::max(nmax, n[b]);
+		eoff[b + 1] = eoff[b] + n[b];
+		if (n[b] > 0) {
+			order.push_back(b);
+			doff[b] = total;
+			// A (m x m), W (m), E (m), info; regions 256-byte aligned.
+			total += (sizeof(double) * size_t(n[b]) * size_t(n[b] + 2) + 64 + 255) & ~size_t(255);
+		}
 	}
-	if (nmax == 0)
+	if (order.empty())
 		return;
+	// Largest first, so that the longest solves start at once.
+	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
 
 	std::lock_guard<std::mutex> lock(g_mutex);
-	SolverContext&              ctx = context_for_current_device();
-	const size_t bytes = sizeof(double) * size_t(nmax) * size_t(nmax + 2) + 64;
-	char*        buf   = static_cast<char*>(pool_device_alloc(bytes));
-	double*      dA    = reinterpret_cast<double*>(buf);
-	double*      dW    = dA + nmax * nmax;
-	double*      dE    = dW + nmax;
-	rocblas_int* dInfo = reinterpret_cast<rocblas_int*>(dE + nmax);
-	try {
-		int64_t off = 0;
-		for (int b = 0; b < nblocks; ++b) {
-			const int64_t m = n[b];
-			if (m == 0)
-				continue;
-			const size_t mat = sizeof(double) * size_t(m) * size_t(m);
-			DMK_HIP_CHECK(hipMemcpyAsync(dA, blocks[b], mat, hipMemcpyHostToDevice, ctx.stream));
-			check_rocblas(rocsolver_dsyevd(ctx.handle, rocblas_evect_original, rocblas_fill_upper,
-			                               rocblas_int(m), dA, rocblas_int(m), dW, dE, dInfo),
-			              "rocsolver_dsyevd");
-			rocblas_int info = 0;
-			DMK_HIP_CHECK(hipMemcpyAsync(blocks[b], dA, mat, hipMemcpyDeviceToHost, ctx.stream));
-			DMK_HIP_CHECK(hipMemcpyAsync(eigs + off, dW, sizeof(double) * size_t(m),
-			                             hipMemcpyDeviceToHost, ctx.stream));
-			DMK_HIP_CHECK(hipMemcpyAsync(&info, dInfo, sizeof(rocblas_int), hipMemcpyDeviceToHost,
-			                             ctx.stream));
-			DMK_HIP_CHECK(hipStreamSynchronize(ctx.stream));
-			if (info != 0)
-				throw std::runtime_error("dmk_density_diag_d: dsyevd did not converge on block "
-				                         + std::to_string(b) + " (info "
-				                         + std::to_string(int(info)) + ")");
-			off += m;
+	const int      nw   = std::min(kSolverStreams, int(order.size()));
+	SolverContext* ctxs = contexts_for_current_device(nw);
+	int            dev  = 0;
+	DMK_HIP_CHECK(hipGetDevice(&dev));
+	char* buf = static_cast<char*>(pool_device_alloc(total));
+
+	std::atomic<size_t>             next { 0 };
+	std::atomic<bool>               failed { false };
+	std::vector<std::exception_ptr> errs(static_cast<size_t>(nblocks), nullptr);
+	auto work = [&](int w) {
+		if (w > 0 && hipSetDevice(dev) != hipSuccess) {
+			failed = true;
+			errs[size_t(order[0])] = std::make_exception_ptr(
+			    std::runtime_error("dmk_density_diag_d: hipSetDevice failed in a worker"));
+			return;
 		}
-	} catch (...) {
-		// Drain the stream before the buffer goes back to the pool; the first error is the
-		// one reported.
-		(void)hipStreamSynchronize(ctx.stream);
-		pool_device_free(buf);
-		throw;
-	}
+		for (size_t i = next.fetch_add(1); i < order.size() && !failed; i = next.fetch_add(1)) {
+			const int b = order[i];
+			try {
+				diag_one(ctxs[w], b, n[b], blocks[b], eigs + eoff[b], buf + doff[b]);
+			} catch (...) {
+				errs[size_t(b)] = std::current_exception();
+				failed          = true;
+			}
+		}
+		// Drain the stream before the buffer goes back to the pool.
+		(void)hipStreamSynchronize(ctxs[w].stream);
+	};
+	std::vector<std::thread> pool;
+	for (int w = 1; w < nw; ++w)
+		pool.emplace_back(work, w);
+	work(0);
+	for (auto& t : pool)
+		t.join();
 	pool_device_free(buf);
+	// The error of the lowest-numbered failing block is the one reported.
+	for (const auto& e : errs)
+		if (e)
+			std::rethrow_exception(e);
 }
 
 } // namespace dmk

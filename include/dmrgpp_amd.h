@@ -290,7 +290,8 @@ int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t 
  * major, in host memory at blocks[b] with leading dimension n[b] (0 <= n[b] <= 32768); only its
  * upper triangle is read. It is overwritten by its orthonormal eigenvectors (column j belongs to
  * eigenvalue j), and its eigenvalues, ascending, go to eigs[o_b .. o_b + n[b]) with
- * o_b = n[0] + ... + n[b-1]. rocSOLVER dsyevd on the device; synchronous. The reference's
+ * o_b = n[0] + ... + n[b-1]. rocSOLVER dsyevd on the device, up to four blocks at once on
+ * their own streams and host threads (largest first); synchronous. The reference's
  * EnforcePhase (EnforcePhase.h:23-39) multiplies each eigenvector by the sign of a norm, which is
  * +1 for real entries, so no sign convention is imposed here either. */
 int dmk_density_diag_d(int nblocks, const int64_t* n, double* const* blocks, double* eigs);

@@ -1,13 +1,15 @@
 """The truncation's density-matrix eigensolver on the GPU (dmk_density_diag_d, the replacement of
 DiagBlockDiagMatrix::diagonalise): eigenpairs against the host LAPACK, degenerate spectra, and a
-sweep whose truncation runs on the GPU against the same sweep on the host."""
+sweep whose truncation runs on the GPU against the same sweep on the host. Also the truncation's
+change of basis on the GPU (sweep.GpuBasisChange) against the host's BSOp.transform."""
 import ctypes
 
 import numpy as np
 import pytest
 
 import dmrgpp_amd as dm
-from dmrgpp_amd import configs
+from dmrgpp_amd import configs, sweep
+from dmrgpp_amd.blocks import Basis, BSOp
 
 
 def _density_like(n, rank, seed):
@@ -79,3 +81,57 @@ def test_gpu_truncation_on_gpu_matches_host():
     assert abs(eh - eg) < 1e-10 * abs(eh)
     assert np.max(np.abs(np.array(sh) - np.array(sg))) < 1e-9
     assert np.max(np.abs(np.array(dh) - np.array(dg))) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_basis_change_matches_host_transform():
+    # Three groups of 40, 7 and 5 states truncated to 12, 0 and 5: blocks between kept groups,
+    # blocks touching the dropped group (left out), and one W matrix shared by several blocks
+    # and operators.
+    rng = np.random.default_rng(11)
+    old = Basis([(0,), (1,), (2,)], [40, 7, 5])
+    nb = Basis([(0,), (2,)], [12, 5])
+    q0, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    q2, _ = np.linalg.qr(rng.standard_normal((5, 5)))
+    W = {0: (np.asfortranarray(q0[:, :12]), (0, 12)), 2: (np.asfortranarray(q2), (1, 5))}
+    sizes = old.sizes
+    blk = lambda a, b: np.asfortranarray(rng.standard_normal((sizes[a], sizes[b])))  # noqa: E731
+    ops = [BSOp(old, old, {(0, 0): blk(0, 0), (2, 2): blk(2, 2), (1, 1): blk(1, 1)}),
+           BSOp(old, old, {(0, 2): blk(0, 2), (2, 0): blk(2, 0), (1, 0): blk(1, 0)}, True),
+           BSOp(old, old, {(0, 0): blk(0, 0)})]
+    got = sweep.GpuBasisChange()(ops, W, nb)
+    for op, g in zip(ops, got):
+        want = op.transform(W, nb, W, nb)
+        assert g.fermionic == op.fermionic
+        assert set(g.blocks) == set(want.blocks)
+        for key, b in want.blocks.items():
+            assert g.blocks[key].shape == b.shape
+            assert np.max(np.abs(g.blocks[key] - b)) < 1e-13 * max(1.0, np.max(np.abs(b)))
+
+
+@pytest.mark.gpu
+def test_gpu_basis_change_sweep_matches_host():
+    # The same Hubbard sweep with the change of basis on the GPU and on the host.
+    runs = {}
+    for key, on_gpu in (("host", False), ("gpu", True)):
+        e, recs, _ = configs.run_config("hubbard-chain-8-U0", log=lambda s: None,
+                                        gpu_basis=on_gpu)
+        runs[key] = (e, [r.energy for r in recs], [r.discarded for r in recs])
+    (eh, sh, dh), (eg, sg, dg) = runs["host"], runs["gpu"]
+    assert abs(eh - eg) < 1e-10 * abs(eh)
+    assert np.max(np.abs(np.array(sh) - np.array(sg))) < 1e-9
+    assert np.max(np.abs(np.array(dh) - np.array(dg))) < 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_density_eig_splits_host_and_device():
+    # Blocks above the threshold go to the device (several streams at once), the rest to the
+    # host LAPACK on the calling thread meanwhile; every block comes back in its own slot.
+    sizes = [5, 700, 70, 300, 1, 530, 30, 64]
+    rhos = [_density_like(n, max(1, n // 2), seed=20 + i) for i, n in enumerate(sizes)]
+    got = sweep.GpuDensityEig(64)(rhos)
+    for a, (w, v) in zip(rhos, got):
+        n = a.shape[0]
+        assert w.shape == (n,) and v.shape == (n, n)
+        assert np.max(np.abs(w - np.linalg.eigvalsh(a))) < 1e-13
+        assert np.linalg.norm(a @ v - v * w) < 1e-12
