@@ -12,6 +12,14 @@ diagonal method is the fallback).
 Phase 2 runs the remaining sweeps from the same block stores with plain Lanczos, Davidson +
 diagonal, Davidson + MLP and the validated choice, each step started from the same transformed
 wave function, and counts matvecs and time.
+
+  python -m dmrgpp_amd.learned --config hubbard-cylinder-16x4 --superblock-at 8
+
+runs the workload at its full m on one superblock instead (sweeps at m = 4096 keep the host
+side of the driver busy for longer than a measurement should take): the infinite growth stops
+at the given step, once both blocks hold m states, and that superblock's matvec is timed in the
+three precisions and solved by Lanczos and by Davidson (diagonal correction) to the same FP64
+residual from the same start vector.
 """
 from __future__ import annotations
 
@@ -162,6 +170,83 @@ def compare(name: str, m=None, train_sweeps=1, test_sweeps=1, tol=1e-9, train_it
     return out
 
 
+class _KeepLast:
+    """sweep.run solver that keeps the last superblock it solved."""
+
+    def __init__(self, inner):
+        self.inner, self.prob = inner, None
+
+    def __call__(self, prob):
+        self.prob = prob
+        return self.inner(prob)
+
+
+def superblock_compare(name: str, m=None, at=8, tol=1e-9, max_matvecs=300, log=print):
+    """One superblock of the workload at its full m (see the module docstring)."""
+    from . import BatchedGemm, DeviceArray
+
+    c = CONFIGS[name]
+    m = m or c["m"]
+    rec = _KeepLast(sweep.GpuSolver(precision="fp64"))
+    t0 = time.perf_counter()
+    sweep.run(c["model"](), c["lattice"](), c["target"], m, [], rec, log=log, stop_after=at,
+              eig=sweep.GpuDensityEig(512))
+    prob = rec.prob
+    out = {"config": name, "m": m, "growth_steps": at, "growth_s": time.perf_counter() - t0,
+           "n": prob.n, "npatches": prob.npatches, "nterms": len(prob.blocks),
+           "histogram": prob.histogram()}
+    log(f"superblock: n={prob.n} patches={prob.npatches} terms={len(prob.blocks)}")
+    x = np.random.default_rng(1234).uniform(-1.0, 1.0, prob.n)
+    dx, dy = DeviceArray.from_host(x), DeviceArray(prob.n)
+    op = BatchedGemm(prob, "fp64")
+    flops = op.info()["flops_algorithmic"]
+    for prec in ("fp64", "fp32", "bf16x3"):
+        op.set_precision(prec)
+        op.autotune(dx, dy, 3)
+        for _ in range(3):
+            op.apply_Htarget(dx, dy)
+        op.sync()
+        t = time.perf_counter()
+        for _ in range(10):
+            op.apply_Htarget(dx, dy)
+        op.sync()
+        ms = 1e3 * (time.perf_counter() - t) / 10
+        out[prec] = {"ms_per_matvec": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+                     "tuning": op.tuning()}
+        log(f"{prec}: {ms:.3f} ms/matvec, {flops / (ms * 1e-3) / 1e12:.1f} TF/s")
+    op.close()
+    dx.free()
+    dy.free()
+    for key, solver in (("lanczos", LanczosResidualSolver(tol=tol, max_steps=max_matvecs)),
+                        ("davidson-diag", DavidsonSolver("diag", tol=tol,
+                                                         max_matvecs=max_matvecs))):
+        t = time.perf_counter()
+        e, _, r = solver(prob)
+        dt = time.perf_counter() - t
+        r["z_dev"].free()
+        out[key] = {"energy": e, "matvecs": r["matvecs"], "wall_s": dt,
+                    "residual": r.get("residual", None)}
+        log(f"{key:14s} E={e:.12f} matvecs={r['matvecs']} wall={dt:.2f}s "
+            f"residual={r.get('residual', float('nan')):.2e}")
+    out["matvec_ratio_lanczos_vs_davidson"] = (out["lanczos"]["matvecs"]
+                                               / max(1, out["davidson-diag"]["matvecs"]))
+    # The mixed-precision ground state (FP32 / BF16x3 Lanczos, FP64 Davidson refinement) against
+    # FP64 Lanczos, all stopped at the energy criterion of the sweeps.
+    for prec in ("fp64", "fp32", "bf16x3"):
+        t = time.perf_counter()
+        e, _, r = sweep.GpuSolver(precision=prec, refine=2)(prob)
+        dt = time.perf_counter() - t
+        if hasattr(r.get("z_dev"), "free"):
+            r["z_dev"].free()
+        out["ground_" + prec] = {"energy": e, "matvecs": r.get("matvecs"), "wall_s": dt,
+                                 "residual": r.get("residual")}
+        log(f"ground {prec:7s} E={e:.12f} matvecs={r.get('matvecs')} wall={dt:.2f}s")
+    e64 = out["ground_fp64"]["energy"]
+    for prec in ("fp32", "bf16x3"):
+        out["ground_" + prec]["rel_dE_vs_fp64"] = abs(out["ground_" + prec]["energy"] - e64) / abs(e64)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="hubbard-ladder-8x2", choices=sorted(CONFIGS))
@@ -169,8 +254,19 @@ def main(argv=None):
     ap.add_argument("--train-sweeps", type=int, default=1)
     ap.add_argument("--test-sweeps", type=int, default=1)
     ap.add_argument("--train-iters", type=int, default=3000)
+    ap.add_argument("--superblock-at", type=int, default=0, metavar="STEP",
+                    help="time one superblock at full m, reached after STEP growth steps, "
+                         "instead of running sweeps")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
+    if a.superblock_at > 0:
+        out = superblock_compare(a.config, a.m, a.superblock_at,
+                                 log=lambda s: print(s, flush=True))
+        print(json.dumps(out), flush=True)
+        if a.json:
+            with open(a.json, "w") as fh:
+                json.dump(out, fh)
+        return 0
     out = compare(a.config, a.m, a.train_sweeps, a.test_sweeps, train_iters=a.train_iters,
                   log=lambda s: print(s, flush=True))
     print(json.dumps(out), flush=True)
