@@ -145,3 +145,20 @@ def test_gpu_learned_preconditioner_comparison():
     assert out["samples"] > 0 and out["loss"] <= out["loss_untrained"]
     assert out["davidson-learned"]["matvecs"] <= 1.1 * out["davidson-diag"]["matvecs"]
     assert out["lanczos"]["matvecs"] > out["davidson-diag"]["matvecs"]
+
+
+@pytest.mark.gpu
+def test_gpu_superblock_compare_small():
+    # The one-superblock comparison on a small Hubbard ladder superblock: Lanczos and Davidson
+    # reach the FP64 residual at the same energy, and the mixed-precision ground states
+    # (FP32 / BF16x3 with FP64 refinement) match FP64 Lanczos to 1e-10.
+    from dmrgpp_amd import learned
+
+    out = learned.superblock_compare("hubbard-ladder-8x2", at=5, log=lambda s: None)
+    assert out["n"] > 10000 and out["npatches"] > 1
+    e = out["lanczos"]["energy"]
+    assert out["lanczos"]["residual"] < 1.1e-9 and out["davidson-diag"]["residual"] < 1.1e-9
+    assert abs(out["davidson-diag"]["energy"] - e) < 1e-10 * abs(e)
+    for prec in ("fp64", "fp32", "bf16x3"):
+        assert out[prec]["ms_per_matvec"] > 0
+        assert abs(out["ground_" + prec]["energy"] - e) < 1e-10 * abs(e)
