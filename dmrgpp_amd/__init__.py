@@ -172,6 +172,7 @@ SIGNATURES = {
                                  ctypes.POINTER(KrylovParams), _VP, ctypes.POINTER(KrylovStats)]),
     "dmk_diagonal_d": (_I, [_VP, _VP]),
     "dmk_density_diag_d": (_I, [_I, _VP, _VP, _VP]),
+    "dmk_density_svd_d": (_I, [_I, _VP, _VP, _VP, _VP, _VP]),
     "dmk_mlp_create": (_I, [ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_VP)]),
     "dmk_mlp_destroy": (None, [_VP]),
     "dmk_mlp_get_params": (_I, [_VP, _VP]),
@@ -267,6 +268,31 @@ def density_diag(blocks):
     for a in mats:
         k = a.shape[0]
         out.append((eigs[off:off + k].copy(), a))
+        off += k
+    return out
+
+
+def density_svd(blocks):
+    """Singular values and all left singular vectors of wave-function blocks on the GPU
+    (dmk_density_svd_d, the truncation's DensityMatrixSvd::diag): returns [(s descending, u)]
+    with u square (rows x rows)."""
+    mats = [np.asfortranarray(b, dtype=np.float64) for b in blocks]
+    for a in mats:
+        assert a.ndim == 2
+    nb = len(mats)
+    rows = (ctypes.c_int64 * max(nb, 1))(*[a.shape[0] for a in mats])
+    cols = (ctypes.c_int64 * max(nb, 1))(*[a.shape[1] for a in mats])
+    us = [np.empty((a.shape[0], a.shape[0]), order="F") for a in mats]
+    aps = (ctypes.c_void_p * max(nb, 1))(*[a.ctypes.data for a in mats])
+    ups = (ctypes.c_void_p * max(nb, 1))(*[u.ctypes.data for u in us])
+    s = np.zeros(max(1, sum(min(a.shape) for a in mats)))
+    _check(lib().dmk_density_svd_d(nb, ctypes.addressof(rows), ctypes.addressof(cols),
+                                   ctypes.addressof(aps), ctypes.addressof(ups), s.ctypes.data),
+           "dmk_density_svd_d")
+    out, off = [], 0
+    for a, u in zip(mats, us):
+        k = min(a.shape)
+        out.append((s[off:off + k].copy(), u))
         off += k
     return out
 

@@ -41,14 +41,19 @@ CONFIGS = {
 
 
 def run_config(name: str, precision="fp64", m=None, sweeps=None, stop_after=None, log=print,
-               solver=None, wft=None, refine=2, gpu_trunc_min=0, gpu_basis=False):
+               solver=None, wft=None, refine=2, gpu_trunc_min=0, gpu_basis=False, svd=False):
     """gpu_trunc_min > 0: the truncation diagonalises density-matrix blocks of at least that
-    many rows on the GPU; gpu_basis: it also changes the basis of the block operators there."""
+    many rows on the GPU; gpu_basis: it also changes the basis of the block operators there;
+    svd: the truncation takes the SVD of the wave-function blocks instead, on the host (faster
+    than rocSOLVER's dgesvd at these sizes; sweep.SvdDensityEig(device=True) runs it there)."""
     c = CONFIGS[name]
     m = m or c["m"]
     nsw = c["sweeps"] if sweeps is None else sweeps
     solver = solver or sweep.GpuSolver(precision=precision, refine=refine)
-    eig = sweep.GpuDensityEig(gpu_trunc_min) if gpu_trunc_min > 0 else None
+    if svd:
+        eig = sweep.SvdDensityEig()
+    else:
+        eig = sweep.GpuDensityEig(gpu_trunc_min) if gpu_trunc_min > 0 else None
     t0 = time.perf_counter()
     e, recs = sweep.run(c["model"](), c["lattice"](), c["target"], m, [m] * nsw, solver, log=log,
                         stop_after=stop_after, wft=wft, eig=eig,
@@ -74,12 +79,14 @@ def main(argv=None):
                          "GPU (0: all on the host)")
     ap.add_argument("--gpu-basis", action="store_true",
                     help="do the truncation's change of basis on the GPU")
+    ap.add_argument("--svd", action="store_true",
+                    help="truncate by the SVD of the wave-function blocks (DensityMatrixSvd)")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     e, recs, wall = run_config(a.config, a.precision, a.m, a.sweeps, a.stop_after,
                                log=lambda s: print(s, flush=True), wft=True if a.wft else None,
                                refine=a.refine, gpu_trunc_min=a.gpu_trunc_min,
-                               gpu_basis=a.gpu_basis)
+                               gpu_basis=a.gpu_basis, svd=a.svd)
     sweep_steps = [r for r in recs if r.phase != "infinite"]
     out = {"config": a.config, "precision": a.precision, "energy": e, "wall_s": wall,
            "steps": len(recs), "matvecs": sum(r.matvecs for r in recs),

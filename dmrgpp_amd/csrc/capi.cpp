@@ -43,6 +43,8 @@ struct dmk_handle {
 namespace dmk {
 
 void density_diag(int nblocks, const int64_t* n, double* const* blocks, double* eigs);
+void density_svd(int nblocks, const int64_t* rows, const int64_t* cols,
+                 const double* const* blocks, double* const* u, double* s);
 void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
 void launch_sector_gather(const double* v, const int64_t* map, int64_t n, double* y, hipStream_t s);
 void launch_sector_scatter_add(const double* x, const int64_t* map, int64_t n, double* v,
@@ -562,6 +564,15 @@ int dmk_density_diag_d(int nblocks, const int64_t* n, double* const* blocks, dou
 {
 	return guarded([&] {
 		dmk::density_diag(nblocks, n, blocks, eigs);
+		return DMK_OK;
+	});
+}
+
+int dmk_density_svd_d(int nblocks, const int64_t* rows, const int64_t* cols,
+                      const double* const* blocks, double* const* u, double* s)
+{
+	return guarded([&] {
+		dmk::density_svd(nblocks, rows, cols, blocks, u, s);
 		return DMK_OK;
 	});
 }

@@ -296,6 +296,20 @@ int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t 
  * +1 for real entries, so no sign convention is imposed here either. */
 int dmk_density_diag_d(int nblocks, const int64_t* n, double* const* blocks, double* eigs);
 
+/* The SVD form of the same step: replaces DensityMatrixSvd::diag (DensityMatrixSvd.h:517-554),
+ * whose ParallelSvd::doTask (DensityMatrixSvd.h:382-410) calls PsimagLite::Svd with jobz 'A' on
+ * every group's wave-function matrix M_b (the group's states x its partner states, weighted by
+ * the square roots of the target weights); the eigenvectors of the density matrix M_b M_b^T are
+ * the left singular vectors and its eigenvalues the squared singular values. Block b is
+ * rows[b] x cols[b], column major, at blocks[b] with leading dimension rows[b] (both sizes
+ * 0..32768); it is not modified. u[b] receives all rows[b] x rows[b] left singular vectors
+ * (column j belongs to singular value j; the columns past min(rows, cols) span the null space,
+ * and a block with no columns gets the identity), s[o_b .. o_b + min(rows[b], cols[b])) the
+ * singular values in descending order, o_b = sum of min(rows, cols) over the earlier blocks.
+ * rocSOLVER dgesvd on the device, up to four blocks at once; synchronous. */
+int dmk_density_svd_d(int nblocks, const int64_t* rows, const int64_t* cols,
+                      const double* const* blocks, double* const* u, double* s);
+
 /* Variable-size batched GEMM: replaces dmrg_Xgemm_vbatch<T> (GPUPlugin/dmrg_vbatch.h:118-133,
  * dmrg_vbatch.cpp:129-652), the free function the reference plugin and its test_vbatch driver
  * call. For every entry e of every group g: C_e = alpha op(A_e) op(B_e) + beta C_e, column major,

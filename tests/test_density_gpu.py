@@ -1,14 +1,16 @@
 """The truncation's density-matrix eigensolver on the GPU (dmk_density_diag_d, the replacement of
 DiagBlockDiagMatrix::diagonalise): eigenpairs against the host LAPACK, degenerate spectra, and a
 sweep whose truncation runs on the GPU against the same sweep on the host. Also the truncation's
-change of basis on the GPU (sweep.GpuBasisChange) against the host's BSOp.transform."""
+change of basis on the GPU (sweep.GpuBasisChange) against the host's BSOp.transform, and the SVD
+form of the truncation (dmk_density_svd_d, the replacement of DensityMatrixSvd::diag) against the
+host LAPACK."""
 import ctypes
 
 import numpy as np
 import pytest
 
 import dmrgpp_amd as dm
-from dmrgpp_amd import configs, sweep
+from dmrgpp_amd import configs, models, sweep
 from dmrgpp_amd.blocks import Basis, BSOp
 
 
@@ -135,3 +137,64 @@ def test_gpu_density_eig_splits_host_and_device():
         assert w.shape == (n,) and v.shape == (n, n)
         assert np.max(np.abs(w - np.linalg.eigvalsh(a))) < 1e-13
         assert np.linalg.norm(a @ v - v * w) < 1e-12
+
+
+def test_density_svd_rejects_bad_arguments():
+    # Argument checks of dmk_density_svd_d run before any device call.
+    lib = dm.lib()
+    rows = (ctypes.c_int64 * 1)(4)
+    cols = (ctypes.c_int64 * 1)(-1)
+    ptrs = (ctypes.c_void_p * 1)(None)
+    s = (ctypes.c_double * 4)()
+    rc = lib.dmk_density_svd_d(1, ctypes.addressof(rows), ctypes.addressof(cols),
+                               ctypes.addressof(ptrs), ctypes.addressof(ptrs), ctypes.addressof(s))
+    assert rc != 0 and b"out of range" in lib.dmk_last_error()
+    cols[0] = 3
+    rc = lib.dmk_density_svd_d(1, ctypes.addressof(rows), ctypes.addressof(cols),
+                               ctypes.addressof(ptrs), ctypes.addressof(ptrs), ctypes.addressof(s))
+    assert rc != 0 and b"null" in lib.dmk_last_error()
+    assert lib.dmk_density_svd_d(-1, None, None, None, None, None) != 0
+    assert lib.dmk_density_svd_d(0, None, None, None, None, None) == 0
+
+
+@pytest.mark.gpu
+def test_gpu_density_svd_matches_lapack():
+    # Tall, wide, square, rank-deficient, single-row and single-column blocks, and one with no
+    # columns (identity), in one call: singular values against LAPACK, U orthogonal, and
+    # M = U S V^T reproduced through the leading columns of U.
+    rng = np.random.default_rng(7)
+    shapes = [(700, 300), (200, 640), (513, 513), (64, 1), (1, 90), (40, 0), (333, 333)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    mats[-1] = mats[-1][:, :20] @ rng.standard_normal((20, 333))  # rank 20
+    out = dm.density_svd(mats)
+    for m, (sv, u) in zip(mats, out):
+        r, c = m.shape
+        k = min(r, c)
+        assert sv.shape == (k,) and u.shape == (r, r)
+        assert np.linalg.norm(u.T @ u - np.eye(r)) < 1e-11 * np.sqrt(r)
+        if k == 0:
+            assert np.array_equal(u, np.eye(r))
+            continue
+        s0 = np.linalg.svd(m, compute_uv=False)
+        assert np.all(np.diff(sv) <= 1e-12 * s0[0])
+        assert np.max(np.abs(sv - s0)) < 1e-12 * s0[0]
+        # The leading k columns of U carry M: U_k^T M has the singular values as row norms.
+        proj = u[:, :k].T @ m
+        assert np.max(np.abs(np.linalg.norm(proj, axis=1) - sv)) < 1e-11 * s0[0]
+        assert np.linalg.norm(m - u[:, :k] @ proj) < 1e-11 * np.linalg.norm(m)
+
+
+@pytest.mark.gpu
+def test_gpu_svd_truncation_matches_host():
+    # The SVD form of the truncation with every block on the GPU: the host-eigensolver sweep's
+    # energies and discarded weights.
+    model, lat = models.hubbard(0.0), models.chain(8)
+    runs = {}
+    for key, eig in (("host", None), ("svd", sweep.SvdDensityEig(min_size=1, device=True))):
+        e, recs = sweep.run(model, lat, lambda N: (N // 2, N // 2), 256, [256],
+                            sweep.GpuSolver(precision="fp64"), log=lambda s: None, eig=eig)
+        runs[key] = (e, [r.energy for r in recs], [r.discarded for r in recs])
+    (eh, sh, dh), (es, ss, ds) = runs["host"], runs["svd"]
+    assert abs(eh - es) < 1e-10 * abs(eh)
+    assert np.max(np.abs(np.array(sh) - np.array(ss))) < 1e-9
+    assert np.max(np.abs(np.array(dh) - np.array(ds))) < 1e-10

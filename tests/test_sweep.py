@@ -194,3 +194,32 @@ def test_gpu_wft_sweep_matches_host_chains():
     assert abs(e_gpu - e_host) < 1e-9
     assert checked and max(checked) < 1e-13
     assert all(r.info.get("wft_init") for r in recs if r.phase != "infinite")
+
+
+def test_svd_truncation_pairs_match_density_eigenpairs():
+    # The SVD form of the truncation (DensityMatrixSvd) on the host: the eigenpairs built from
+    # the wave-function blocks equal those of M M^T, for tall, wide, square, rank-deficient and
+    # single-column blocks; the zero eigenvalues past min(rows, cols) are exact zeros.
+    rng = np.random.default_rng(5)
+    shapes = [(7, 3), (3, 7), (6, 6), (9, 1), (1, 4), (5, 5)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    mats[-1] = mats[-1][:, :2] @ rng.standard_normal((2, 5))   # rank 2
+    got = sweep.SvdDensityEig(device=False)(mats)
+    want = sweep.host_eig([m @ m.T for m in mats])
+    for m, (w, v), (w0, v0) in zip(mats, got, want):
+        r = m.shape[0]
+        assert w.shape == (r,) and v.shape == (r, r)
+        assert np.all(np.diff(w) >= 0)
+        assert np.max(np.abs(w - w0)) < 1e-12 * max(1.0, w0[-1])
+        assert np.linalg.norm(v.T @ v - np.eye(r)) < 1e-12
+        assert np.linalg.norm((m @ m.T) @ v - v * w) < 1e-11 * max(1.0, w0[-1])
+        k = min(m.shape)
+        if k < r:
+            assert np.all(w[:r - k] == 0.0)
+
+
+def test_sweep_with_svd_truncation_exact():
+    # The whole sweep with the SVD form of the truncation: the exact Heisenberg L = 8 energy.
+    e, _ = sweep.run(models.heisenberg(1.0), models.chain(8), lambda N: (0,), 64, [64],
+                     OracleSolver(), log=lambda s: None, eig=sweep.SvdDensityEig(device=False))
+    assert abs(e - E_HEIS_L8) < 1e-10
