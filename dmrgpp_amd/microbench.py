@@ -9,18 +9,23 @@
 
   python -m dmrgpp_amd.microbench [--replay F.npz [--load-tuning T.json]] [--iters 30]
   python -m dmrgpp_amd.microbench --replay F.npz --multi 1,2,4,8 [--iters 12]
+  python -m dmrgpp_amd.microbench --density 1093,986,654,538 [--iters 5]
 
 
 Each case reports the average matvec time from HIP events (one warm-up, then iters calls) and
 the algorithmic TFLOP/s in FP64, FP32 and BF16x3. With --multi, the replayed superblock is
 multiplied into several vectors per launch (dmk_apply_multi_d, the batched form of the Krylov
 time step) for each listed count, in FP64 and FP32, timed by wall clock around iters calls.
+With --density, the truncation's eigensolver (dmk_density_diag_d) diagonalises random density
+matrices of the listed sizes (the largest symmetry blocks of an m=2048 ladder truncation by
+default), all in one call and one at a time, timed by wall clock.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import sys
+import time
 
 import numpy as np
 
@@ -75,6 +80,34 @@ def run_multi(prob, counts, iters, log):
     return out
 
 
+def run_density(sizes, iters, log):
+    from . import density_diag
+
+    rng = np.random.default_rng(3)
+    mats = []
+    for n in sizes:
+        x = rng.standard_normal((max(1, n // 2), n))
+        rho = x.T @ x
+        mats.append(rho / np.trace(rho))
+    density_diag(mats)  # start-up of the solver's streams and workspaces: not timed
+    out = {"sizes": list(sizes)}
+    t = time.perf_counter()
+    for _ in range(iters):
+        density_diag(mats)
+    out["ms_together"] = 1e3 * (time.perf_counter() - t) / iters
+    single = []
+    for a in mats:
+        t = time.perf_counter()
+        for _ in range(iters):
+            density_diag([a])
+        single.append(1e3 * (time.perf_counter() - t) / iters)
+    out["ms_each"] = single
+    out["ms_one_at_a_time"] = sum(single)
+    log(f"density_diag {list(sizes)}: {out['ms_together']:.1f} ms together, "
+        f"{out['ms_one_at_a_time']:.1f} ms one at a time ({', '.join(f'{v:.1f}' for v in single)})")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--replay", default=None, help="superblock .npz written by bench.py")
@@ -85,9 +118,15 @@ def main(argv=None):
     ap.add_argument("--multi", default=None,
                     help="comma-separated vector counts: time dmk_apply_multi_d on the replayed "
                          "superblock instead of the single-vector cases")
+    ap.add_argument("--density", default=None, nargs="?", const="1093,986,654,538",
+                    help="comma-separated block sizes: time the truncation eigensolver instead")
     a = ap.parse_args(argv)
     tunings = json.load(open(a.load_tuning)) if a.load_tuning else None
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    if a.density:
+        sizes = [int(c) for c in a.density.split(",")]
+        print(json.dumps(run_density(sizes, a.iters, log)), flush=True)
+        return 0
     if a.multi:
         from .problem import load_npz
 
