@@ -137,8 +137,12 @@ void for_blocks_concurrently(const char* what, int nblocks, const std::vector<in
 		(void)hipStreamSynchronize(ctxs[w].stream);
 	};
 	std::vector<std::thread> pool;
-	for (int w = 1; w < nw; ++w)
-		pool.emplace_back(work, w);
+	try {
+		for (int w = 1; w < nw; ++w)
+			pool.emplace_back(work, w);
+	} catch (...) {
+		// No more threads: the ones started and this one take all the blocks.
+	}
 	work(0);
 	for (auto& t : pool)
 		t.join();
