@@ -125,24 +125,54 @@ class Product:
         # Basis sorts by qn; qns are already sorted so group indices are unchanged.
         assert self.basis.qns == qns
 
-    def kron(self, opA: BSOp | None, opB: BSOp | None, coef: float = 1.0) -> BSOp:
-        """coef * (opA (x) opB) in the product basis; None stands for the identity."""
+    def kron(self, opA: BSOp | None, opB: BSOp | None, coef: float = 1.0,
+             signA: np.ndarray | None = None, acc: BSOp | None = None) -> BSOp:
+        """coef * (opA (x) opB) in the product basis. None stands for the identity; with signA,
+        a None opA is diag(signA[g]) on each group g of A (the fermion parity). With acc, the
+        product is added into acc's blocks (which acc must own) and acc is returned.
+
+        Element (i na + p, j na' + q) of the (gA, gB) sub-block is b[i, j] a[p, q]. The sub-block
+        is addressed as a 4-index strided view (p, i, q, j), so an identity factor touches only
+        its diagonal and a dense pair loops over the nonzeros of the smaller factor: no dense
+        Kronecker product of an identity is ever formed."""
+        from numpy.lib.stride_tricks import as_strided
+
         A, B = self.A, self.B
         blocksA = opA.blocks if opA is not None else {(g, g): None for g in range(A.ngroups)}
         blocksB = opB.blocks if opB is not None else {(g, g): None for g in range(B.ngroups)}
-        out = {}
+        out = acc.blocks if acc is not None else {}
         for (gao, gai), a in blocksA.items():
+            sa = coef * (float(signA[gao]) if (a is None and signA is not None) else 1.0)
+            na_o, na_i = A.sizes[gao], A.sizes[gai]
             for (gbo, gbi), b in blocksB.items():
                 Go, oo = self.place[(gao, gbo)]
                 Gi, oi = self.place[(gai, gbi)]
-                na_o, nb_o = A.sizes[gao], B.sizes[gbo]
-                na_i, nb_i = A.sizes[gai], B.sizes[gbi]
-                am = np.eye(na_o) if a is None else a
-                bm = np.eye(nb_o) if b is None else b
+                nb_o, nb_i = B.sizes[gbo], B.sizes[gbi]
                 blk = out.get((Go, Gi))
                 if blk is None:
                     blk = np.zeros((self.basis.sizes[Go], self.basis.sizes[Gi]), order="F")
                     out[(Go, Gi)] = blk
-                blk[oo:oo + na_o * nb_o, oi:oi + na_i * nb_i] += coef * np.kron(bm, am)
+                assert blk.flags.f_contiguous
+                sub = blk[oo:oo + na_o * nb_o, oi:oi + na_i * nb_i]
+                r, c = sub.strides
+                if a is None and b is None:      # (p, i, p, i)
+                    d = as_strided(sub, (na_o, nb_o), (r + c, (r + c) * na_o))
+                    d += sa
+                elif a is None:                  # (p, i, p, j): b on every diagonal p
+                    d = as_strided(sub, (na_o, nb_o, nb_i), (r + c, r * na_o, c * na_i))
+                    d += sa * b[None, :, :]
+                elif b is None:                  # (p, i, q, i): a on every diagonal i
+                    d = as_strided(sub, (na_o, na_i, nb_o), (r, c, r * na_o + c * na_i))
+                    d += sa * a[:, :, None]
+                else:
+                    v = as_strided(sub, (na_o, nb_o, na_i, nb_i), (r, r * na_o, c, c * na_i))
+                    if b.size <= a.size:
+                        for i, j in zip(*np.nonzero(b)):
+                            v[:, i, :, j] += (sa * b[i, j]) * a
+                    else:
+                        for p_, q_ in zip(*np.nonzero(a)):
+                            v[p_, :, q_, :] += (sa * a[p_, q_]) * b
+        if acc is not None:
+            return acc
         ferm = (opA is not None and opA.fermionic) != (opB is not None and opB.fermionic)
         return BSOp(self.basis, self.basis, out, ferm)

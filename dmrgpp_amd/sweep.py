@@ -70,23 +70,22 @@ def grow(A: Block, B: Block, model: SiteModel, lattice: Lattice, keep) -> Block:
     """A (x) B with A's modes first: H, and the operators of the sites for which keep(s)."""
     P = Product(A.basis, B.basis)
     H = P.kron(A.H, None)
-    H.add_(P.kron(None, B.H))
+    P.kron(None, B.H, acc=H)
     for i, j, v in cross_bonds(lattice, A, B):
         for oi, oj, c, ferm in model.bond_terms(v):
             a = A.ops[(i, oi)]
             if ferm:
                 a = a.times_parity_in(A.parity)
-            H.add_(P.kron(a, B.ops[(j, oj)], c))
+            P.kron(a, B.ops[(j, oj)], c, acc=H)
     ops = {}
-    pA = BSOp(A.basis, A.basis,
-              {(g, g): np.eye(s, order="F") * A.parity[g] for g, s in enumerate(A.basis.sizes)})
     for (s, name), op in A.ops.items():
         if keep(s):
             ops[(s, name)] = P.kron(op, None)
             ops[(s, name)].fermionic = op.fermionic
     for (s, name), op in B.ops.items():
         if keep(s):
-            ops[(s, name)] = P.kron(pA if op.fermionic else None, op)
+            # A fermion operator of B carries the parity of A's modes (Jordan-Wigner string).
+            ops[(s, name)] = P.kron(None, op, signA=A.parity if op.fermionic else None)
             ops[(s, name)].fermionic = op.fermionic
     parity = np.array([_parity_of(model, q) for q in P.basis.qns])
     return Block(sorted(A.sites + B.sites), P.basis, H, ops, parity, product=P)
