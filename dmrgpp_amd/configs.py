@@ -29,6 +29,11 @@ CONFIGS = {
     "heisenberg-ladder-32x2": dict(model=lambda: models.heisenberg(1.0),
                                    lattice=lambda: models.ladder(32, 2), target=lambda N: (0,),
                                    m=2048, sweeps=2),
+    # Fermi-Hubbard two-leg ladder 24x2, U=4, t_leg = t_rung = 1, half filling, m=2048 (the
+    # time-evolution workload's lattice), one finite sweep.
+    "hubbard-ladder-24x2": dict(model=lambda: models.hubbard(4.0),
+                                lattice=lambda: models.ladder(24, 2),
+                                target=lambda N: (N // 2, N // 2), m=2048, sweeps=1),
     # Small cases used by the tests.
     "heisenberg-chain-8": dict(model=lambda: models.heisenberg(1.0), lattice=lambda: models.chain(8),
                                target=lambda N: (0,), m=16, sweeps=1),
@@ -93,6 +98,13 @@ def main(argv=None):
            "ms_matvec": sum(r.ms_matvec for r in recs),
            "ms_lanczos": sum(r.ms_lanczos for r in recs),
            "sweep_wall_s_est": sum(r.ms_lanczos for r in sweep_steps) / 1e3,
+           "sweep_steps": len(sweep_steps),
+           "sweep_step_s": (sum(sum(v for k, v in r.info.get("timing", {}).items()
+                                    if k in ("grow_ms", "kron_ms", "wft_ms", "truncate_ms"))
+                                + r.ms_lanczos for r in sweep_steps)
+                            / 1e3 / max(1, len(sweep_steps))),
+           "sweep_eigensolve_ms_per_step": (sum(r.ms_lanczos for r in sweep_steps)
+                                            / max(1, len(sweep_steps))),
            "records": [r.__dict__ for r in recs]}
     print(json.dumps({k: v for k, v in out.items() if k != "records"}), flush=True)
     if a.json:
