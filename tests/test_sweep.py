@@ -223,3 +223,62 @@ def test_sweep_with_svd_truncation_exact():
     e, _ = sweep.run(models.heisenberg(1.0), models.chain(8), lambda N: (0,), 64, [64],
                      OracleSolver(), log=lambda s: None, eig=sweep.SvdDensityEig(device=False))
     assert abs(e - E_HEIS_L8) < 1e-10
+
+
+def _product_permutation(P):
+    # Position of each product-basis state in the full Kronecker product kron(B, A) (B's index
+    # major, the order numpy.kron uses for kron(B_dense, A_dense)).
+    A, B = P.A, P.B
+    perm = np.empty(A.dim * B.dim, dtype=np.int64)
+    for (ga, gb), (G, o) in P.place.items():
+        na, nb = A.sizes[ga], B.sizes[gb]
+        for i in range(nb):
+            for p in range(na):
+                perm[P.basis.offsets[G] + o + i * na + p] = \
+                    (B.offsets[gb] + i) * A.dim + A.offsets[ga] + p
+    return perm
+
+
+@pytest.mark.parametrize("big_a", [True, False])
+def test_product_kron_matches_numpy_kron(big_a):
+    # Product.kron against numpy.kron of the dense operators, permuted into the product basis:
+    # identity factors on either side, the fermion-parity form of an identity (signA), dense
+    # pairs with sparse entries, and accumulation into an existing operator.
+    from dmrgpp_amd.blocks import Basis, BSOp, Product
+
+    rng = np.random.default_rng(8)
+    big = Basis([(0,), (1,), (2,)], [7, 11, 4])
+    small = Basis([(0,), (1,)], [1, 2])
+    A, B = (big, small) if big_a else (small, big)
+
+    def op(basis):
+        blocks = {}
+        for go, so in enumerate(basis.sizes):
+            for gi, si in enumerate(basis.sizes):
+                if (go + gi) % 2 == 0 or go == gi + 1:
+                    m = rng.standard_normal((so, si))
+                    m[rng.random((so, si)) < 0.4] = 0.0
+                    blocks[(go, gi)] = np.asfortranarray(m)
+        return BSOp(basis, basis, blocks)
+
+    a, b = op(A), op(B)
+    sign = np.array([1.0, -1.0, -1.0][:A.ngroups])
+    P = Product(A, B)
+    perm = _product_permutation(P)
+    Ia, Ib = np.eye(A.dim), np.eye(B.dim)
+    Sa = np.diag(np.repeat(sign, A.sizes))
+
+    def want(Ad, Bd, c=1.0):
+        return c * np.kron(Bd, Ad)[np.ix_(perm, perm)]
+
+    for got, ref in ((P.kron(a, None), want(a.dense(), Ib)),
+                     (P.kron(None, b, 0.5), want(Ia, b.dense(), 0.5)),
+                     (P.kron(a, b, -1.5), want(a.dense(), b.dense(), -1.5)),
+                     (P.kron(None, None, 2.0), want(Ia, Ib, 2.0)),
+                     (P.kron(None, b, signA=sign), want(Sa, b.dense()))):
+        assert np.max(np.abs(got.dense() - ref)) < 1e-13
+    H = P.kron(a, None)
+    P.kron(None, b, acc=H)
+    P.kron(a, b, 0.25, acc=H)
+    ref = want(a.dense(), Ib) + want(Ia, b.dense()) + want(a.dense(), b.dense(), 0.25)
+    assert np.max(np.abs(H.dense() - ref)) < 1e-13
