@@ -203,6 +203,51 @@ int dmk_apply_z(dmk_handle* handle, const double* vin, double* vout)
 	});
 }
 
+int dmk_apply_shared(dmk_handle* handle, const void* vin, void* vout)
+{
+	// Whether a kernel may read / write the vector in place: device memory, and managed memory
+	// (Kokkos::SharedSpace allocations). Pageable and pinned host vectors are copied: the
+	// matvec reads its input several times, which over PCIe would cost more than the copies.
+	auto in_place = [](const void* ptr) {
+		hipPointerAttribute_t a {};
+		if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+			(void)hipGetLastError(); // unregistered host memory: clear the sticky status
+			return false;
+		}
+		return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
+	};
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (!vin || !vout)
+			throw std::invalid_argument("dmk_apply_shared: null vector");
+		const bool    cplx  = handle->is_complex;
+		const size_t  bytes = size_t(cplx ? 2 * handle->nc : p.n()) * sizeof(double);
+		const bool    di    = in_place(vin);
+		const bool    dout  = in_place(vout);
+		double*       tmp   = nullptr; // device copies of the host vectors
+		if (!di || !dout)
+			tmp = static_cast<double*>(dmk::pool_device_alloc(2 * bytes));
+		const double* x = di ? static_cast<const double*>(vin) : tmp;
+		double*       y = dout ? static_cast<double*>(vout) : tmp + bytes / sizeof(double);
+		try {
+			if (!di)
+				DMK_HIP_CHECK(hipMemcpyAsync(tmp, vin, bytes, hipMemcpyHostToDevice, p.stream()));
+			const int st = cplx ? dmk_apply_z(handle, x, y) : dmk_apply_d(handle, x, y);
+			if (st != DMK_OK)
+				throw std::runtime_error(dmk::last_error());
+			if (!dout)
+				DMK_HIP_CHECK(hipMemcpyAsync(vout, y, bytes, hipMemcpyDeviceToHost, p.stream()));
+			DMK_HIP_CHECK(hipStreamSynchronize(p.stream()));
+		} catch (...) {
+			(void)hipStreamSynchronize(p.stream());
+			dmk::pool_device_free(tmp);
+			throw;
+		}
+		dmk::pool_device_free(tmp);
+		return DMK_OK;
+	});
+}
+
 int dmk_set_sector_map(dmk_handle* handle, const int64_t* map)
 {
 	return guarded([&] {

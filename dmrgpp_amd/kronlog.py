@@ -34,7 +34,8 @@ def _split(line: str):
 
 def read(path: str):
     """Returns (y, entries); entries are dicts with nc, np, out, in, ic, applyAdjoint,
-    performTranspose, offsetX, offsetY and the A / B blocks as numpy arrays."""
+    performTranspose, offsetX, offsetY, the A / B blocks as numpy arrays and the format each was
+    logged in (A_format, B_format: "dense" or "sparse")."""
     with open(path) as fh:
         lines = [ln.rstrip("\n") for ln in fh]
     i, y, entries = 0, None, []
@@ -75,7 +76,7 @@ def read(path: str):
                 mat[r, c] = v
         else:
             raise ValueError(f"{path}:{i}: unknown block format {kind!r}")
-        return i, mat
+        return i, mat, kind
 
     while i < len(lines):
         ln = lines[i].strip()
@@ -87,11 +88,13 @@ def read(path: str):
         if ln.startswith("# nc, total"):
             f = [int(t) for t in _split(lines[i + 1])]
             i += 2
-            i, a = block(i)
-            i, b = block(i)
+            if len(f) != 9:
+                raise ValueError(f"{path}:{i + 2}: expected 9 integers, got {len(f)}")
+            i, a, fa = block(i)
+            i, b, fb = block(i)
             entries.append({"nc": f[0], "np": f[1], "out": f[2], "in": f[3], "ic": f[4],
                             "applyAdjoint": f[5], "performTranspose": f[6], "offsetX": f[7],
-                            "offsetY": f[8], "A": a, "B": b})
+                            "offsetY": f[8], "A": a, "B": b, "A_format": fa, "B_format": fb})
             continue
         i += 1
     return y, entries

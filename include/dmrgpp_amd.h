@@ -129,6 +129,14 @@ int dmk_apply_host_d(dmk_handle* handle, const double* vin, double* vout);
  * (real H times a complex vector: real and imaginary parts go through one two-column launch). */
 int dmk_apply_z(dmk_handle* handle, const double* vin, double* vout);
 
+/* The apply_Htarget(Kokkos::View<const KokkosScalar*, Kokkos::SharedSpace> vin,
+ * Kokkos::View<KokkosScalar*, Kokkos::SharedSpace> vout) of BatchedGemm.h:30-31 on the views'
+ * data pointers: vout = H vin with vin and vout in any memory — device, managed
+ * (Kokkos::SharedSpace is hipMallocManaged memory on ROCm) or host. Device and managed vectors
+ * are read and written in place; host vectors go through device copies. Real or complex by the
+ * handle (n or 2 n_complex doubles). Synchronous: returns after vout is written. */
+int dmk_apply_shared(dmk_handle* handle, const void* vin, void* vout);
+
 /* Sector ordering of the caller's vectors: map[ip] (host, n entries, a permutation of 0..n-1)
  * is the sector index of patch-order element ip, i.e. permutationInverse[i + j*nl] - offset for
  * the ip = vstart[patch] + iright + ileft*sizeRight of InitKronHamiltonian::copyIn

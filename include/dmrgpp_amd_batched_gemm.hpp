@@ -4,10 +4,15 @@
 // (-DDMRG_BATCHED_GEMM_TYPE=Plugin) can use it in place of dmrg/GPUPlugin/BatchedGemm.{h,cpp}:
 // the constructor takes the same vectors, apply_Htarget computes vout = H vin (beta = 0).
 //
+// apply_Htarget takes the reference's Kokkos::View<const KokkosScalar*, Kokkos::SharedSpace> /
+// Kokkos::View<KokkosScalar*, Kokkos::SharedSpace> pair unchanged (any view type with data() and
+// extent(0): managed, device or host memory, see dmk_apply_shared), so
+// BatchedGemmPluginSc::matrixVector (BatchedGemmPluginSc.h:139-161) compiles as it is; the header
+// BatchedGemm.h beside this one gives the class the reference's global name.
+//
 // Differences from the reference class, by design:
-//  * apply_Htarget takes plain pointers (device pointers, or host pointers with
-//    apply_Htarget_host) instead of Kokkos::View<..., SharedSpace>; a Kokkos caller passes
-//    view.data();
+//  * two pointer overloads as well: apply_Htarget(const T*, T*) on device vectors and
+//    apply_Htarget_host on host vectors;
 //  * failures throw std::runtime_error carrying dmk_last_error() (the reference asserts);
 //  * T = std::complex<double> runs through dmk_create_z / dmk_apply_z (the real embedding of the
 //    complex blocks on the device).
@@ -21,6 +26,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace dmrgpp_amd {
@@ -68,6 +74,29 @@ public:
 		check(dmk_sync(h_), "dmk_sync");
 	}
 
+	// The reference signature (BatchedGemm.h:30-31): vout = H vin on the views' memory (a
+	// Kokkos::View in SharedSpace, or any view type with data() and extent(0) whose element type
+	// has the layout of T, such as Kokkos::complex<double> for std::complex<double>); returns
+	// when vout is written. Views shorter than the superblock vector are rejected.
+	template <class VI, class VO, class = decltype(std::declval<const VI&>().data()),
+	          class = decltype(std::declval<const VO&>().data()),
+	          class = decltype(std::declval<const VI&>().extent(0)),
+	          class = decltype(std::declval<const VO&>().extent(0))>
+	void apply_Htarget(const VI& vin, const VO& vout)
+	{
+		static_assert(sizeof(*vin.data()) == sizeof(T) && sizeof(*vout.data()) == sizeof(T),
+		              "apply_Htarget: view element type must have the size of T");
+		if (size_t(vin.extent(0)) < n_ || size_t(vout.extent(0)) < n_)
+			throw std::runtime_error("dmrgpp_amd::BatchedGemm::apply_Htarget: view of "
+			                         + std::to_string(size_t(vin.extent(0))) + " / "
+			                         + std::to_string(size_t(vout.extent(0)))
+			                         + " elements for a superblock vector of "
+			                         + std::to_string(n_));
+		check(dmk_apply_shared(h_, static_cast<const void*>(vin.data()),
+		                       static_cast<void*>(vout.data())),
+		      "dmk_apply_shared");
+	}
+
 	// Same on host vectors (copies in and out).
 	void apply_Htarget_host(const T* vin, T* vout)
 	{
@@ -101,6 +130,7 @@ public:
 	void set_precision(int precision) { check(dmk_set_precision(h_, precision), "dmk_set_precision"); }
 
 	dmk_handle* handle() const { return h_; }
+	std::size_t size() const { return n_; } // superblock vector length in elements of T
 
 private:
 	static void check(int status, const char* what)

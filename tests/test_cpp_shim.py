@@ -1,5 +1,7 @@
-"""The header-only C++ plugin class (include/dmrgpp_amd_batched_gemm.hpp) compiles against the
-library, and on the GPU passes the restated reference unit test test_BatchedGemm.cpp."""
+"""The header-only C++ plugin class (include/dmrgpp_amd_batched_gemm.hpp) and the drop-in header
+with the reference's name (include/BatchedGemm.h) compile against the library; on the GPU they
+pass the restated reference unit test test_BatchedGemm.cpp and the call pattern of
+BatchedGemmPluginSc (constructor + apply_Htarget on a pair of shared-space views)."""
 import os
 import subprocess
 
@@ -7,7 +9,8 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CPP = os.path.join(HERE, "cpp")
-BIN = os.path.join(CPP, "_build", "test_batched_gemm_shim")
+BINS = {name: os.path.join(CPP, "_build", name)
+        for name in ("test_batched_gemm_shim", "test_plugin_dropin")}
 
 
 def _build():
@@ -16,14 +19,16 @@ def _build():
 
 def test_shim_compiles_and_links():
     _build()
-    assert os.path.exists(BIN)
+    for path in BINS.values():
+        assert os.path.exists(path)
 
 
 @pytest.mark.gpu
-def test_shim_batched_gemm_on_gpu():
-    if not os.path.exists(BIN):
+@pytest.mark.parametrize("name", sorted(BINS))
+def test_shim_on_gpu(name):
+    if not os.path.exists(BINS[name]):
         _build()
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([BINS[name]], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "passed" in r.stdout

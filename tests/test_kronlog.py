@@ -1,4 +1,6 @@
 """The KronLogger dump format reader/writer (dmrgpp_amd.kronlog)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -26,48 +28,61 @@ def test_write_read_round_trip_is_exact(tmp_path, dtype):
     _same(prob, kronlog.to_problem(entries))
 
 
-def _emit_reference_layout(path, d):
-    # The kron_data31 entries in the dump's own layout: dense rows, or MatrixMarket coordinate
-    # for the blocks the reference stored sparse, lower-part entries transposed.
-    with open(path, "w") as fh:
-        fh.write("## KronLogger: Hello from ctor\nVector\n%d\n" % len(d["y"]))
-        for v in d["y"]:
-            fh.write(repr(v) + "\n")
-        for e in d["entries"]:
-            fh.write("\n# nc, total, outPatch, inPatch, ic, applyAdjoint, performTranspose, "
-                     "offsetX, offsetY\n")
-            fh.write(" ".join(str(e[k]) for k in ("nc", "np", "out", "in", "ic", "applyAdjoint",
-                                                   "performTranspose", "offsetX", "offsetY"))
-                     + "\n")
-            for tag in ("A", "B"):
-                blk = e[tag]
-                M = np.array(blk["data"]).reshape((blk["rows"], blk["cols"]), order="F")
-                fh.write(f"## {tag}: {blk['format']}\n")
-                if blk["format"] == "dense":
-                    fh.write(f"{M.shape[0]} {M.shape[1]}\n")
-                    for r in range(M.shape[0]):
-                        fh.write(" ".join(repr(float(v)) for v in M[r]) + "\n")
-                else:
-                    nz = np.argwhere(M != 0)
-                    fh.write("%%MatrixMarket matrix coordinate real general\n")
-                    fh.write(f"{M.shape[0]} {M.shape[1]} {len(nz)}\n")
-                    for r, c in nz:
-                        fh.write(f"{r + 1} {c + 1} {float(M[r, c])!r}\n")
+DUMP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kron_data31_0_ref.txt")
 
 
-def test_reads_the_kron31_dump_layout(tmp_path):
+def test_reads_the_reference_dump_bit_exact():
+    # The reference's own dump TestSuite/inputs/kron_data31_0_ref.txt (KronLogger.hh:133-174,
+    # written by the input31 run: Heisenberg L=8), kept as a data fixture, read by kronlog.read:
+    # every value equals the independent conversion kron_data31.json (tests/golden/make_kron31.py)
+    # bit for bit, including the dense / sparse (MatrixMarket) format of every block.
     d = fixtures.kron31_entries()
-    path = tmp_path / "kron_data31.txt"
-    _emit_reference_layout(path, d)
-    y, entries = kronlog.read(str(path))
-    assert len(entries) == len(d["entries"]) and y.size == 70
+    y, entries = kronlog.read(DUMP)
+    assert y.size == 70 and len(entries) == len(d["entries"])
+    np.testing.assert_array_equal(y, np.array(d["y"]))
+    for e, want in zip(entries, d["entries"]):
+        for k in ("nc", "np", "out", "in", "ic", "applyAdjoint", "performTranspose", "offsetX",
+                  "offsetY"):
+            assert e[k] == want[k], k
+        for tag in ("A", "B"):
+            blk = want[tag]
+            M = np.array(blk["data"]).reshape((blk["rows"], blk["cols"]), order="F")
+            assert e[f"{tag}_format"] == blk["format"]
+            np.testing.assert_array_equal(e[tag], M)
+    formats = {e["A_format"] for e in entries} | {e["B_format"] for e in entries}
+    assert formats == {"dense", "sparse"}
+    # The replayed problem is the fixture's full problem and the ED-pinned L=8 sector.
     prob = kronlog.to_problem(entries, name="kron31")
-    full, _, y_ref, _ = fixtures.kron31_problem()
+    full, _, _, _ = fixtures.kron31_problem()
     _same(full, prob)
-    np.testing.assert_array_equal(y, y_ref)
-    # The replayed problem is the ED-pinned Heisenberg L=8 sector.
+    off = prob.offsets
+    for e in entries:
+        assert off[e["out"]] == e["offsetX"] and off[e["in"]] == e["offsetY"]
     H = oracle.kron_form(prob)
     assert abs(np.linalg.eigvalsh(0.5 * (H + H.T))[0] - (-3.374932598687897)) < 1e-9
+
+
+@pytest.mark.gpu
+def test_gpu_replays_the_reference_dump():
+    # The dump read by kronlog.read, through the HIP matvec (FP64) against the oracle and the
+    # explicit Kronecker form, and its ground state by the device Lanczos against ED.
+    import dmrgpp_amd as dm
+
+    y, entries = kronlog.read(DUMP)
+    prob = kronlog.to_problem(entries, name="kron31")
+    want = oracle.plugin_apply(prob, y)
+    np.testing.assert_allclose(want, oracle.kron_form(prob) @ y, atol=1e-13)
+    op = dm.BatchedGemm(prob, "fp64")
+    dx, dy = dm.DeviceArray.from_host(y), dm.DeviceArray(prob.n)
+    op.apply_Htarget(dx, dy)
+    op.sync()
+    got = dy.download()
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+    init = dm.DeviceArray.from_host(np.random.default_rng(1234).uniform(-1, 1, prob.n))
+    dz = dm.DeviceArray(prob.n)
+    r = op.lanczos_ground(init, dz)
+    assert abs(r["energy"] - (-3.374932598687897)) < 1e-10 * 3.4
+    op.close()
 
 
 def test_reader_rejects_adjoint_and_malformed(tmp_path):

@@ -215,3 +215,104 @@ def test_comb_patch_sizes_follow_state_counting():
     assert max(L) == round(70 * 70 / 65536 * 8000) and max(R) == round(70 * 70 / 65536 * 2000)
     assert abs(sum(L) - 8000) < 100 and abs(sum(R) - 2000) < 60
     assert len(L) == len(R) and all(v >= 1 for v in L + R)
+
+
+def _sweep_superblocks(model, lat, m, count, target=(0,)):
+    """The first `count` superblocks the sweep driver hands to the eigensolver (identity
+    factors, CSR-sparse link blocks and dense block Hamiltonians, as DMRG++ produces them)."""
+    from dmrgpp_amd import sweep
+
+    got = []
+
+    class Capture:
+        def __call__(self, prob):
+            got.append(prob)
+            init = np.random.default_rng(1234).uniform(-1, 1, prob.n)
+            e, z, s, _, _ = oracle.lanczos(prob, init)
+            return e, z, {"steps": s, "matvecs": s}
+
+    sweep.run(model, lat, lambda N: target, m, [], Capture(), log=lambda s: None,
+              stop_after=count)
+    return got
+
+
+def test_kron_numpy_matches_oracle_with_identity_and_sparse_factors():
+    # The timed CPU baseline (oracle/kron_numpy.py) against the C++ restatement of the plugin and
+    # of KronConnections, and against the explicit Kronecker form, on sweep-driver superblocks:
+    # connection 0 and 1 carry identity factors, the link operators of the enlarged blocks fall
+    # under the 0.2 density threshold and go through the CSR branches.
+    from dmrgpp_amd import models
+    from oracle import kron_numpy
+
+    probs = _sweep_superblocks(models.heisenberg(), models.ladder(4, 2), 24, 3)
+    probs += _sweep_superblocks(models.hubbard(4.0), models.chain(6), 32, 2, target=(3, 3))
+    seen = {"eye": 0, "csr": 0, "dense": 0}
+    for prob in probs:
+        prep = kron_numpy.prepare(prob)
+        for terms in prep.values():
+            for _, fa, fb, _ in terms:
+                seen[fa.kind] += 1
+                seen[fb.kind] += 1
+        x = np.random.default_rng(5).uniform(-1, 1, prob.n)
+        want = oracle.plugin_apply(prob, x)
+        got = kron_numpy.apply(prob, prep, x)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+        np.testing.assert_allclose(got, oracle.kron_connections(prob, x),
+                                   atol=1e-12 * np.abs(want).max())
+        if prob.n <= 400:
+            np.testing.assert_allclose(got, oracle.kron_form(prob) @ x,
+                                       atol=1e-12 * np.abs(want).max())
+    assert min(seen.values()) > 0, seen
+
+
+def test_kron_numpy_factor_kinds_follow_the_threshold():
+    # ArrayOfMatStruct.h:307: dense iff nnz >= 0.2 rows cols (so a 5x5 identity is dense and a
+    # 10x10 one CSR); a CSR identity is one GEMM.
+    from oracle import kron_numpy
+    from dmrgpp_amd.problem import KronProblem
+
+    rng = np.random.default_rng(4)
+    a_dense = rng.uniform(-1, 1, (10, 10))
+    a_sparse = np.zeros((10, 10))
+    a_sparse[1, 3] = 2.0
+    a_sparse[4, 0] = -1.0  # 2 of 100 < 0.2 * 100
+    eye = np.eye(10)
+    b = rng.uniform(-1, 1, (3, 3))
+    blocks = {(0, 0, 0): (a_dense, b), (0, 0, 1): (a_sparse, b), (0, 0, 2): (eye, b),
+              (0, 0, 3): (a_dense, np.eye(3))}  # a 3x3 identity is dense (3 >= 1.8)
+    prob = KronProblem(L=[10], R=[3], nC=4, blocks=blocks)
+    prep = kron_numpy.prepare(prob)
+    kinds = [(fa.kind, fb.kind) for _, fa, fb, _ in prep[0]]
+    assert kinds == [("dense", "dense"), ("csr", "dense"), ("eye", "dense"), ("dense", "dense")]
+    small = kron_numpy.prepare(KronProblem(L=[5], R=[3], nC=1,
+                                           blocks={(0, 0, 0): (np.eye(5), b)}))
+    assert small[0][0][1].kind == "dense"
+    x = rng.uniform(-1, 1, prob.n)
+    want = sum(np.kron(a, bb) for a, bb in blocks.values()) @ x
+    np.testing.assert_allclose(kron_numpy.apply(prob, prep, x), want, atol=1e-13)
+    # A planted mistake (the identity term dropped) must fail the same comparison.
+    bad = {k: v for k, v in blocks.items() if k != (0, 0, 2)}
+    bprob = KronProblem(L=[10], R=[3], nC=4, blocks=bad)
+    wrong = kron_numpy.apply(bprob, kron_numpy.prepare(bprob), x)
+    assert np.abs(wrong - want).max() > 1e-3
+
+
+def test_kron_numpy_lanczos_matches_oracle_lanczos():
+    # The full-size FP64 oracle energy (numpy matvec + LanczosCore recurrence) against the C++
+    # restatement's Lanczos and against ED on the kron31 dump (Heisenberg L=8, -3.374932599).
+    from oracle import kron_numpy
+
+    full, _, _, _ = fixtures.kron31_problem()
+    init = np.random.default_rng(1234).uniform(-1, 1, full.n)
+    e_np, steps_np, mv = kron_numpy.lanczos(full, kron_numpy.prepare(full), init)
+    e_c, _, steps_c, _, _ = oracle.lanczos(full, init)
+    assert abs(e_np - e_c) < 1e-12 * abs(e_c)
+    assert steps_np == steps_c and mv == steps_np
+    assert abs(e_np - (-3.374932598687897)) < 1e-10
+    from dmrgpp_amd import models
+
+    for prob in _sweep_superblocks(models.heisenberg(), models.ladder(4, 2), 24, 3)[1:]:
+        init = np.random.default_rng(7).uniform(-1, 1, prob.n)
+        e_np, s_np, _ = kron_numpy.lanczos(prob, kron_numpy.prepare(prob), init)
+        e_c, _, s_c, _, _ = oracle.lanczos(prob, init)
+        assert abs(e_np - e_c) < 1e-12 * abs(e_c) and s_np == s_c
