@@ -193,6 +193,7 @@ SIGNATURES = {
     "dmk_memcpy_h2d": (_I, [_VP, _VP, _SZ]),
     "dmk_memcpy_d2h": (_I, [_VP, _VP, _SZ]),
     "dmk_device_count": (_I, [ctypes.POINTER(_I)]),
+    "dmk_pool_counters": (_I, [_I, _VP]),
     "dmk_device_copy_d": (_I, [_VP, _VP, _SZ]),
     "dmk_last_error": (ctypes.c_char_p, []),
 }
@@ -224,6 +225,16 @@ def device_count() -> int:
     c = ctypes.c_int(0)
     _check(lib().dmk_device_count(ctypes.byref(c)), "dmk_device_count")
     return c.value
+
+
+def pool_counters(host: bool = False) -> dict:
+    """Books of the device (or pinned host) allocation cache: bytes in use / cached, allocator
+    calls, buffers given back, cache hits, and releases of pointers the cache did not hold
+    (0 in a correct run)."""
+    out = np.zeros(6, dtype=np.uint64)
+    _check(lib().dmk_pool_counters(1 if host else 0, out.ctypes.data), "dmk_pool_counters")
+    keys = ("used_bytes", "cached_bytes", "allocs", "frees", "reuses", "bad_releases")
+    return {k: int(v) for k, v in zip(keys, out)}
 
 
 def tridiag_lowest(a, b):

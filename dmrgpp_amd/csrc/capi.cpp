@@ -210,10 +210,12 @@ int dmk_apply_shared(dmk_handle* handle, const void* vin, void* vout)
 	// matvec reads its input several times, which over PCIe would cost more than the copies.
 	auto in_place = [](const void* ptr) {
 		hipPointerAttribute_t a {};
-		if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
-			(void)hipGetLastError(); // unregistered host memory: clear the sticky status
+		const hipError_t      e = hipPointerGetAttributes(&a, ptr);
+		// Unregistered host memory makes the query fail; its status must not reach the launch
+		// checks of this call.
+		(void)hipGetLastError();
+		if (e != hipSuccess)
 			return false;
-		}
 		return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
 	};
 	return guarded([&] {
@@ -828,6 +830,22 @@ int dmk_memcpy_d2h(void* dst, const void* src, size_t bytes)
 {
 	return guarded([&] {
 		DMK_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+		return DMK_OK;
+	});
+}
+
+int dmk_pool_counters(int host, uint64_t* out)
+{
+	return guarded([&] {
+		if (!out)
+			throw std::invalid_argument("dmk_pool_counters: null pointer");
+		const dmk::PoolCounters c = host ? dmk::pool_host_counters() : dmk::pool_device_counters();
+		out[0] = c.used_bytes;
+		out[1] = c.cached_bytes;
+		out[2] = c.allocs;
+		out[3] = c.frees;
+		out[4] = c.reuses;
+		out[5] = c.bad_releases;
 		return DMK_OK;
 	});
 }

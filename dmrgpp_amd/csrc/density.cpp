@@ -103,7 +103,7 @@ namespace {
 // Runs one(ctx, b, region) for the blocks in `order` (largest first) on up to kSolverStreams
 // host threads, each with its own context; region is the block's part of one device buffer
 // (offsets doff, `total` bytes). Every stream is drained before the buffer goes back to the
-// pool; the error of the lowest-numbered failing block is the one reported.
+// pool; the error of the lowest-numbered failing block is the one reported (else a worker's).
 template <typename Fn>
 void for_blocks_concurrently(const char* what, int nblocks, const std::vector<int>& order,
                              const std::vector<size_t>& doff, size_t total, Fn one)
@@ -117,11 +117,14 @@ void for_blocks_concurrently(const char* what, int nblocks, const std::vector<in
 
 	std::atomic<size_t>             next { 0 };
 	std::atomic<bool>               failed { false };
+	// One slot per block (each block is run by one worker) plus one per worker for the
+	// worker's own set-up failure: no slot has two writers.
 	std::vector<std::exception_ptr> errs(static_cast<size_t>(nblocks), nullptr);
+	std::vector<std::exception_ptr> werrs(static_cast<size_t>(nw), nullptr);
 	auto work = [&](int w) {
 		if (w > 0 && hipSetDevice(dev) != hipSuccess) {
-			failed = true;
-			errs[size_t(order[0])] = std::make_exception_ptr(
+			failed   = true;
+			werrs[size_t(w)] = std::make_exception_ptr(
 			    std::runtime_error(std::string(what) + ": hipSetDevice failed in a worker"));
 			return;
 		}
@@ -148,6 +151,9 @@ void for_blocks_concurrently(const char* what, int nblocks, const std::vector<in
 		t.join();
 	pool_device_free(buf);
 	for (const auto& e : errs)
+		if (e)
+			std::rethrow_exception(e);
+	for (const auto& e : werrs)
 		if (e)
 			std::rethrow_exception(e);
 }

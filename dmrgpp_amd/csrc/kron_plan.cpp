@@ -919,8 +919,8 @@ void Plan::retune(const Tuning& t)
 			chunks_.push_back(ReduceChunk { int32_t(r),
 			                                int32_t(std::min<int64_t>(kReduceChunk, reduce_[r].count - s0)),
 			                                s0 });
-	pool_device_free(d_reduce_);
-	pool_device_free(d_chunks_);
+	pool_device_release(d_reduce_);
+	pool_device_release(d_chunks_);
 	d_reduce_ = dev_alloc_copy(reduce_);
 	d_chunks_ = dev_alloc_copy(chunks_);
 	st1_.upload();
@@ -928,8 +928,7 @@ void Plan::retune(const Tuning& t)
 	stats_.stage1_tiles = int64_t(st1_.tiles.size());
 	stats_.stage2_tiles = int64_t(st2_.tiles.size());
 	ws_bytes_           = 0; // force the workspace re-check
-	pool_device_free(d_ws_);
-	d_ws_ = nullptr;
+	pool_device_release(d_ws_);
 	ensure_scratch(1);
 }
 

@@ -208,14 +208,31 @@ Decomp decompose(Plan& plan, LanczosWorkspace& ws, const double* init, int max_s
 			double* v1 = ws.ring(p);
 			double* v2 = ws.ring((p + 1) % 3);
 			DMK_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-			plan.apply(v1, n, v2, n, 1, 0);
-			launch_dot_at(v1, v2, n, part, A, ws.jdev(), 0, s);
-			launch_lanczos_update_at(v2, v1, v0, A, B, ws.jdev(), n, part, NRM, s);
-			launch_finish_step(v2, ws.basis(0), n, NRM, ws.jdev(), s);
-			launch_step_scalars(A, B, NRM, ws.jdev(), maxn, report, s);
-			(void)hipMemcpyAsync(host, report, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
-			hipError_t ec = hipStreamEndCapture(s, &sg.g[p]);
+			// Every call recorded into the capture is checked, and the capture is ended on every
+			// path: a failure inside it (a launch error, a copy the runtime refuses, an exception
+			// of the matvec) must not leave the stream capturing or a partial graph behind.
+			hipError_t ec = hipSuccess;
+			try {
+				plan.apply(v1, n, v2, n, 1, 0);
+				launch_dot_at(v1, v2, n, part, A, ws.jdev(), 0, s);
+				launch_lanczos_update_at(v2, v1, v0, A, B, ws.jdev(), n, part, NRM, s);
+				launch_finish_step(v2, ws.basis(0), n, NRM, ws.jdev(), s);
+				launch_step_scalars(A, B, NRM, ws.jdev(), maxn, report, s);
+				ec = hipGetLastError();
+				if (ec == hipSuccess)
+					ec = hipMemcpyAsync(host, report, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+			} catch (...) {
+				hipGraph_t partial = nullptr;
+				(void)hipStreamEndCapture(s, &partial);
+				if (partial)
+					(void)hipGraphDestroy(partial);
+				throw;
+			}
+			const hipError_t ee = hipStreamEndCapture(s, &sg.g[p]);
 			DMK_HIP_CHECK(ec);
+			DMK_HIP_CHECK(ee);
+			if (!sg.g[p])
+				throw std::runtime_error("lanczos: stream capture produced no graph");
 			DMK_HIP_CHECK(hipGraphInstantiate(&sg.e[p], sg.g[p], nullptr, nullptr, 0));
 		}
 		const auto t0 = std::chrono::steady_clock::now();

@@ -419,6 +419,13 @@ int dmk_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int dmk_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int dmk_device_count(int* count);
 
+/* Books of the library's allocation caches (no reference counterpart; diagnostics): for the
+ * device pool (host = 0) or the pinned host pool (host = 1), out[0..6) = bytes in use, bytes
+ * cached, allocator calls, buffers returned to the allocator, requests served from the cache,
+ * and releases of pointers the cache did not hold as in use (a stale pointer in the library:
+ * 0 in a correct run). Host-only bookkeeping: no GPU call. */
+int dmk_pool_counters(int host, uint64_t* out);
+
 /* dst[i] = src[i], i < n, by a plain grid-stride copy kernel on the default stream (synchronous).
  * No reference counterpart: it calibrates HBM byte counters (FETCH_SIZE / WRITE_SIZE) against a
  * kernel of known traffic. */

@@ -63,8 +63,15 @@ void run(BatchedGemm<T>& bg, const char* kind, const std::vector<T>& vin, std::v
 		++failures;
 		return;
 	}
+	// hipMemset is for device-accessible memory only: on a plain host array it fails and leaves
+	// a sticky error for the next hipGetLastError of the process.
 	(void)hipMemcpy(di, vin.data(), n * sizeof(T), hipMemcpyDefault);
-	(void)hipMemset(dout, 0, n * sizeof(T));
+	if (!std::strcmp(kind, "host"))
+		std::memset(static_cast<void*>(dout), 0, n * sizeof(T));
+	else if (hipMemset(dout, 0, n * sizeof(T)) != hipSuccess) {
+		std::printf("%s: hipMemset failed\n", kind);
+		++failures;
+	}
 	SharedView<const T> vin_view { di, n };
 	SharedView<T>       vout_view { dout, n };
 	bg.apply_Htarget(vin_view, vout_view);
