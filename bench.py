@@ -8,14 +8,16 @@ right block e+E of 2m states each) is the matvec workload. Untimed setup.
 A step is one matvec y = H x with x and y resident in HBM. value = matvecs per second of the
 FP32-MFMA matvec (FP32 accumulation), the mode the mixed-precision eigensolver uses, provided
 the ground state of the same superblock by FP32 Lanczos + FP64 refinement matches the FP64
-Lanczos energy to |dE| < 1e-10 relative (the gate is reported; without it the line says so).
+oracle energy (the CPU Lanczos recurrence over the CPU Kron matvec, same start vector) to
+|dE| < 1e-10, absolute and relative (the gate is reported; without it the line says so).
 Beside it: the FP64 and BF16x3 matvec rates with their errors against FP64; the ground state in
-all three modes; a bounded finite-sweep segment (steps started from the transformed previous
-wave function) in each mode with the estimated full-sweep time; the roofline of the dominant
-grouped-GEMM launch of each mode from HIP events on its stream; and the CPU baseline
-(KronConnections restatement with numpy BLAS) on a bounded number of matvecs.
+all modes against the oracle energy; one whole finite sweep (every step started from the
+transformed previous wave function) in FP64 and in the mixed modes, timed; the roofline of the
+dominant grouped-GEMM launch of each mode from HIP events on its stream; and the CPU baseline
+(the reference's CPU Kron path restated with host BLAS and CSR factors) on a bounded number of
+matvecs, with its y against the GPU FP64 y.
 
-Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--sweep-steps 6] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--sweep-steps -1] [--no-cpu]
                        [--save-problem F.npz | --load-problem F.npz]
 """
 import argparse
@@ -29,7 +31,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 157.3}  # bf16x3: FP32-equivalent work
+# MI355X dense peaks: FP64 MFMA, FP32 MFMA, bf16 MFMA. A BF16x3 launch runs three bf16 MFMAs per
+# product (hi*hi + hi*lo + lo*hi), so its roofline counts 3x the algorithmic flops of the
+# product against the bf16 peak.
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+MFMA_WORK = {"fp64": 1, "fp32": 1, "bf16x3": 3}
 M = 2048
 LX = 32
 # HBM bytes per launch of the dominant kernel (stage 2) from rocprofv3 --pmc FETCH_SIZE and
@@ -70,17 +76,16 @@ def host_cpu_model():
     return "unknown"
 
 
-def cpu_baseline(prob, x, budget_s=12.0, max_calls=200):
+def cpu_baseline(prob, x, y_gpu64, budget_s=12.0, max_calls=200):
+    """The reference's CPU Kron path (oracle/kron_numpy: KronConnections with dense / CSR /
+    identity factors as ArrayOfMatStruct stores them) timed on this host on a bounded number
+    of matvecs of the bench superblock; its y is compared with the GPU FP64 matvec's."""
     from oracle import kron_numpy
 
-    try:
-        from threadpoolctl import threadpool_info
-
-        threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
-    except Exception:
-        threads = os.cpu_count() or 1
+    threads = kron_numpy.blas_threads()
     prep = kron_numpy.prepare(prob)
-    kron_numpy.apply(prob, prep, x)  # warm-up
+    y_cpu = kron_numpy.apply(prob, prep, x)  # warm-up, and the parity check of the GPU y
+    err = float(np.linalg.norm(y_gpu64 - y_cpu) / np.linalg.norm(y_cpu))
     t0 = time.perf_counter()
     calls = 0
     while calls < max_calls:
@@ -91,8 +96,11 @@ def cpu_baseline(prob, x, budget_s=12.0, max_calls=200):
     dt = time.perf_counter() - t0
     return {"value": round(calls / dt, 4), "unit": "matvecs/s", "cores": int(threads),
             "kind": "port", "host": f"{host_cpu_model()}, {os.cpu_count()} logical CPUs",
+            "blas": kron_numpy.blas_library(),
+            "gpu_fp64_rel_err_vs_cpu": err,
             "sample": f"{calls} matvecs of the same m=2048 ladder superblock (KronConnections "
-                      f"restatement, numpy BLAS GEMMs, {threads} BLAS threads), {dt:.1f} s"}
+                      f"restatement: dense blocks in BLAS GEMMs, sparse and identity factors as "
+                      f"CSR, {threads} BLAS threads), {dt:.1f} s"}, prep
 
 
 def save_problem(prob, path):
@@ -127,8 +135,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--sweep-steps", type=int, default=6,
-                    help="finite-sweep DMRG steps timed per precision mode (0 = skip)")
+    ap.add_argument("--sweep-steps", type=int, default=-1,
+                    help="finite-sweep DMRG steps timed per precision mode (-1 = one whole "
+                         "finite sweep, 0 = skip)")
+    ap.add_argument("--no-oracle-energy", action="store_true",
+                    help="skip the FP64 oracle Lanczos of the bench superblock on the CPU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--save-problem", default=None, help="write the superblock to this .npz")
     ap.add_argument("--no-autotune", action="store_true",
@@ -240,56 +251,56 @@ def main():
         log(f"lanczos {mode}: E={r['energy']:.14f} steps={r['steps']} matvecs={r['matvecs']} "
             f"first={r['matvecs_first']} residual={r['residual']:.2e} {r['ms_total']:.1f} ms")
     e64 = lz["fp64"]["energy"]
-    dE = {m: abs(lz[m]["energy"] - e64) for m in lz}
     op.close()
 
-    # ---- bounded finite-sweep segment in both modes, continuing from the grown blocks
+    # ---- one finite sweep per mode (or a bounded segment), continuing from the grown blocks
     sweep_out = None
-    if args.sweep_steps > 0 and state:
+    if args.sweep_steps != 0 and state:
         sweep_out = {}
-        steps_per_sweep = 2 * (lattice.nsites - 3)
-        # Every step of these segments starts from the transformed wave function; the mixed
-        # modes run those solves in their own precision here (mixed_warm), to time them.
-        # The truncation diagonalises density-matrix blocks of 512 rows or more on the GPU
-        # while the host LAPACK does the smaller ones; the "fp64 host-trunc" segment keeps
-        # every block on the host for comparison.
+        stop = None if args.sweep_steps < 0 else args.sweep_steps
+        # Every step of a finite sweep starts from the transformed wave function; the mixed
+        # modes run those solves in their own precision (mixed_warm) with the FP64 Davidson
+        # refinement. The truncation diagonalises density-matrix blocks of 512 rows or more on
+        # the GPU while the host LAPACK does the smaller ones.
         mk = lambda p, r: sweep.GpuSolver(precision=p, refine=r, mixed_warm=True)  # noqa: E731
         gpu_eig = sweep.GpuDensityEig(512)
         # rocSOLVER loads its kernels on the first call of each size class, and each of the
         # solver's four streams sizes its workspace on its first block: not timed.
         w = np.random.default_rng(0).standard_normal((1280, 1280))
         dm.density_diag([w + w.T] * 4 + [np.eye(64) + 0.5])
-        for mode, solver, eig in (("fp64", sweep.GpuSolver(precision="fp64"), gpu_eig),
-                                  ("fp64 host-trunc", sweep.GpuSolver(precision="fp64"), None),
-                                  ("fp32+refine", mk("fp32", 1), gpu_eig),
-                                  ("bf16x3+refine", mk("bf16x3", 1), gpu_eig),
-                                  ("fp32+davidson", mk("fp32", 2), gpu_eig),
-                                  ("bf16x3+davidson", mk("bf16x3", 2), gpu_eig)):
+        for mode, solver in (("fp64", sweep.GpuSolver(precision="fp64")),
+                             ("fp32+davidson", mk("fp32", 2)),
+                             ("bf16x3+davidson", mk("bf16x3", 2))):
             st = sweep.clone_state(state)
             t = time.perf_counter()
             e, recs = sweep.run(model, lattice, lambda N: (0,), M, [M], solver, log=log,
-                                state=st, resume=True, stop_after=args.sweep_steps, wft=True,
-                                eig=eig)
+                                state=st, resume=True, stop_after=stop, wft=True, eig=gpu_eig)
             dt = time.perf_counter() - t
             per = dt / max(1, len(recs))
             lz_ms = sum(r.ms_lanczos for r in recs) / max(1, len(recs))
             tr_ms = sum(r.info["timing"]["truncate_ms"] for r in recs) / max(1, len(recs))
             parts = {k: round(sum(r.info["timing"].get("truncate_" + k, 0.0) for r in recs)
                               / max(1, len(recs)), 1) for k in ("rho_ms", "eig_ms", "basis_ms")}
-            sweep_out[mode] = {"steps": len(recs), "s_per_step": round(per, 4),
+            sweep_out[mode] = {"steps": len(recs), "whole_sweep": stop is None,
+                               "sweep_s": round(dt, 2), "s_per_step": round(per, 4),
                                "eigensolve_ms_per_step": round(lz_ms, 1),
+                               "eigensolve_s": round(sum(r.ms_lanczos for r in recs) / 1e3, 2),
                                "truncate_ms_per_step": round(tr_ms, 1),
                                "truncate_parts_ms_per_step": parts,
-                               "sweep_s_est": round(per * steps_per_sweep, 2),
                                "lanczos_matvecs": int(sum(r.matvecs for r in recs)),
-                               "energy_last": e}
-            log(f"sweep segment {mode}: {len(recs)} steps, {per:.3f} s/step "
-                f"(eigensolve {lz_ms:.1f} ms/step, truncation {tr_ms:.1f} ms/step {parts}), "
-                f"full sweep ~{per * steps_per_sweep:.1f} s")
-        for mode in ("fp32+refine", "bf16x3+refine", "fp32+davidson", "bf16x3+davidson"):
-            sweep_out[mode]["eigensolve_speedup_vs_fp64"] = round(
+                               "energy_last": e,
+                               "energy_min": min(r.energy for r in recs)}
+            log(f"sweep {mode}: {len(recs)} steps in {dt:.2f} s, {per:.3f} s/step "
+                f"(eigensolve {lz_ms:.1f} ms/step, truncation {tr_ms:.1f} ms/step {parts})")
+        e_ref = sweep_out["fp64"]["energy_min"]
+        for mode in ("fp32+davidson", "bf16x3+davidson"):
+            so = sweep_out[mode]
+            so["sweep_speedup_vs_fp64"] = round(sweep_out["fp64"]["sweep_s"]
+                                                / max(1e-9, so["sweep_s"]), 3)
+            so["eigensolve_speedup_vs_fp64"] = round(
                 sweep_out["fp64"]["eigensolve_ms_per_step"]
-                / max(1e-9, sweep_out[mode]["eigensolve_ms_per_step"]), 3)
+                / max(1e-9, so["eigensolve_ms_per_step"]), 3)
+            so["rel_dE_min_vs_fp64"] = abs(so["energy_min"] - e_ref) / abs(e_ref)
 
     def roofline_of(prec):
         r = results[prec]
@@ -297,28 +308,46 @@ def main():
             kname, kms, kflops = "stage2", r["ms_stage2"], info["flops_stage2"]
         else:
             kname, kms, kflops = "stage1", r["ms_stage1"], info["flops_stage1"]
-        achieved = kflops / (kms * 1e-3) / 1e12
+        achieved = MFMA_WORK[prec] * kflops / (kms * 1e-3) / 1e12
         kern = {"fp64": "panel_gemm_kernel<double,OpF64>",
                 "fp32": "panel_gemm_kernel<double,OpF32|OpF32x32>",
                 "bf16x3": "panel_bf16x3_kernel"}[prec]
-        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec],
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[prec], 4),
-                "traffic": TRAFFIC.get((prec, shape_key(tunings[prec]))),
-                "kernel": f"{kern} ({kname})",
-                "ms_per_launch": round(kms, 4), "flops_per_launch": kflops}
+        out = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec],
+               "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[prec], 4),
+               "traffic": TRAFFIC.get((prec, shape_key(tunings[prec]))),
+               "kernel": f"{kern} ({kname})",
+               "ms_per_launch": round(kms, 4), "flops_per_launch": kflops}
+        if prec == "bf16x3":
+            out["note"] = ("achieved counts the 3 bf16 MFMA products per multiply-add against "
+                           "the bf16 dense peak; FP32-equivalent rate "
+                           f"{kflops / (kms * 1e-3) / 1e12:.1f} TFLOP/s")
+        return out
 
     r = results["fp32"]
     value = 1e3 / r["ms_per_step"]
     roofline = roofline_of("fp32")
 
-    cpu = None
+    cpu, prep = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(prob, x)
+        cpu, prep = cpu_baseline(prob, x, y64)
         log("cpu baseline:", cpu)
+    # FP64 oracle energy of the bench superblock: the CPU Lanczos recurrence (LanczosCore) over
+    # the CPU Kron matvec from the same start vector; every GPU mode is gated against it.
+    oracle_e = None
+    if not args.no_cpu and not args.no_oracle_energy:
+        from oracle import kron_numpy
 
-    e_rel = {m: dE[m] / abs(e64) for m in dE}
+        t_or = time.perf_counter()
+        eo, steps_o, mv_o = kron_numpy.lanczos(prob, prep, x)
+        oracle_e = {"energy": eo, "steps": steps_o, "matvecs": mv_o,
+                    "s": round(time.perf_counter() - t_or, 1)}
+        log(f"oracle lanczos (CPU, FP64): E={eo:.14f} steps={steps_o} in {oracle_e['s']} s")
+
+    e_ref = oracle_e["energy"] if oracle_e else e64
+    e_rel = {m: abs(lz[m]["energy"] - e_ref) / abs(e_ref) for m in lz}
+    e_abs = {m: abs(lz[m]["energy"] - e_ref) for m in lz}
     line = {
-        "metric": "superblock matvecs/s + sweep time at m=2048, mixed vs FP64 GPU at |dE|<1e-10",
+        "metric": "superblock matvecs/s + sweep time at m=2048, mixed vs FP64 GPU at |\u0394E|<1e-10",
         "value": round(value, 3),
         "unit": "matvecs/s",
         "n_gpus": 1,
@@ -337,8 +366,11 @@ def main():
                    "nterms": len(prob.blocks), "flops_per_matvec": info["flops_algorithmic"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "gate": {"mode": "fp32+refine", "rel_dE": e_rel["fp32+refine"],
-                 "passed": bool(e_rel["fp32+refine"] < 1e-10)},
+        "gate": {"mode": "fp32+refine",
+                 "against": "fp64 oracle (CPU Lanczos)" if oracle_e else "fp64 GPU Lanczos",
+                 "rel_dE": e_rel["fp32+refine"], "abs_dE": e_abs["fp32+refine"],
+                 "passed": bool(e_rel["fp32+refine"] < 1e-10 and e_abs["fp32+refine"] < 1e-10)},
+        "oracle_energy": oracle_e,
         "modes": {prec: {"matvecs_per_s": round(1e3 / results[prec]["ms_per_step"], 3),
                          "ms_per_step": round(results[prec]["ms_per_step"], 4),
                          "rel_err_vs_fp64": results[prec].get("rel_err", 0.0),
@@ -347,7 +379,8 @@ def main():
                                                     / 1e12, 3),
                          "roofline": roofline_of(prec), "tuning": tunings[prec]}
                   for prec in ("fp64", "fp32", "bf16x3")},
-        "ground_state": {m: {"energy": lz[m]["energy"], "rel_dE_vs_fp64": e_rel[m],
+        "ground_state": {m: {"energy": lz[m]["energy"], "rel_dE_vs_oracle": e_rel[m],
+                             "abs_dE_vs_oracle": e_abs[m],
                              "gate_1e-10": bool(e_rel[m] < 1e-10),
                              "ms": round(lz[m]["ms_total"], 2), "matvecs": lz[m]["matvecs"],
                              "matvecs_first": lz[m]["matvecs_first"],
