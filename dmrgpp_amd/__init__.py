@@ -148,6 +148,8 @@ SIGNATURES = {
     "dmk_apply_d": (_I, [_VP, _VP, _VP]),
     "dmk_apply_host_d": (_I, [_VP, _VP, _VP]),
     "dmk_apply_shared": (_I, [_VP, _VP, _VP]),
+    "dmk_apply_noalloc_d": (_I, [_VP, _VP, _VP]),
+    "dmk_noalloc_scratch_bytes": (_I, [_VP, ctypes.POINTER(_SZ)]),
     "dmk_apply_multi_d": (_I, [_VP, _I, _VP, _SZ, _VP, _SZ, _I]),
     "dmk_set_sector_map": (_I, [_VP, _VP]),
     "dmk_matvec_sector_d": (_I, [_VP, _VP, _VP]),

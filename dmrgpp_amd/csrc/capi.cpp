@@ -5,6 +5,7 @@
 #include "kron_plan.h"
 #include "krylov.h"
 #include "lanczos.h"
+#include "noalloc_scratch.h"
 #include "pool.h"
 #include "tridiag.h"
 #include "vbatch.h"
@@ -246,6 +247,30 @@ int dmk_apply_shared(dmk_handle* handle, const void* vin, void* vout)
 			throw;
 		}
 		dmk::pool_device_free(tmp);
+		return DMK_OK;
+	});
+}
+
+int dmk_apply_noalloc_d(dmk_handle* handle, const double* vin, double* vout)
+{
+	static_assert(DMK_NOALLOC_SCRATCH_BYTES == dmk::kNoallocScratchBytes,
+	              "header and library disagree on the static scratch size");
+	return guarded([&] {
+		dmk::Plan& p = plan_of(handle);
+		if (handle->is_complex)
+			throw std::invalid_argument("dmk_apply_noalloc_d: complex handle (use dmk_apply_z)");
+		p.apply_on_scratch(vin, vout, dmk::noalloc_scratch_device_ptr(),
+		                   dmk::kNoallocScratchBytes);
+		return DMK_OK;
+	});
+}
+
+int dmk_noalloc_scratch_bytes(const dmk_handle* handle, size_t* bytes)
+{
+	return guarded([&] {
+		if (!handle || !handle->plan || !bytes)
+			throw std::invalid_argument("dmk_noalloc_scratch_bytes: null argument");
+		*bytes = handle->plan->scratch_layout(nullptr, nullptr);
 		return DMK_OK;
 	});
 }

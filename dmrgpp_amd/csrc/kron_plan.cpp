@@ -17,6 +17,8 @@
 //    apply_Htarget_sparse_not_ready.cpp:99-276).
 #include "kron_plan.h"
 
+#include "noalloc_scratch.h"
+
 #include "pool.h"
 
 #include <algorithm>
@@ -1028,7 +1030,7 @@ void Plan::set_precision(int p)
 // Input preparation, stage 1, stage 2 and the split-k reduction on the plan's stream; an event
 // is recorded between the stages when mid is given.
 void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec,
-                  int beta, hipEvent_t mid)
+                  int beta, hipEvent_t mid, void* d_bx, void* d_ws, void* d_xin)
 {
 	const int64_t  bxs = stats_.bx_elems;
 	const int64_t  wss = stats_.ws_elems;
@@ -1037,9 +1039,9 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 	const int      v1  = tuning_.mfma32;
 	const int      dp  = tuning_.depth;
 	if (precision_ == PREC_FP64) {
-		double* bx = static_cast<double*>(d_bx_);
-		double* ws = static_cast<double*>(d_ws_);
-		double* xp = static_cast<double*>(d_xin_);
+		double* bx = static_cast<double*>(d_bx);
+		double* ws = static_cast<double*>(d_ws);
+		double* xp = static_cast<double*>(d_xin);
 		double* xt = xp + int64_t(nvec) * xp_elems_;
 		launch_prep_x<double>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
 		                      stream_);
@@ -1056,11 +1058,11 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		                                     vout, ldout, nvec, beta, stream_);
 	} else if (precision_ == PREC_BF16X3) {
 		// (hi, lo) bf16 arrays: hi parts of all vectors, then the lo parts.
-		uint16_t*     xph = static_cast<uint16_t*>(d_xin_);
+		uint16_t*     xph = static_cast<uint16_t*>(d_xin);
 		uint16_t*     xpl = xph + int64_t(nvec) * xp_elems_;
 		uint16_t*     xth = xpl + int64_t(nvec) * xp_elems_;
 		uint16_t*     xtl = xth + int64_t(nvec) * xt_elems_;
-		uint16_t*     bh  = static_cast<uint16_t*>(d_bx_);
+		uint16_t*     bh  = static_cast<uint16_t*>(d_bx);
 		uint16_t*     bl  = bh + int64_t(nvec) * bxs;
 		const int64_t na = stats_.apack_elems, nb = stats_.bpack_elems;
 		launch_prep_x_bf16x2(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xph, xpl, xth, xtl,
@@ -1073,16 +1075,16 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		                    true, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
-		float* ws = static_cast<float*>(d_ws_);
+		float* ws = static_cast<float*>(d_ws);
 		launch_panel_bf16x3(st2_.bm, st2_.bn, v1, dp, st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
 		                    int(st2_.tiles.size()), hi, lo, vs, vout, ldout, ws, nullptr, nullptr,
 		                    wss, false, nvec, beta, stream_);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                    vout, ldout, nvec, beta, stream_);
 	} else {
-		float* bx = static_cast<float*>(d_bx_);
-		float* ws = static_cast<float*>(d_ws_);
-		float* xp = static_cast<float*>(d_xin_);
+		float* bx = static_cast<float*>(d_bx);
+		float* ws = static_cast<float*>(d_ws);
+		float* xp = static_cast<float*>(d_xin);
 		float* xt = xp + int64_t(nvec) * xp_elems_;
 		launch_prep_x<float>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
 		                     stream_);
@@ -1110,7 +1112,43 @@ void Plan::apply(const double* vin, int64_t ldin, double* vout, int64_t ldout, i
 	if (nvec > 1 && (ldin < stats_.n || ldout < stats_.n))
 		throw std::runtime_error("dmk_apply: vector stride smaller than n");
 	ensure_scratch(nvec);
-	launch(vin, ldin, vout, ldout, nvec, beta, nullptr);
+	launch(vin, ldin, vout, ldout, nvec, beta, nullptr, d_bx_, d_ws_, d_xin_);
+}
+
+size_t Plan::scratch_layout(size_t* off_ws, size_t* off_xin) const
+{
+	// Element sizes as ensure_scratch: FP64 doubles, FP32 floats, BF16x3 (hi, lo) bf16 pairs.
+	const size_t elem  = (precision_ == PREC_FP64) ? sizeof(double) : sizeof(float);
+	auto         up256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+	const size_t bx    = up256(size_t(std::max<int64_t>(stats_.bx_elems, 1)) * elem);
+	const size_t ws    = up256(size_t(std::max<int64_t>(stats_.ws_elems, 1)) * elem);
+	const size_t xin   = up256(size_t(std::max<int64_t>(xp_elems_ + xt_elems_, 1)) * elem);
+	if (off_ws)
+		*off_ws = bx;
+	if (off_xin)
+		*off_xin = bx + ws;
+	return bx + ws + xin;
+}
+
+void Plan::apply_on_scratch(const double* vin, double* vout, void* scratch, size_t bytes)
+{
+	if (!vin || !vout)
+		throw std::invalid_argument("dmk_apply_noalloc_d: null vector");
+	if (!scratch || (reinterpret_cast<uintptr_t>(scratch) & 255) != 0)
+		throw std::invalid_argument("dmk_apply_noalloc_d: scratch must be 256-byte aligned");
+	size_t       off_ws = 0, off_xin = 0;
+	const size_t need   = scratch_layout(&off_ws, &off_xin);
+	if (need > bytes)
+		throw std::invalid_argument("dmk_apply_noalloc_d: this superblock's matvec needs "
+		                            + std::to_string(need) + " bytes of scratch in the current "
+		                            + "precision, more than the " + std::to_string(bytes)
+		                            + " bytes of the static scratch");
+	char* base = static_cast<char*>(scratch);
+	// The stage-2 panels are read in whole 16-byte vectors past the end of each row's K range:
+	// the row padding of the stage-1 region must hold zeros, and another plan may have used
+	// the region since.
+	launch_fill_zero(base, off_ws, stream_);
+	launch(vin, stats_.n, vout, stats_.n, 1, 0, nullptr, base, base + off_ws, base + off_xin);
 }
 
 void Plan::profile(const double* vin, double* vout, int iters, double* ms1, double* ms2,
@@ -1122,7 +1160,7 @@ void Plan::profile(const double* vin, double* vout, int iters, double* ms1, doub
 	ensure_scratch(1);
 	for (int it = 0; it < iters; ++it) {
 		DMK_HIP_CHECK(hipEventRecord(ev_[0], stream_));
-		launch(vin, stats_.n, vout, stats_.n, 1, 0, ev_[1]);
+		launch(vin, stats_.n, vout, stats_.n, 1, 0, ev_[1], d_bx_, d_ws_, d_xin_);
 		DMK_HIP_CHECK(hipEventRecord(ev_[2], stream_));
 		DMK_HIP_CHECK(hipEventSynchronize(ev_[2]));
 		float a = 0, b = 0, c = 0;

@@ -102,6 +102,17 @@ public:
 	// vout = H vin (beta = 0) or vout += H vin (beta = 1) for nvec vectors; device pointers.
 	void apply(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec, int beta);
 
+	// Scratch of one single-vector matvec in the current precision, as three 256-byte aligned
+	// regions of one buffer: stage-1 products at 0, split-k partials at *off_ws, input panels
+	// at *off_xin. Returns the total size in bytes.
+	size_t scratch_layout(size_t* off_ws, size_t* off_xin) const;
+
+	// vout = H vin (one vector, beta = 0) with every scratch buffer carved from the caller's
+	// device memory at `scratch` (`bytes` bytes, 256-byte aligned, at least scratch_layout()):
+	// no allocation, no synchronisation, no hipMemsetAsync (the stage-1 region's zero padding is
+	// restored by a fill kernel on every call). Enqueued on the plan's stream.
+	void apply_on_scratch(const double* vin, double* vout, void* scratch, size_t bytes);
+
 	void set_precision(int p);
 	int  precision() const { return precision_; }
 
@@ -137,8 +148,9 @@ private:
 	void ensure_scratch(int nvec);
 	void ensure_fp32();
 	void ensure_bf16();
+	// The matvec on the given scratch buffers (the plan's own, or a caller's).
 	void launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec, int beta,
-	            hipEvent_t mid);
+	            hipEvent_t mid, void* bx, void* ws, void* xin);
 
 	int         precision_ = PREC_FP64;
 	Tuning      tuning_;

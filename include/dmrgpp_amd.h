@@ -137,6 +137,25 @@ int dmk_apply_z(dmk_handle* handle, const double* vin, double* vout);
  * handle (n or 2 n_complex doubles). Synchronous: returns after vout is written. */
 int dmk_apply_shared(dmk_handle* handle, const void* vin, void* vout);
 
+/* vout = H vin like dmk_apply_d (one real vector, device memory, beta = 0, the handle's current
+ * precision), allocating nothing: no hipMalloc / hipFree and no allocating library call, and no
+ * hipMemsetAsync. No counterpart in the reference; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid allocation). The matvec's
+ * scratch (stage-1 products, split-k partials, input panels) lives in a static device array of
+ * DMK_NOALLOC_SCRATCH_BYTES per GPU, held from the library's first use on that GPU until the
+ * process ends; a handle whose matvec needs more in its current precision is rejected with
+ * DMK_ERR_ARGUMENT before any work is enqueued (dmk_noalloc_scratch_bytes tells). All calls on
+ * one GPU share that array: calls that overlap in time on one GPU (several streams or threads)
+ * race and must be serialised by the caller. Enqueued on the handle's stream; returns before
+ * completion (dmk_sync waits). The handle must have been created, and its precision set, before
+ * (both allocate). */
+#define DMK_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int dmk_apply_noalloc_d(dmk_handle* handle, const double* vin, double* vout);
+
+/* Scratch bytes one matvec of the handle needs in its current precision (what
+ * dmk_apply_noalloc_d compares with DMK_NOALLOC_SCRATCH_BYTES). Host-only. */
+int dmk_noalloc_scratch_bytes(const dmk_handle* handle, size_t* bytes);
+
 /* Sector ordering of the caller's vectors: map[ip] (host, n entries, a permutation of 0..n-1)
  * is the sector index of patch-order element ip, i.e. permutationInverse[i + j*nl] - offset for
  * the ip = vstart[patch] + iright + ileft*sizeRight of InitKronHamiltonian::copyIn
