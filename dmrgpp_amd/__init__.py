@@ -195,6 +195,13 @@ SIGNATURES = {
     "dmk_memcpy_h2d": (_I, [_VP, _VP, _SZ]),
     "dmk_memcpy_d2h": (_I, [_VP, _VP, _SZ]),
     "dmk_device_count": (_I, [ctypes.POINTER(_I)]),
+    "dmk_facade_create_d": (_I, [_SZ, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_int64, _I,
+                                 ctypes.POINTER(_VP)]),
+    "dmk_facade_create_z": (_I, [_SZ, _SZ, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_int64, _I,
+                                 ctypes.POINTER(_VP)]),
+    "dmk_facade_matvec": (_I, [_VP, _VP, _VP]),
+    "dmk_facade_get_info": (_I, [_VP, _VP]),
+    "dmk_facade_destroy": (None, [_VP]),
     "dmk_pool_counters": (_I, [_I, _VP]),
     "dmk_device_copy_d": (_I, [_VP, _VP, _SZ]),
     "dmk_last_error": (ctypes.c_char_p, []),
@@ -621,6 +628,69 @@ class BatchedGemm:
     def close(self):
         if getattr(self, "handle", None):
             lib().dmk_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FacadeInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nnz", ctypes.c_int64), ("matvecs", ctypes.c_int64),
+                ("ms_matvec_total", ctypes.c_double), ("stored", ctypes.c_int32),
+                ("ldaggerl", ctypes.c_int32), ("is_complex", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class Facade:
+    """Mirror of MatrixVectorKron (Engine/MatrixVectorKron/MatrixVectorKron.h:113-166) over the
+    C ABI: matvec(x, y) returns x + op(y) with op = H, or L^dagger L with ldaggerl; H is stored
+    as CSR when the rank is at most max_rank_stored (MaxMatrixRankStored), otherwise the Kron
+    plugin computes it. Host vectors in and out (device copies inside)."""
+
+    def __init__(self, prob: KronProblem, max_rank_stored: int = 0, ldaggerl: bool = False):
+        if prob.dtype == np.float64:
+            create, self.is_complex = lib().dmk_facade_create_d, False
+        elif prob.dtype == np.complex128:
+            create, self.is_complex = lib().dmk_facade_create_z, True
+        else:
+            raise DmkError(f"unsupported scalar type {prob.dtype}")
+        args = prob.plugin_args()
+        h = ctypes.c_void_p()
+        _check(create(args.noperator, args.npatches, ctypes.addressof(args.L),
+                      ctypes.addressof(args.R), ctypes.addressof(args.A),
+                      ctypes.addressof(args.ldA), ctypes.addressof(args.B),
+                      ctypes.addressof(args.ldB), ctypes.c_int64(int(max_rank_stored)),
+                      1 if ldaggerl else 0, ctypes.byref(h)), "dmk_facade_create")
+        self.handle = h.value
+        self.n = prob.n
+
+    def matvec(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+        dt = np.complex128 if self.is_complex else np.float64
+        xv = np.ascontiguousarray(x, dtype=dt)
+        yv = np.ascontiguousarray(y, dtype=dt)
+        dx = DeviceArray.from_host(xv.view(np.float64))
+        dy = DeviceArray.from_host(yv.view(np.float64))
+        try:
+            _check(lib().dmk_facade_matvec(self.handle, dx.ptr, dy.ptr), "dmk_facade_matvec")
+            return dx.download().view(dt)
+        finally:
+            dx.free()
+            dy.free()
+
+    def info(self) -> dict:
+        i = FacadeInfo()
+        _check(lib().dmk_facade_get_info(self.handle, ctypes.byref(i)), "dmk_facade_get_info")
+        return i.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().dmk_facade_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -809,16 +809,23 @@ std::string Plan::histogram_key() const
 
 void Plan::retune(const Tuning& t)
 {
-	// 64 / 128 in every mode; FP64 and FP32 also take 96 x 96, 96 x 64 and 64 x 96.
-	const bool allow96 = precision_ != PREC_BF16X3;
+	// 64 / 128 in every mode; FP64 and FP32 also take 96 x 96, 96 x 64 and 64 x 96, and the
+	// 32-wide tiles 32 x {32, 64, 128} and {64, 128} x 32.
+	const bool allow_more = precision_ != PREC_BF16X3;
 	auto valid = [&](int bm, int bn) {
 		if ((bm == 64 || bm == 128) && (bn == 64 || bn == 128))
 			return true;
-		return allow96 && ((bm == 96 && (bn == 96 || bn == 64)) || (bm == 64 && bn == 96));
+		if (!allow_more)
+			return false;
+		if ((bm == 96 && (bn == 96 || bn == 64)) || (bm == 64 && bn == 96))
+			return true;
+		return (bm == 32 && (bn == 32 || bn == 64 || bn == 128))
+		       || (bn == 32 && (bm == 64 || bm == 128));
 	};
 	if (!valid(t.bm1, t.bn1) || !valid(t.bm2, t.bn2) || t.splitk < 0)
 		throw std::invalid_argument("dmk tuning: tile shapes must be 64 or 128 (FP64 / FP32 "
-		                            "also 96x96, 96x64, 64x96), splitk >= 0");
+		                            "also 96x96, 96x64, 64x96, 32x32, 32x64, 64x32, 32x128, "
+		                            "128x32), splitk >= 0");
 	if (t.mfma32 != 0 && t.mfma32 != 1)
 		throw std::invalid_argument("dmk tuning: mfma32 must be 0 or 1");
 	if (t.depth != 1 && t.depth != 2)
@@ -1212,7 +1219,9 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		                                       { 128, 128 } };
 	if (precision_ != PREC_BF16X3)
 		for (const auto& sh : { std::make_pair(96, 96), std::make_pair(96, 64),
-		                        std::make_pair(64, 96) })
+		                        std::make_pair(64, 96), std::make_pair(64, 32),
+		                        std::make_pair(32, 64), std::make_pair(32, 32),
+		                        std::make_pair(128, 32) })
 			tile_shapes.push_back(sh);
 	for (int pass = 0; pass < 2; ++pass) {
 		std::vector<Tuning> c;

@@ -819,22 +819,43 @@ void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* task
 		DMK_PLAUNCH(64, 96, BK, OP);                                                       \
 	else                                                                                       \
 		throw std::runtime_error("panel gemm: unsupported tile shape");
+	// 32-wide tiles (16-wide wave tiles, 16x16 MFMA blocks): small patches, such as the comb
+	// shape's patches of 1-30 states, and stage-1 tasks whose row count is a few R_I.
+#define DMK_PSHAPES32(BK, OP)                                                                      \
+	if (bm == 64 && bn == 32)                                                                  \
+		DMK_PLAUNCH(64, 32, BK, OP);                                                       \
+	else if (bm == 32 && bn == 64)                                                             \
+		DMK_PLAUNCH(32, 64, BK, OP);                                                       \
+	else if (bm == 32 && bn == 32)                                                             \
+		DMK_PLAUNCH(32, 32, BK, OP);                                                       \
+	else if (bm == 128 && bn == 32)                                                            \
+		DMK_PLAUNCH(128, 32, BK, OP);                                                      \
+	else if (bm == 32 && bn == 128)                                                            \
+		DMK_PLAUNCH(32, 128, BK, OP);                                                      \
+	else                                                                                       \
+		throw std::runtime_error("panel gemm: unsupported tile shape");
 	const bool s96 = bm == 96 || bn == 96;
+	const bool s32 = bm == 32 || bn == 32;
 	if constexpr (std::is_same<T, double>::value) {
 		if (s96) {
 			DMK_PSHAPES96(16, OpF64)
+		} else if (s32) {
+			DMK_PSHAPES32(16, OpF64)
 		} else {
 			DMK_PSHAPES(16, OpF64)
 		}
 	} else {
 		if (s96) {
 			DMK_PSHAPES96(32, OpF32)
+		} else if (s32) {
+			DMK_PSHAPES32(32, OpF32)
 		} else if (variant == 1) {
 			DMK_PSHAPES(32, OpF32x32)
 		} else {
 			DMK_PSHAPES(32, OpF32)
 		}
 	}
+#undef DMK_PSHAPES32
 #undef DMK_PSHAPES96
 #undef DMK_PSHAPES
 #undef DMK_PLAUNCH

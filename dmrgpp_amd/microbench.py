@@ -7,7 +7,7 @@
         X[i] = (i + 1) / n;
   (iii) a replay of a saved superblock (bench.py --save-problem), e.g. the m=2048 ladder.
 
-  python -m dmrgpp_amd.microbench [--replay F.npz [--load-tuning T.json]] [--iters 30]
+  python -m dmrgpp_amd.microbench [--replay F.npz [--load-tuning T.json]] [--iters 30] [--autotune]
   python -m dmrgpp_amd.microbench --replay F.npz --multi 1,2,4,8 [--iters 12]
   python -m dmrgpp_amd.microbench --density 1093,986,654,538 [--iters 5]
 
@@ -32,7 +32,7 @@ import numpy as np
 from . import problems
 
 
-def run_case(name, prob, x, iters, log, tunings=None):
+def run_case(name, prob, x, iters, log, tunings=None, autotune=False):
     from . import BatchedGemm, DeviceArray
 
     op = BatchedGemm(prob, "fp64")
@@ -44,11 +44,14 @@ def run_case(name, prob, x, iters, log, tunings=None):
         op.set_precision(prec)
         if tunings and prec in tunings:
             op.set_tuning(**tunings[prec])
+        elif autotune:
+            op.autotune(dx, dy, 5)
         op.profile(dx, dy, 1)
         _, _, ms = op.profile(dx, dy, iters)
         out[prec] = {"ms": round(ms, 4), "tflops": round(info["flops_algorithmic"] / (ms * 1e-3)
                                                          / 1e12, 2)}
-        log(f"{name:28s} {prec:7s} {ms:8.3f} ms  {out[prec]['tflops']:7.2f} TF/s")
+        out[prec]["tuning"] = op.tuning()
+        log(f"{name:28s} {prec:7s} {ms:8.3f} ms  {out[prec]['tflops']:7.2f} TF/s  {op.tuning()}")
     op.close()
     return out
 
@@ -118,6 +121,8 @@ def main(argv=None):
     ap.add_argument("--multi", default=None,
                     help="comma-separated vector counts: time dmk_apply_multi_d on the replayed "
                          "superblock instead of the single-vector cases")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune each case and precision (else the default launch shapes)")
     ap.add_argument("--density", default=None, nargs="?", const="1093,986,654,538",
                     help="comma-separated block sizes: time the truncation eigensolver instead")
     a = ap.parse_args(argv)
@@ -139,16 +144,18 @@ def main(argv=None):
     res = []
     p1 = problems.block_diagonal_problem()
     res.append(run_case("bench_BatchedGemm-40-patches", p1,
-                        np.random.default_rng(42).uniform(-1, 1, p1.n), a.iters, log))
+                        np.random.default_rng(42).uniform(-1, 1, p1.n), a.iters, log,
+                        autotune=a.autotune))
     p2 = problems.comb_problem()
-    res.append(run_case("test_vbatch-comb-8+8", p2, (np.arange(p2.n) + 1.0) / p2.n, a.iters, log))
+    res.append(run_case("test_vbatch-comb-8+8", p2, (np.arange(p2.n) + 1.0) / p2.n, a.iters, log,
+                        autotune=a.autotune))
     if a.replay:
         from .problem import load_npz
 
         p3 = load_npz(a.replay, name="superblock")
         res.append(run_case("replay:" + p3.name, p3,
                             np.random.default_rng(1234).uniform(-1, 1, p3.n), a.iters, log,
-                            tunings))
+                            tunings, autotune=a.autotune))
     print(json.dumps(res), flush=True)
 
 

@@ -3,6 +3,7 @@ together by the Krylov time step, all targets in one grouped Kron launch per Kry
 
   python -m dmrgpp_amd.timeevo --config hubbard-ladder-24x2 [--m 2048] [--targets 4]
                                [--dt 0.05] [--steps 40] [--precision fp64[,fp32,bf16x3]]
+                               [--save-problem F.npz] (with --steps 0: save and stop)
 
 The superblock is the middle of the lattice after infinite-size growth (sweep driver, GPU
 Lanczos). The targets are the ground state and seeded random vectors of the same sector
@@ -110,12 +111,20 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--json", default=None)
+    ap.add_argument("--save-problem", default=None,
+                    help="write the superblock to this .npz (for microbench --replay)")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     t0 = time.perf_counter()
     prob, z, e = superblock(a.config, a.m, log=log)
     log(f"superblock n={prob.n} patches={prob.npatches} terms={len(prob.blocks)} E0={e:.12f} "
         f"({time.perf_counter() - t0:.1f} s)")
+    if a.save_problem:
+        from .problem import save_npz
+
+        save_npz(prob, a.save_problem)
+        if a.steps == 0:
+            return 0
     out = {"config": a.config, "E0": e, "npatches": prob.npatches, "nterms": len(prob.blocks),
            "runs": {}}
     for prec in a.precision.split(","):

@@ -78,7 +78,8 @@ typedef struct dmk_info {
 } dmk_info;
 
 /* Launch shape of a handle: output tile (64 or 128 each way; FP64 and FP32 also 96 x 96,
- * 96 x 64 and 64 x 96) of the stage-1 and stage-2 grouped-GEMM launches, the split-k
+ * 96 x 64, 64 x 96 and the 32-wide tiles 32 x 32, 32 x 64, 64 x 32, 32 x 128, 128 x 32) of the
+ * stage-1 and stage-2 grouped-GEMM launches, the split-k
  * granularity of stage 2 (target contraction length per piece, 0 = never split; pieces are
  * summed in a fixed order, so results stay deterministic),
  * the workgroup order (0 = by decreasing task cost, 1 = XCD-aware regions), the MFMA block form
@@ -430,6 +431,44 @@ int dmk_davidson_default_params(dmk_davidson_params* params);
 int dmk_davidson_ground_d(dmk_handle* handle, const double* init, double* z,
                           const dmk_davidson_params* params, dmk_mlp* mlp,
                           dmk_davidson_result* result);
+
+/* ---- The matvec facade of a superblock sector: MatrixVectorKron
+ * (Engine/MatrixVectorKron/MatrixVectorKron.h:113-166) as MatrixVector::create selects it
+ * (Engine/MatrixVector.hpp:133-150). x += op(y), op = H, or L^dagger L with ldaggerl (the
+ * LdaggerL option, MatrixVectorKron.h:151-160) where the blocks describe L. A superblock whose
+ * rank is at most max_rank_stored (MaxMatrixRankStored, ParametersDmrgSolver.h:326; the
+ * reference's default 0 never stores) is assembled once into CSR on the device and every call
+ * is one sparse product (two with ldaggerl; stored ranks are limited to 16384); otherwise the
+ * Kron plugin computes the product, and L^dagger comes from a second plugin handle built from
+ * the adjoint blocks (the reference runs that adjoint on its CPU Kron path). Each call is timed
+ * on the host and accumulated (DeltaClock matrixVectorProduct, MatrixVectorKron.h:139-166).
+ * x and y are device vectors of n elements (interleaved complex for _z); synchronous. */
+typedef struct dmk_facade dmk_facade;
+
+typedef struct dmk_facade_info {
+	int64_t n;               /* rank (vector length in elements of T) */
+	int64_t nnz;             /* stored path: nonzeros of H (0 on the Kron path) */
+	int64_t matvecs;         /* calls so far */
+	double  ms_matvec_total; /* accumulated wall time of the calls (DeltaClock) */
+	int32_t stored;          /* 1: stored CSR path, 0: Kron plugin */
+	int32_t ldaggerl;
+	int32_t is_complex;
+	int32_t pad;
+} dmk_facade_info;
+
+int  dmk_facade_create_d(size_t noperator, size_t npatches, const size_t* left_patch_size,
+                         const size_t* right_patch_size, const double* const* Amatrix,
+                         const size_t* ld_Amatrix, const double* const* Bmatrix,
+                         const size_t* ld_Bmatrix, int64_t max_rank_stored, int ldaggerl,
+                         dmk_facade** facade);
+int  dmk_facade_create_z(size_t noperator, size_t npatches, const size_t* left_patch_size,
+                         const size_t* right_patch_size, const double* const* Amatrix,
+                         const size_t* ld_Amatrix, const double* const* Bmatrix,
+                         const size_t* ld_Bmatrix, int64_t max_rank_stored, int ldaggerl,
+                         dmk_facade** facade);
+int  dmk_facade_matvec(dmk_facade* facade, double* x, const double* y);
+int  dmk_facade_get_info(const dmk_facade* facade, dmk_facade_info* info);
+void dmk_facade_destroy(dmk_facade* facade);
 
 /* Device memory helpers for callers without their own allocator. */
 int dmk_device_alloc(size_t bytes, void** ptr);

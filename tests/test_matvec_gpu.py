@@ -227,6 +227,13 @@ TUNINGS = [
          xcd_order=0, mfma32=0, prefetch=1),
     dict(stage1_tile_m=96, stage1_tile_n=96, stage2_tile_m=96, stage2_tile_n=64, splitk=48,
          xcd_order=1, mfma32=0, prefetch=2),
+    # 32-wide tiles (FP64 / FP32; BF16x3 rejects them).
+    dict(stage1_tile_m=32, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=32, splitk=16,
+         xcd_order=1, mfma32=1, prefetch=2),
+    dict(stage1_tile_m=32, stage1_tile_n=32, stage2_tile_m=128, stage2_tile_n=32, splitk=0,
+         xcd_order=0, mfma32=0, prefetch=1),
+    dict(stage1_tile_m=32, stage1_tile_n=128, stage2_tile_m=32, stage2_tile_n=32, splitk=32,
+         xcd_order=1, mfma32=0, prefetch=2),
 ]
 
 
@@ -234,12 +241,12 @@ TUNINGS = [
 @pytest.mark.parametrize("tuning", TUNINGS)
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-4)])
 def test_gpu_tunings_match_oracle(tuning, prec, tol):
-    # Every tile shape, forced split-k (pieces of 16-48 along K, ragged last piece), both
+    # Every tile shape (64 / 128, 96-wide, 32-wide), forced split-k (pieces of 16-48 along K, ragged last piece), both
     # workgroup orders and both FP32 MFMA forms give the oracle's result; the same tuning twice
     # is bitwise identical.
-    uses96 = 96 in (tuning["stage1_tile_m"], tuning["stage1_tile_n"], tuning["stage2_tile_m"],
-                    tuning["stage2_tile_n"])
-    if prec == "bf16x3" and uses96:
+    shapes = (tuning["stage1_tile_m"], tuning["stage1_tile_n"], tuning["stage2_tile_m"],
+              tuning["stage2_tile_n"])
+    if prec == "bf16x3" and (96 in shapes or 32 in shapes):
         op = dm.BatchedGemm(problems.random_problem([5, 7], [6, 3], 2, seed=1), prec)
         with pytest.raises(dm.DmkError):
             op.set_tuning(**tuning)
