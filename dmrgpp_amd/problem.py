@@ -119,19 +119,21 @@ class PluginArgs:
         self.npatches = npch
         self.L = (ctypes.c_size_t * npch)(*[int(v) for v in prob.L])
         self.R = (ctypes.c_size_t * npch)(*[int(v) for v in prob.R])
-        self.A = (ctypes.c_void_p * total)()
-        self.B = (ctypes.c_void_p * total)()
-        self.ldA = (ctypes.c_size_t * total)()
-        self.ldB = (ctypes.c_size_t * total)()
+        # numpy arrays behind the ctypes views (O(blocks) Python work, not O(np^2 nC)).
+        lda = np.tile(np.asarray(prob.L, dtype=np.uint64), npch * nC)
+        ldb = np.tile(np.asarray(prob.R, dtype=np.uint64), npch * nC)
+        pa = np.zeros(total, dtype=np.uint64)
+        pb = np.zeros(total, dtype=np.uint64)
         self._keep = []
-        for i in range(total):
-            I = i % npch
-            self.ldA[i] = int(prob.L[I])
-            self.ldB[i] = int(prob.R[I])
         for (I, J, k), (a, b) in prob.blocks.items():
             i = prob.index(I, J, k)
             self._keep += [a, b]
-            self.A[i] = a.ctypes.data
-            self.B[i] = b.ctypes.data
-            self.ldA[i] = a.shape[0]
-            self.ldB[i] = b.shape[0]
+            pa[i] = a.ctypes.data
+            pb[i] = b.ctypes.data
+            lda[i] = a.shape[0]
+            ldb[i] = b.shape[0]
+        self._arrays = (lda, ldb, pa, pb)
+        self.A = (ctypes.c_void_p * total).from_buffer(pa)
+        self.B = (ctypes.c_void_p * total).from_buffer(pb)
+        self.ldA = (ctypes.c_size_t * total).from_buffer(lda)
+        self.ldB = (ctypes.c_size_t * total).from_buffer(ldb)

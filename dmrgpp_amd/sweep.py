@@ -447,7 +447,9 @@ class GpuSolver:
         r["z_dev"] = dz
         r["plan"] = {k: info[k] for k in ("nterms", "flops_algorithmic", "stage1_tiles",
                                           "stage2_tiles")}
-        r["timing"] = {"plan_ms": 1e3 * (t1 - t0), "lanczos_ms": 1e3 * (t2 - t1)}
+        r["timing"] = {"plan_ms": 1e3 * (t1 - t0), "lanczos_ms": 1e3 * (t2 - t1),
+                       "plan_scan_ms": info["ms_scan"], "plan_stage_ms": info["ms_stage"],
+                       "plan_upload_ms": info["ms_upload"]}
         return r["energy"], z, r
 
 
@@ -552,7 +554,9 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
         tm = rep.get("timing", {})
         log(f"{phase:8s} center={l:3d} m={m:5d} n={prob.n:9d} patches={prob.npatches:4d} "
             f"terms={len(prob.blocks):5d} E={e:.12f} steps={rec.lanczos_steps:3d} "
-            f"solve={rec.ms_lanczos:7.1f}ms (plan {tm.get('plan_ms', 0):6.1f} lanczos "
+            f"solve={rec.ms_lanczos:7.1f}ms (plan {tm.get('plan_ms', 0):6.1f} "
+            f"[scan {tm.get('plan_scan_ms', 0):.1f} stage {tm.get('plan_stage_ms', 0):.1f} "
+            f"upload {tm.get('plan_upload_ms', 0):.1f}] lanczos "
             f"{tm.get('lanczos_ms', 0):6.1f}) grow={tm['grow_ms']:6.1f} kron={tm['kron_ms']:6.1f} "
             f"wft={tm['wft_ms']:6.1f} trunc={tm['truncate_ms']:6.1f} disc={disc:.2e}")
         return rec

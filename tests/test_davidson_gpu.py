@@ -129,22 +129,30 @@ def test_gpu_davidson_matches_lanczos_on_qn_problem():
 
 @pytest.mark.gpu
 def test_gpu_learned_preconditioner_comparison():
-    # Train on the first sweep of a Hubbard chain, then compare plain Lanczos, Davidson with the
-    # diagonal correction and Davidson with the validated learned correction on the next sweep,
-    # all at FP64 residual 1e-9: equal energies, and the learned method (the network, or the
-    # diagonal fallback when validation rejects it) needs no more matvecs than the diagonal one
-    # beyond a 10% margin.
+    # Train on the first sweep of a Hubbard chain, then run the next sweep with plain Lanczos,
+    # Davidson with the diagonal correction and Davidson with the network's own correction (no
+    # fallback), all at FP64 residual 1e-9, each step from the same transformed wave function.
+    # Equal energies; the network's matvec count is compared with the diagonal one directly.
+    # The measured outcome is that the network does not gain: it needs 1.15x the diagonal
+    # count here (1.14x on an 8x2 ladder and 1.22x on a 16x4 cylinder at m = 256), so the
+    # assertion pins that it converges without costing more than 25 % extra matvecs, and that the
+    # validation step, which decides from its own counts, keeps the diagonal method whenever
+    # the network is not at least 5 % better on the validation superblocks.
     from dmrgpp_amd import learned
 
     out = learned.compare("hubbard-chain-12", train_sweeps=1, test_sweeps=1, train_iters=1500,
-                          log=lambda s: None)
+                          log=print)
     e = out["lanczos"]["energy"]
     for key in ("davidson-diag", "davidson-mlp", "davidson-learned"):
         assert abs(out[key]["energy"] - e) < 1e-9 * abs(e)
         assert out[key]["max_residual"] < 1.1e-9
-    assert out["samples"] > 0 and out["loss"] <= out["loss_untrained"]
-    assert out["davidson-learned"]["matvecs"] <= 1.1 * out["davidson-diag"]["matvecs"]
-    assert out["lanczos"]["matvecs"] > out["davidson-diag"]["matvecs"]
+    assert out["samples"] > 0 and out["loss"] < out["loss_untrained"]
+    mlp, diag = out["davidson-mlp"]["matvecs"], out["davidson-diag"]["matvecs"]
+    print(f"network / diagonal matvecs: {mlp} / {diag} = {mlp / diag:.3f}")
+    assert mlp <= 1.25 * diag
+    val = out["validation_matvecs"]
+    assert out["chosen"] == ("mlp" if val["mlp"] < 0.95 * val["diag"] else "diag")
+    assert out["lanczos"]["matvecs"] > diag
 
 
 @pytest.mark.gpu
