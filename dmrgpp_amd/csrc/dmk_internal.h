@@ -1,5 +1,5 @@
 // Internal descriptors shared by the host planner (kron_plan.cpp) and the HIP kernels
-// (kron_kernels.hip). Not part of the public C ABI.
+// (panel_kernels.hip, kron_kernels.hip, initkron_kernels.hip). Not part of the public C ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,47 +8,6 @@
 #include <string>
 
 namespace dmk {
-
-// Operand bases. A segment operand is (base id, element offset); the base table is filled per
-// launch, so the same descriptors serve every call whatever vin/vout the caller passes.
-enum BaseId : int32_t
-{
-	BASE_APACK = 0, // packed left-block factors (A panels, identity-free)
-	BASE_BPACK = 1, // packed right-block factors (row-stacked B blocks per (I,J))
-	BASE_X     = 2, // input vector (patch order) in the compute precision
-	BASE_BX    = 3, // stage-1 products B*X_J, laid out as the stage-2 panels
-	BASE_COUNT = 4
-};
-
-// One contribution C(MxN) += L(MxK) * R(KxN) to a task. L is column major (ldl). R is column
-// major K x N (ldr) when rtrans == 0, or column major N x K used transposed when rtrans == 1.
-struct Seg {
-	int64_t loff;
-	int64_t roff;
-	int32_t lbase;
-	int32_t rbase;
-	int32_t ldl;
-	int32_t ldr;
-	int32_t k;
-	int32_t rtrans;
-};
-
-// One output matrix. Element (r, c) of the MxN result is stored at
-//   C[(r % rsplit) + ((r / rsplit) * N + c) * ldc]
-// which is plain column-major storage when rsplit == M; stage 1 uses rsplit = R_I to lay the
-// row-stacked products B_k*X_J side by side as the stage-2 panel columns.
-struct Task {
-	int64_t coff;      // element offset of C in its output buffer
-	int32_t m;
-	int32_t n;
-	int32_t ldc;
-	int32_t rsplit;
-	int32_t seg_begin;
-	int32_t seg_end;
-	int32_t out_is_y;  // 1: C lives in the caller's output vector (type TO), 0: in the
-	                   // workspace (BX or split-k partials, compute type)
-	int32_t pad;
-};
 
 // One fixed-order reduction of split-k partials into the output: for e < count,
 //   out[yoff + e] (+)= sum_{s < nsplit} ws[wsoff + s * count + e].

@@ -608,9 +608,13 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	for (auto& e : ev_)
 		DMK_HIP_CHECK(hipEventCreate(&e));
 	// Upload every distinct block a copy job reads, compact and column major, through pinned
-	// staging (filled in parallel), then build the panels on the device.
-	const auto t_stage0 = std::chrono::steady_clock::now();
-	int64_t    raw_elems = 0;
+	// staging, then build the panels on the device. The staging is filled in chunks of about
+	// kStageChunk bytes in upload order, by parallel host threads, and each chunk's DMA is
+	// queued as soon as it is filled, so the transfer of one chunk overlaps the copying of the
+	// next.
+	constexpr size_t kStageChunk = size_t(16) << 20;
+	const auto       t_stage0    = std::chrono::steady_clock::now();
+	int64_t          raw_elems   = 0;
 	for (const Copy& c : copies) {
 		Distinct& b = blocks[size_t(c.src)];
 		if (b.raw_off < 0) {
@@ -622,17 +626,10 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	for (size_t d = 0; d < blocks.size(); ++d)
 		if (blocks[d].raw_off >= 0)
 			uploaded.push_back(int32_t(d));
-	const size_t rbytes = size_t(std::max<int64_t>(raw_elems, 1)) * sizeof(double);
-	double*      hraw   = static_cast<double*>(pool_host_alloc(rbytes));
-	parallel_for(uploaded.size(), [&](size_t u) {
-		const Distinct& b = blocks[size_t(uploaded[u])];
-		double*         d = hraw + b.raw_off;
-		if (b.ld == b.rows)
-			std::memcpy(d, b.p, size_t(b.rows * b.cols) * sizeof(double));
-		else
-			for (int64_t j = 0; j < b.cols; ++j)
-				std::memcpy(d + j * b.rows, b.p + j * b.ld, size_t(b.rows) * sizeof(double));
+	std::sort(uploaded.begin(), uploaded.end(), [&](int32_t a, int32_t b) {
+		return blocks[size_t(a)].raw_off < blocks[size_t(b)].raw_off;
 	});
+	const size_t          rbytes = size_t(std::max<int64_t>(raw_elems, 1)) * sizeof(double);
 	std::vector<PackJob>  jobs;
 	std::vector<PackTile> ptiles;
 	jobs.reserve(copies.size());
@@ -651,27 +648,59 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			for (int32_t j0 = 0; j0 < std::max(j.cols, 1); j0 += 64)
 				ptiles.push_back(PackTile { jid, i0, j0, 0 });
 	}
-	stats_.ms_stage = ms_since(t_stage0);
-	const auto t_up0 = std::chrono::steady_clock::now();
 	const size_t abytes = size_t(std::max<int64_t>(asize, 1)) * sizeof(double);
 	const size_t bbytes = size_t(std::max<int64_t>(bsize, 1)) * sizeof(double);
 	d_apack_            = static_cast<double*>(pool_device_alloc(abytes));
 	d_bpack_            = static_cast<double*>(pool_device_alloc(bbytes));
-	double*   draw      = static_cast<double*>(pool_device_alloc(rbytes));
-	PackJob*  djobs     = dev_alloc_copy(jobs);
-	PackTile* dtiles    = dev_alloc_copy(ptiles);
-	hipError_t e1 = hipMemcpyAsync(draw, hraw, rbytes, hipMemcpyHostToDevice, stream_);
-	if (e1 == hipSuccess)
+	double*   hraw      = nullptr;
+	double*   draw      = nullptr;
+	PackJob*  djobs     = nullptr;
+	PackTile* dtiles    = nullptr;
+	auto      release   = [&] {
+                (void)hipStreamSynchronize(stream_);
+                pool_host_release(hraw);
+                pool_device_release(draw);
+                pool_device_release(djobs);
+                pool_device_release(dtiles);
+	};
+	std::chrono::steady_clock::time_point t_up0;
+	try {
+		hraw   = static_cast<double*>(pool_host_alloc(rbytes));
+		draw   = static_cast<double*>(pool_device_alloc(rbytes));
+		djobs  = dev_alloc_copy(jobs);
+		dtiles = dev_alloc_copy(ptiles);
+		for (size_t u0 = 0; u0 < uploaded.size();) {
+			const int64_t e0 = blocks[size_t(uploaded[u0])].raw_off;
+			int64_t       e1 = e0;
+			size_t        u1 = u0;
+			while (u1 < uploaded.size() && size_t(e1 - e0) * sizeof(double) < kStageChunk) {
+				const Distinct& b = blocks[size_t(uploaded[u1])];
+				e1                = b.raw_off + b.rows * b.cols;
+				++u1;
+			}
+			parallel_for(u1 - u0, [&](size_t k) {
+				const Distinct& b = blocks[size_t(uploaded[u0 + k])];
+				double*         d = hraw + b.raw_off;
+				if (b.ld == b.rows)
+					std::memcpy(d, b.p, size_t(b.rows * b.cols) * sizeof(double));
+				else
+					for (int64_t j = 0; j < b.cols; ++j)
+						std::memcpy(d + j * b.rows, b.p + j * b.ld, size_t(b.rows) * sizeof(double));
+			});
+			DMK_HIP_CHECK(hipMemcpyAsync(draw + e0, hraw + e0, size_t(e1 - e0) * sizeof(double),
+			                             hipMemcpyHostToDevice, stream_));
+			u0 = u1;
+		}
+		stats_.ms_stage = ms_since(t_stage0);
+		t_up0           = std::chrono::steady_clock::now();
 		launch_pack_panels(djobs, dtiles, int(ptiles.size()), draw, d_apack_, d_bpack_, stream_);
-	hipError_t e2 = hipGetLastError();
-	hipError_t e3 = hipStreamSynchronize(stream_);
-	pool_host_free(hraw);
-	pool_device_free(draw);
-	pool_device_free(djobs);
-	pool_device_free(dtiles);
-	DMK_HIP_CHECK(e1);
-	DMK_HIP_CHECK(e2);
-	DMK_HIP_CHECK(e3);
+		DMK_HIP_CHECK(hipGetLastError());
+		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+	} catch (...) {
+		release();
+		throw;
+	}
+	release();
 	stats_.ms_upload = ms_since(t_up0);
 	stats_.raw_bytes = double(rbytes);
 	d_prep_tiles_   = dev_alloc_copy(prep_tiles_);

@@ -1,12 +1,15 @@
 // Variable-size batched GEMM: dmrg_Xgemm_vbatch (GPUPlugin/dmrg_vbatch.cpp:129-652) on the
-// grouped MFMA kernel.
+// K-major panel kernel of the matvec (panel_kernels.hip).
 //
 // The reference expands its groups into per-entry arrays and hands them to MAGMA
-// *_vbatched_max_nocheck (or a BLAS loop). Here every entry becomes one task of the grouped
-// kernel (two for complex scalars), after its operands are staged into one workspace as compact
-// column-major matrices. Staging makes transposes, conjugation and complex scalars a property of
-// the copy, so the GEMM launch only ever sees the (N, N) real form, and it lets every operand be
-// addressed as an offset into one allocation. Complex products run as real ones:
+// *_vbatched_max_nocheck (or a BLAS loop). Here every entry becomes one task of the panel
+// kernel (two for complex scalars), after its operands are staged into one workspace in the
+// panel kernel's K-major layout: the task computes C^T = op(B)^T op(A)^T, whose left operand is
+// op(B) column major (each column contiguous along k) and whose right operand is op(A)^T column
+// major (each row of op(A) contiguous along k), both with leading dimensions padded to 8 and
+// zero filled; C^T written by rows is C by columns. Staging makes transposes, conjugation and
+// complex scalars a property of the copy, so the GEMM launch only ever sees real K-major
+// operands, addressed as offsets into one allocation. Complex products run as real ones:
 //   Re C = Ar Br + (-Ai) Bi,   Im C = Ar Bi + Ai Br   (two segments per task).
 // The scatter then forms alpha W + beta C in the caller's C (C is not read when beta == 0, as
 // in BLAS). The call is synchronous, like the reference's.
@@ -24,10 +27,10 @@
 namespace dmk {
 
 template <typename T, typename TO>
-void launch_grouped_gemm(int bm, int bn, const Task* tasks, const Seg* segs, const Tile* tiles,
-                         int ntiles, const T* const* bases, int64_t xstride, int64_t bxstride,
-                         TO* out_y, int64_t ystride, T* out_ws, int64_t wsstride, int nvec,
-                         int beta, hipStream_t stream, int variant = 0);
+void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* tasks,
+                       const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
+                       const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
+                       int64_t wsstride, int nvec, int beta, hipStream_t stream);
 
 namespace {
 
@@ -56,13 +59,16 @@ int trans_code(char t, const char* which)
 	}
 }
 
-constexpr int64_t kChunk = 65536; // elements per staging / scatter workgroup
+constexpr int64_t kChunk = 8192; // elements per staging / scatter workgroup (32 per thread)
 
-// Appends copy jobs staging op(X) (rows x cols) from src into the workspace at dst.
-void stage(std::vector<VbCopy>& jobs, const double* src, int ld, int rows, int cols, int64_t dst,
-           int flags)
+// Appends copy jobs staging op(X) (rows x cols) from src into the workspace at dst, column
+// major with the leading dimension padded to 8 (panel_ld(rows)) and the padding zero filled.
+// Returns the staged size in elements.
+int64_t stage(std::vector<VbCopy>& jobs, const double* src, int ld, int rows, int cols,
+              int64_t dst, int flags)
 {
-	const int64_t total = int64_t(rows) * cols;
+	const int     ldd   = int(panel_ld(rows));
+	const int64_t total = int64_t(ldd) * cols;
 	for (int64_t s = 0; s < total; s += kChunk) {
 		VbCopy j{};
 		j.src   = src;
@@ -72,8 +78,10 @@ void stage(std::vector<VbCopy>& jobs, const double* src, int ld, int rows, int c
 		j.rows  = rows;
 		j.ld    = ld;
 		j.flags = flags;
+		j.ldd   = ldd;
 		jobs.push_back(j);
 	}
+	return total;
 }
 
 template <typename T> T* upload(const std::vector<T>& v, std::vector<void*>& owned)
@@ -127,52 +135,57 @@ void gemm_vbatch(bool cplx, char transa, char transb, const int* m, const int* n
 
 	std::vector<VbCopy> copies;
 	std::vector<VbOut>  outs;
-	std::vector<Task>   tasks;
-	std::vector<Seg>    segs;
+	std::vector<PTask>  tasks;
+	std::vector<PSeg>   segs;
 	std::vector<double> cost;
 	int64_t             wsz = 0, csz = 0;
-	auto seg = [&](int64_t loff, int64_t roff, int mm, int kk) {
-		Seg s{};
-		s.loff   = loff;
-		s.roff   = roff;
-		s.lbase  = BASE_APACK;
-		s.rbase  = BASE_APACK;
-		s.ldl    = mm;
-		s.ldr    = kk;
-		s.k      = kk;
-		s.rtrans = 0;
+	// Segment of C^T(n x m) += L(n x k) R(k x m): L = staged op(B) part at lo, R = staged
+	// op(A)^T part at ro, both K-major with leading dimension panel_ld(k).
+	auto seg = [&](int64_t lo, int64_t ro, int kk) {
+		PSeg s{};
+		s.loff  = lo;
+		s.roff  = ro;
+		s.lbase = PBASE_APACK;
+		s.rbase = PBASE_APACK;
+		s.ldl   = int32_t(panel_ld(kk));
+		s.ldr   = int32_t(panel_ld(kk));
+		s.k     = kk;
 		segs.push_back(s);
 	};
-	auto task = [&](int64_t coff, int mm, int nn, int sb, int se) {
-		Task t{};
+	// Task C^T (n x m) written by rows of length m: element (j, i) at coff + i + j m, i.e. C
+	// (m x n) column major at coff.
+	auto task = [&](int64_t coff, int mm, int nn, int kk, int sb, int se) {
+		PTask t{};
 		t.coff      = coff;
-		t.m         = mm;
-		t.n         = nn;
-		t.ldc       = mm;
-		t.rsplit    = mm;
+		t.m         = nn;
+		t.n         = mm;
+		t.rsplit    = nn;
+		t.sm        = mm;
 		t.seg_begin = sb;
 		t.seg_end   = se;
 		t.out_is_y  = 1;
 		tasks.push_back(t);
-		cost.push_back(double(mm) * nn * std::max(1, se > sb ? segs[size_t(sb)].k : 1));
+		cost.push_back(double(mm) * nn * std::max(1, kk) * std::max(1, se - sb));
 	};
 	for (const Entry& e : ents) {
-		const int64_t mk = int64_t(e.m) * e.k, kn = int64_t(e.k) * e.n, mn = int64_t(e.m) * e.n;
+		const int64_t mn   = int64_t(e.m) * e.n;
 		const bool    mult = e.k > 0 && !alpha_zero;
-		// A operand flags: transpose for T and C; conjugation negates the imaginary part.
-		const int fa = (ta ? VB_TRANS : 0) | cx;
+		// op(A)^T is staged (k x m): transposed unless op(A) already transposes; op(B) (k x n)
+		// as it is. Conjugation negates the imaginary part.
+		const int fa = (ta ? 0 : VB_TRANS) | cx;
 		const int fb = (tb ? VB_TRANS : 0) | cx;
 		if (!cplx) {
-			const int64_t ao = wsz, bo = wsz + (mult ? mk : 0);
+			int64_t ao = 0, bo = 0;
 			if (mult) {
-				stage(copies, e.a, e.lda, e.m, e.k, ao, fa);
-				stage(copies, e.b, e.ldb, e.k, e.n, bo, fb);
-				wsz += mk + kn;
+				ao = wsz;
+				wsz += stage(copies, e.a, e.lda, e.k, e.m, wsz, fa);
+				bo = wsz;
+				wsz += stage(copies, e.b, e.ldb, e.k, e.n, wsz, fb);
 			}
 			const int sb = int(segs.size());
 			if (mult)
-				seg(ao, bo, e.m, e.k);
-			task(csz, e.m, e.n, sb, int(segs.size()));
+				seg(bo, ao, e.k);
+			task(csz, e.m, e.n, e.k, sb, int(segs.size()));
 			VbOut o{};
 			o.dst    = e.c;
 			o.src_re = csz;
@@ -187,32 +200,36 @@ void gemm_vbatch(bool cplx, char transa, char transb, const int* m, const int* n
 			}
 			csz += mn;
 		} else {
-			// Ar, Ai, -Ai (m x k) then Br, Bi (k x n); conj of A or B flips the imaginary sign.
-			const int     na  = (ta == 2) ? VB_NEG : 0;
-			const int     nb  = (tb == 2) ? VB_NEG : 0;
-			const int64_t ar = wsz, ai = wsz + mk, nai = wsz + 2 * mk, br = wsz + 3 * mk,
-			              bi = wsz + 3 * mk + kn;
+			// Ar, Ai, -Ai (as op(A)^T, k x m), then Br, Bi (k x n); conj of A or B flips the
+			// imaginary sign.
+			const int na = (ta == 2) ? VB_NEG : 0;
+			const int nb = (tb == 2) ? VB_NEG : 0;
+			int64_t   ar = 0, ai = 0, nai = 0, br = 0, bi = 0;
 			if (mult) {
-				stage(copies, e.a, e.lda, e.m, e.k, ar, fa);
-				stage(copies, e.a, e.lda, e.m, e.k, ai, fa | VB_IMAG | na);
-				stage(copies, e.a, e.lda, e.m, e.k, nai, fa | VB_IMAG | (na ^ VB_NEG));
-				stage(copies, e.b, e.ldb, e.k, e.n, br, fb);
-				stage(copies, e.b, e.ldb, e.k, e.n, bi, fb | VB_IMAG | nb);
-				wsz += 3 * mk + 2 * kn;
+				ar = wsz;
+				wsz += stage(copies, e.a, e.lda, e.k, e.m, wsz, fa);
+				ai = wsz;
+				wsz += stage(copies, e.a, e.lda, e.k, e.m, wsz, fa | VB_IMAG | na);
+				nai = wsz;
+				wsz += stage(copies, e.a, e.lda, e.k, e.m, wsz, fa | VB_IMAG | (na ^ VB_NEG));
+				br = wsz;
+				wsz += stage(copies, e.b, e.ldb, e.k, e.n, wsz, fb);
+				bi = wsz;
+				wsz += stage(copies, e.b, e.ldb, e.k, e.n, wsz, fb | VB_IMAG | nb);
 			}
 			const int64_t cr = csz, ci = csz + mn;
 			int           sb = int(segs.size());
 			if (mult) {
-				seg(ar, br, e.m, e.k);
-				seg(nai, bi, e.m, e.k);
+				seg(br, ar, e.k);
+				seg(bi, nai, e.k);
 			}
-			task(cr, e.m, e.n, sb, int(segs.size()));
+			task(cr, e.m, e.n, e.k, sb, int(segs.size()));
 			sb = int(segs.size());
 			if (mult) {
-				seg(ar, bi, e.m, e.k);
-				seg(ai, br, e.m, e.k);
+				seg(bi, ar, e.k);
+				seg(br, ai, e.k);
 			}
-			task(ci, e.m, e.n, sb, int(segs.size()));
+			task(ci, e.m, e.n, e.k, sb, int(segs.size()));
 			VbOut o{};
 			o.dst    = e.c;
 			o.src_re = cr;
@@ -234,8 +251,8 @@ void gemm_vbatch(bool cplx, char transa, char transb, const int* m, const int* n
 	std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[size_t(x)] > cost[size_t(y)]; });
 	std::vector<Tile> tiles;
 	for (int t : order)
-		for (int m0 = 0; m0 < tasks[size_t(t)].m; m0 += 64)
-			for (int n0 = 0; n0 < tasks[size_t(t)].n; n0 += 64)
+		for (int m0 = 0; m0 < std::max(tasks[size_t(t)].m, 1); m0 += 64)
+			for (int n0 = 0; n0 < std::max(tasks[size_t(t)].n, 1); n0 += 64)
 				tiles.push_back(Tile{ t, m0, n0, 0 });
 	if (tiles.size() > size_t(0x7fffffff) || copies.size() > size_t(0x7fffffff)
 	    || outs.size() > size_t(0x7fffffff))
@@ -249,14 +266,15 @@ void gemm_vbatch(bool cplx, char transa, char transb, const int* m, const int* n
 		owned.push_back(cw);
 		VbCopy* d_copies = upload(copies, owned);
 		VbOut*  d_outs   = upload(outs, owned);
-		Task*   d_tasks  = upload(tasks, owned);
-		Seg*    d_segs   = upload(segs, owned);
+		PTask*  d_tasks  = upload(tasks, owned);
+		PSeg*   d_segs   = upload(segs, owned);
 		Tile*   d_tiles  = upload(tiles, owned);
 		hipStream_t s = nullptr;
 		launch_vb_copy(d_copies, int(copies.size()), ws, s);
-		const double* bases[BASE_COUNT] = { ws, ws, ws, ws };
-		launch_grouped_gemm<double, double>(64, 64, d_tasks, d_segs, d_tiles, int(tiles.size()),
-		                                    bases, 0, 0, cw, 0, nullptr, 0, 1, 0, s);
+		const double*  bases[PBASE_COUNT] = { ws, ws, ws, ws, ws };
+		const PStrides vs { 0, 0, 0 };
+		launch_panel_gemm<double, double>(64, 64, 0, 2, d_tasks, d_segs, d_tiles,
+		                                  int(tiles.size()), bases, vs, cw, 0, nullptr, 0, 1, 0, s);
 		DMK_HIP_CHECK(hipGetLastError());
 		launch_vb_scatter(d_outs, int(outs.size()), cw, alpha[0], cplx ? alpha[1] : 0.0, beta[0],
 		                  cplx ? beta[1] : 0.0, beta_zero ? 1 : 0, s);

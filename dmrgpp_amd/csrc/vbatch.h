@@ -1,5 +1,5 @@
 // Variable-size batched GEMM over caller pointers (the dmrg_Xgemm_vbatch boundary of the
-// reference plugin, GPUPlugin/dmrg_vbatch.cpp:129-652), built on the grouped MFMA kernel.
+// reference plugin, GPUPlugin/dmrg_vbatch.cpp:129-652), built on the K-major panel kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,8 +17,9 @@ enum VbFlags : int32_t
 	VB_NEG   = 8, // negate
 };
 
-// One chunk [start, start + len) of a compact column-major destination matrix (rows x ?) at
-// element offset dst of the staging workspace, read from src with leading dimension ld.
+// One chunk [start, start + len) of a column-major destination matrix (rows x ?, leading
+// dimension ldd >= rows, zeros in rows [rows, ldd)) at element offset dst of the staging
+// workspace, read from src with leading dimension ld.
 struct VbCopy {
 	const double* src;
 	int64_t       dst;
@@ -27,7 +28,7 @@ struct VbCopy {
 	int32_t       rows;
 	int32_t       ld;
 	int32_t       flags;
-	int32_t       pad;
+	int32_t       ldd;
 };
 
 // One chunk of a result scattered into the caller's C (rows x ?, leading dimension ldc):

@@ -1,9 +1,10 @@
 // Staging and scatter kernels of the variable-size batched GEMM (dmk_gemm_vbatch_{d,z}).
 //
 // A batch entry C = alpha op(A) op(B) + beta C runs in three launches: the operands are staged
-// into one workspace as compact column-major matrices (transposed, conjugated, split into real
-// and imaginary parts as the entry asks), the grouped MFMA GEMM multiplies the staged operands
-// into a workspace result, and the scatter applies alpha and beta into the caller's C. Each
+// into one workspace as column-major matrices with zero-padded leading dimensions (transposed,
+// conjugated, split into real and imaginary parts as the entry asks), the panel MFMA GEMM
+// multiplies the staged operands into a workspace result, and the scatter applies alpha and
+// beta into the caller's C. Each
 // launch takes a descriptor list; each descriptor is one chunk of one matrix.
 #include "dmk_internal.h"
 #include "vbatch.h"
@@ -20,12 +21,22 @@ __global__ __launch_bounds__(256) void vb_copy_kernel(const VbCopy* __restrict__
 	const bool   cx = (jb.flags & VB_CPLX) != 0;
 	const int    im = (jb.flags & VB_IMAG) ? 1 : 0;
 	const double sg = (jb.flags & VB_NEG) ? -1.0 : 1.0;
-	for (int64_t e = jb.start + threadIdx.x; e < jb.start + jb.len; e += blockDim.x) {
-		const int64_t i   = e % jb.rows;
-		const int64_t j   = e / jb.rows;
-		const int64_t idx = tr ? (j + i * int64_t(jb.ld)) : (i + j * int64_t(jb.ld));
-		const double  v   = cx ? jb.src[2 * idx + im] : jb.src[idx];
-		ws[jb.dst + e]    = sg * v;
+	// (i, j) of element e, advanced by the block size without a 64-bit division per element.
+	const int64_t e0 = jb.start + threadIdx.x;
+	int64_t       j  = e0 / jb.ldd;
+	int           i  = int(e0 - j * jb.ldd);
+	for (int64_t e = e0; e < jb.start + jb.len; e += blockDim.x) {
+		double v = 0.0; // the padding rows [rows, ldd)
+		if (i < jb.rows) {
+			const int64_t idx = tr ? (j + int64_t(i) * jb.ld) : (i + j * int64_t(jb.ld));
+			v                 = sg * (cx ? jb.src[2 * idx + im] : jb.src[idx]);
+		}
+		ws[jb.dst + e] = v;
+		i += int(blockDim.x);
+		if (i >= jb.ldd) {
+			j += i / jb.ldd;
+			i %= jb.ldd;
+		}
 	}
 }
 
@@ -35,9 +46,10 @@ __global__ __launch_bounds__(256) void vb_scatter_kernel(const VbOut* __restrict
                                                          int beta_zero)
 {
 	const VbOut jb = jobs[blockIdx.x];
-	for (int64_t e = jb.start + threadIdx.x; e < jb.start + jb.len; e += blockDim.x) {
-		const int64_t i    = e % jb.rows;
-		const int64_t j    = e / jb.rows;
+	const int64_t e0 = jb.start + threadIdx.x;
+	int64_t       j  = e0 / jb.rows;
+	int           i  = int(e0 - j * jb.rows);
+	for (int64_t e = e0; e < jb.start + jb.len; e += blockDim.x) {
 		const int64_t addr = i + j * int64_t(jb.ldc);
 		if (!jb.cplx) {
 			double v = ar * cw[jb.src_re + e];
@@ -57,6 +69,11 @@ __global__ __launch_bounds__(256) void vb_scatter_kernel(const VbOut* __restrict
 			}
 			jb.dst[2 * addr]     = xr;
 			jb.dst[2 * addr + 1] = xi;
+		}
+		i += int(blockDim.x);
+		if (i >= jb.rows) {
+			j += i / jb.rows;
+			i %= jb.rows;
 		}
 	}
 }

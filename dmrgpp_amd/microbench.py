@@ -10,6 +10,7 @@
   python -m dmrgpp_amd.microbench [--replay F.npz [--load-tuning T.json]] [--iters 30] [--autotune]
   python -m dmrgpp_amd.microbench --replay F.npz --multi 1,2,4,8 [--iters 12]
   python -m dmrgpp_amd.microbench --density 1093,986,654,538 [--iters 5]
+  python -m dmrgpp_amd.microbench --vbatch [--iters 30]
 
 
 Each case reports the average matvec time from HIP events (one warm-up, then iters calls) and
@@ -83,6 +84,36 @@ def run_multi(prob, counts, iters, log):
     return out
 
 
+def run_vbatch(iters, log):
+    """The batched GEMM (dmk_gemm_vbatch_d) on the shapes of the wave-function transformation
+    of an m = 2048 ladder step: one (N, N) and one (T, N) product per symmetry block, blocks of
+    the sizes below (m = n = k = s), timed by wall clock around each synchronous call."""
+    from . import DeviceArray, gemm_vbatch
+
+    sizes = [1040, 1038, 986, 654, 538, 245, 134, 60, 17, 4]
+    rng = np.random.default_rng(5)
+    keep, groups = [], []
+    flops = 0.0
+    for s in sizes:
+        a, b, c = (DeviceArray.from_host(rng.uniform(-1, 1, s * s)) for _ in range(3))
+        keep += [a, b, c]
+        groups.append((s, s, s, s, s, s, [(a.ptr, b.ptr, c.ptr)]))
+        flops += 2.0 * s ** 3
+    out = {"sizes": sizes, "flops": flops}
+    for ta in ("N", "T"):
+        gemm_vbatch(ta, "N", groups)  # warm-up
+        t = time.perf_counter()
+        for _ in range(iters):
+            gemm_vbatch(ta, "N", groups)
+        ms = 1e3 * (time.perf_counter() - t) / iters
+        out[ta + "N"] = {"ms": round(ms, 3), "tflops": round(flops / (ms * 1e-3) / 1e12, 2)}
+        log(f"gemm_vbatch {ta}N {len(sizes)} blocks: {ms:.3f} ms, "
+            f"{flops / (ms * 1e-3) / 1e12:.2f} TF/s")
+    for a in keep:
+        a.free()
+    return out
+
+
 def run_density(sizes, iters, log):
     from . import density_diag
 
@@ -121,6 +152,8 @@ def main(argv=None):
     ap.add_argument("--multi", default=None,
                     help="comma-separated vector counts: time dmk_apply_multi_d on the replayed "
                          "superblock instead of the single-vector cases")
+    ap.add_argument("--vbatch", action="store_true",
+                    help="time the batched GEMM on wave-function-transformation shapes instead")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune each case and precision (else the default launch shapes)")
     ap.add_argument("--density", default=None, nargs="?", const="1093,986,654,538",
@@ -128,6 +161,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     tunings = json.load(open(a.load_tuning)) if a.load_tuning else None
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    if a.vbatch:
+        print(json.dumps(run_vbatch(a.iters, log)), flush=True)
+        return 0
     if a.density:
         sizes = [int(c) for c in a.density.split(",")]
         print(json.dumps(run_density(sizes, a.iters, log)), flush=True)

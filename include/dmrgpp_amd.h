@@ -72,8 +72,9 @@ typedef struct dmk_info {
 	int32_t pad2;
 	int64_t n_complex;         /* complex vector length (equals n for a real handle) */
 	double  ms_scan;           /* handle creation, host wall ms: block scan (nonzero, identity) */
-	double  ms_stage;          /*   copy of the uploaded blocks into pinned staging */
-	double  ms_upload;         /*   upload and panel packing on the device */
+	double  ms_stage;          /*   copy of the uploaded blocks into pinned staging, chunk by
+	                              chunk, each chunk's upload queued as it is filled */
+	double  ms_upload;         /*   the rest of the upload and the panel packing on the device */
 	double  raw_bytes;         /*   bytes uploaded */
 } dmk_info;
 
