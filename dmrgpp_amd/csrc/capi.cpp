@@ -392,6 +392,7 @@ void to_c(const dmk::Tuning& t, dmk_tuning* c)
 	c->xcd_order     = t.xcd;
 	c->mfma32        = t.mfma32;
 	c->prefetch      = t.depth;
+	c->kchunk        = t.kchunk;
 }
 } // namespace
 
@@ -419,6 +420,7 @@ int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning)
 		t.xcd    = tuning->xcd_order;
 		t.mfma32 = tuning->mfma32;
 		t.depth  = tuning->prefetch == 0 ? 1 : tuning->prefetch;
+		t.kchunk = tuning->kchunk == 0 ? 1 : tuning->kchunk;
 		plan_of(handle).retune(t);
 		return DMK_OK;
 	});

@@ -37,7 +37,7 @@
 namespace dmk {
 
 template <typename T, typename TO>
-void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* tasks,
+void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const PTask* tasks,
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
                        const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
                        int64_t wsstride, int nvec, int beta, hipStream_t stream);
@@ -608,13 +608,9 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	for (auto& e : ev_)
 		DMK_HIP_CHECK(hipEventCreate(&e));
 	// Upload every distinct block a copy job reads, compact and column major, through pinned
-	// staging, then build the panels on the device. The staging is filled in chunks of about
-	// kStageChunk bytes in upload order, by parallel host threads, and each chunk's DMA is
-	// queued as soon as it is filled, so the transfer of one chunk overlaps the copying of the
-	// next.
-	constexpr size_t kStageChunk = size_t(16) << 20;
-	const auto       t_stage0    = std::chrono::steady_clock::now();
-	int64_t          raw_elems   = 0;
+	// staging (filled in parallel), then build the panels on the device.
+	const auto t_stage0 = std::chrono::steady_clock::now();
+	int64_t    raw_elems = 0;
 	for (const Copy& c : copies) {
 		Distinct& b = blocks[size_t(c.src)];
 		if (b.raw_off < 0) {
@@ -626,10 +622,20 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	for (size_t d = 0; d < blocks.size(); ++d)
 		if (blocks[d].raw_off >= 0)
 			uploaded.push_back(int32_t(d));
-	std::sort(uploaded.begin(), uploaded.end(), [&](int32_t a, int32_t b) {
-		return blocks[size_t(a)].raw_off < blocks[size_t(b)].raw_off;
+	const size_t rbytes = size_t(std::max<int64_t>(raw_elems, 1)) * sizeof(double);
+	// One staging copy and one upload: filling the staging in chunks, each chunk's DMA queued as
+	// it was filled, measured slower on the m = 2048 ladder steps (median plan 37 ms against
+	// 15 ms: the DMA and the host copies compete for the host's memory bandwidth).
+	double*      hraw   = static_cast<double*>(pool_host_alloc(rbytes));
+	parallel_for(uploaded.size(), [&](size_t u) {
+		const Distinct& b = blocks[size_t(uploaded[u])];
+		double*         d = hraw + b.raw_off;
+		if (b.ld == b.rows)
+			std::memcpy(d, b.p, size_t(b.rows * b.cols) * sizeof(double));
+		else
+			for (int64_t j = 0; j < b.cols; ++j)
+				std::memcpy(d + j * b.rows, b.p + j * b.ld, size_t(b.rows) * sizeof(double));
 	});
-	const size_t          rbytes = size_t(std::max<int64_t>(raw_elems, 1)) * sizeof(double);
 	std::vector<PackJob>  jobs;
 	std::vector<PackTile> ptiles;
 	jobs.reserve(copies.size());
@@ -648,59 +654,41 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			for (int32_t j0 = 0; j0 < std::max(j.cols, 1); j0 += 64)
 				ptiles.push_back(PackTile { jid, i0, j0, 0 });
 	}
+	stats_.ms_stage = ms_since(t_stage0);
+	const auto t_up0 = std::chrono::steady_clock::now();
 	const size_t abytes = size_t(std::max<int64_t>(asize, 1)) * sizeof(double);
 	const size_t bbytes = size_t(std::max<int64_t>(bsize, 1)) * sizeof(double);
-	d_apack_            = static_cast<double*>(pool_device_alloc(abytes));
-	d_bpack_            = static_cast<double*>(pool_device_alloc(bbytes));
-	double*   hraw      = nullptr;
-	double*   draw      = nullptr;
-	PackJob*  djobs     = nullptr;
-	PackTile* dtiles    = nullptr;
-	auto      release   = [&] {
-                (void)hipStreamSynchronize(stream_);
-                pool_host_release(hraw);
-                pool_device_release(draw);
-                pool_device_release(djobs);
-                pool_device_release(dtiles);
-	};
-	std::chrono::steady_clock::time_point t_up0;
+	double*   draw   = nullptr;
+	PackJob*  djobs  = nullptr;
+	PackTile* dtiles = nullptr;
 	try {
-		hraw   = static_cast<double*>(pool_host_alloc(rbytes));
-		draw   = static_cast<double*>(pool_device_alloc(rbytes));
-		djobs  = dev_alloc_copy(jobs);
-		dtiles = dev_alloc_copy(ptiles);
-		for (size_t u0 = 0; u0 < uploaded.size();) {
-			const int64_t e0 = blocks[size_t(uploaded[u0])].raw_off;
-			int64_t       e1 = e0;
-			size_t        u1 = u0;
-			while (u1 < uploaded.size() && size_t(e1 - e0) * sizeof(double) < kStageChunk) {
-				const Distinct& b = blocks[size_t(uploaded[u1])];
-				e1                = b.raw_off + b.rows * b.cols;
-				++u1;
-			}
-			parallel_for(u1 - u0, [&](size_t k) {
-				const Distinct& b = blocks[size_t(uploaded[u0 + k])];
-				double*         d = hraw + b.raw_off;
-				if (b.ld == b.rows)
-					std::memcpy(d, b.p, size_t(b.rows * b.cols) * sizeof(double));
-				else
-					for (int64_t j = 0; j < b.cols; ++j)
-						std::memcpy(d + j * b.rows, b.p + j * b.ld, size_t(b.rows) * sizeof(double));
-			});
-			DMK_HIP_CHECK(hipMemcpyAsync(draw + e0, hraw + e0, size_t(e1 - e0) * sizeof(double),
-			                             hipMemcpyHostToDevice, stream_));
-			u0 = u1;
-		}
-		stats_.ms_stage = ms_since(t_stage0);
-		t_up0           = std::chrono::steady_clock::now();
-		launch_pack_panels(djobs, dtiles, int(ptiles.size()), draw, d_apack_, d_bpack_, stream_);
-		DMK_HIP_CHECK(hipGetLastError());
-		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
+		d_apack_ = static_cast<double*>(pool_device_alloc(abytes));
+		d_bpack_ = static_cast<double*>(pool_device_alloc(bbytes));
+		draw     = static_cast<double*>(pool_device_alloc(rbytes));
+		djobs    = dev_alloc_copy(jobs);
+		dtiles   = dev_alloc_copy(ptiles);
 	} catch (...) {
-		release();
+		// A throwing constructor runs no destructor: everything goes back to its pool here.
+		pool_device_release(d_apack_);
+		pool_device_release(d_bpack_);
+		pool_host_release(hraw);
+		pool_device_release(draw);
+		pool_device_release(djobs);
+		pool_device_release(dtiles);
 		throw;
 	}
-	release();
+	hipError_t e1 = hipMemcpyAsync(draw, hraw, rbytes, hipMemcpyHostToDevice, stream_);
+	if (e1 == hipSuccess)
+		launch_pack_panels(djobs, dtiles, int(ptiles.size()), draw, d_apack_, d_bpack_, stream_);
+	hipError_t e2 = hipGetLastError();
+	hipError_t e3 = hipStreamSynchronize(stream_);
+	pool_host_free(hraw);
+	pool_device_free(draw);
+	pool_device_free(djobs);
+	pool_device_free(dtiles);
+	DMK_HIP_CHECK(e1);
+	DMK_HIP_CHECK(e2);
+	DMK_HIP_CHECK(e3);
 	stats_.ms_upload = ms_since(t_up0);
 	stats_.raw_bytes = double(rbytes);
 	d_prep_tiles_   = dev_alloc_copy(prep_tiles_);
@@ -859,6 +847,13 @@ void Plan::retune(const Tuning& t)
 		throw std::invalid_argument("dmk tuning: mfma32 must be 0 or 1");
 	if (t.depth != 1 && t.depth != 2)
 		throw std::invalid_argument("dmk tuning: prefetch must be 1 or 2");
+	if (t.kchunk != 1 && t.kchunk != 2)
+		throw std::invalid_argument("dmk tuning: kchunk must be 1 or 2");
+	if (t.kchunk == 2
+	    && (precision_ == PREC_BF16X3 || t.bm1 + t.bn1 > 128 || t.bm2 + t.bn2 > 128
+	        || t.bm1 == 96 || t.bn1 == 96 || t.bm2 == 96 || t.bn2 == 96))
+		throw std::invalid_argument("dmk tuning: kchunk 2 takes FP64 / FP32 tiles of 32 or 64 "
+		                            "rows each way");
 	if (stream_)
 		DMK_HIP_CHECK(hipStreamSynchronize(stream_));
 	tuning_                 = t;
@@ -1074,6 +1069,7 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 	const PStrides vs { xp_elems_, xt_elems_, bxs };
 	const int      v1  = tuning_.mfma32;
 	const int      dp  = tuning_.depth;
+	const int      kc  = tuning_.kchunk;
 	if (precision_ == PREC_FP64) {
 		double* bx = static_cast<double*>(d_bx);
 		double* ws = static_cast<double*>(d_ws);
@@ -1082,12 +1078,12 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		launch_prep_x<double>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
 		                      stream_);
 		const double* bases[PBASE_COUNT] = { d_apack_, d_bpack_, xp, xt, bx };
-		launch_panel_gemm<double, double>(st1_.bm, st1_.bn, 0, dp, st1_.d_tasks, st1_.d_segs,
+		launch_panel_gemm<double, double>(st1_.bm, st1_.bn, 0, dp, kc, st1_.d_tasks, st1_.d_segs,
 		                                  st1_.d_tiles, int(st1_.tiles.size()), bases, vs, vout,
 		                                  ldout, bx, bxs, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
-		launch_panel_gemm<double, double>(st2_.bm, st2_.bn, 0, dp, st2_.d_tasks, st2_.d_segs,
+		launch_panel_gemm<double, double>(st2_.bm, st2_.bn, 0, dp, kc, st2_.d_tasks, st2_.d_segs,
 		                                  st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
 		                                  ldout, ws, wss, nvec, beta, stream_);
 		launch_splitk_reduce<double, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
@@ -1125,12 +1121,12 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		launch_prep_x<float>(d_prep_tiles_, npt, d_prep_patches_, vin, ldin, xp, xt, vs, nvec,
 		                     stream_);
 		const float* bases[PBASE_COUNT] = { d_apack32_, d_bpack32_, xp, xt, bx };
-		launch_panel_gemm<float, float>(st1_.bm, st1_.bn, v1, dp, st1_.d_tasks, st1_.d_segs,
+		launch_panel_gemm<float, float>(st1_.bm, st1_.bn, v1, dp, kc, st1_.d_tasks, st1_.d_segs,
 		                                st1_.d_tiles, int(st1_.tiles.size()), bases, vs, bx, bxs,
 		                                bx, bxs, nvec, 0, stream_);
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
-		launch_panel_gemm<float, double>(st2_.bm, st2_.bn, v1, dp, st2_.d_tasks, st2_.d_segs,
+		launch_panel_gemm<float, double>(st2_.bm, st2_.bn, v1, dp, kc, st2_.d_tasks, st2_.d_segs,
 		                                 st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
 		                                 ldout, ws, wss, nvec, beta, stream_);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
@@ -1236,7 +1232,12 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		for (const Tuning& t : cands) {
 			if (t == best)
 				continue;
-			const double ms = time_of(t);
+			double ms = 0;
+			try {
+				ms = time_of(t);
+			} catch (const std::invalid_argument&) {
+				continue; // a combination this mode does not take (e.g. kchunk 2 with 128 tiles)
+			}
 			seen.emplace_back(ms, t);
 			if (ms < best_ms) {
 				best_ms = ms;
@@ -1269,6 +1270,15 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 			t.depth  = d;
 			c.push_back(t);
 		}
+		try_set(c);
+		c.clear();
+		// K chunk depth (FP64 / FP32).
+		if (precision_ != PREC_BF16X3)
+			for (int kc : { 1, 2 }) {
+				Tuning t = best;
+				t.kchunk = kc;
+				c.push_back(t);
+			}
 		try_set(c);
 		c.clear();
 		for (const auto& sh : tile_shapes) {

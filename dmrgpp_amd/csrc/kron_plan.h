@@ -60,11 +60,12 @@ struct Tuning {
 	int xcd    = 0; // 1: XCD-aware tile order
 	int mfma32 = 0; // 1: 32x32 MFMA blocks (FP32 32x32x2, BF16x3 32x32x16); FP64 ignores it
 	int depth  = 1; // global loads issued this many K chunks ahead (1 or 2)
+	int kchunk = 1; // K chunk depth in units of the mode's base chunk (1 or 2)
 
 	bool operator==(const Tuning& o) const
 	{
 		return bm1 == o.bm1 && bn1 == o.bn1 && bm2 == o.bm2 && bn2 == o.bn2 && splitk == o.splitk
-		       && xcd == o.xcd && mfma32 == o.mfma32 && depth == o.depth;
+		       && xcd == o.xcd && mfma32 == o.mfma32 && depth == o.depth && kchunk == o.kchunk;
 	}
 };
 

@@ -253,7 +253,7 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 // its stage-2 output in FP64 for the FP64 Lanczos recurrence). Workspace tasks (stage-1 panels,
 // split-k partials) store T and always overwrite.
 template <typename TO, int BM, int BN, int BK, int PF, typename Op>
-__global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 ? 5 : 1)) void panel_gemm_kernel(
+__global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK == 32 ? 5 : 1)) void panel_gemm_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
     PBasesT<typename Op::T> bases, PStrides vs, TO* __restrict__ out_y, int64_t ystride,
     typename Op::T* __restrict__ out_ws, int64_t wsstride, int beta)
@@ -774,8 +774,10 @@ void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hip
 
 // Host-side launchers -------------------------------------------------------------------------
 
+// kchunk 2 doubles the K chunk (FP64 32, FP32 64 elements per LDS stage: twice the MFMAs per
+// barrier, 64 KB of LDS for tiles with bm + bn <= 128).
 template <typename T, typename TO>
-void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* tasks,
+void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const PTask* tasks,
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
                        const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
                        int64_t wsstride, int nvec, int beta, hipStream_t stream)
@@ -834,8 +836,34 @@ void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* task
 		DMK_PLAUNCH(32, 128, BK, OP);                                                      \
 	else                                                                                       \
 		throw std::runtime_error("panel gemm: unsupported tile shape");
+	// Double-depth K chunks: tiles of at most 64 + 64 rows.
+#define DMK_PSHAPES_LONG(BK, OP)                                                                   \
+	if (bm == 64 && bn == 64)                                                                  \
+		DMK_PLAUNCH(64, 64, BK, OP);                                                       \
+	else if (bm == 64 && bn == 32)                                                             \
+		DMK_PLAUNCH(64, 32, BK, OP);                                                       \
+	else if (bm == 32 && bn == 64)                                                             \
+		DMK_PLAUNCH(32, 64, BK, OP);                                                       \
+	else if (bm == 32 && bn == 32)                                                             \
+		DMK_PLAUNCH(32, 32, BK, OP);                                                       \
+	else                                                                                       \
+		throw std::runtime_error("panel gemm: unsupported tile shape for double K chunks");
 	const bool s96 = bm == 96 || bn == 96;
 	const bool s32 = bm == 32 || bn == 32;
+	if (kchunk == 2) {
+		if constexpr (std::is_same<T, double>::value) {
+			DMK_PSHAPES_LONG(32, OpF64)
+		} else {
+			if (variant == 1 && bm == 64 && bn == 64) {
+				DMK_PLAUNCH(64, 64, 64, OpF32x32);
+			} else {
+				DMK_PSHAPES_LONG(64, OpF32)
+			}
+		}
+		return;
+	}
+	if (kchunk != 1)
+		throw std::runtime_error("panel gemm: kchunk must be 1 or 2");
 	if constexpr (std::is_same<T, double>::value) {
 		if (s96) {
 			DMK_PSHAPES96(16, OpF64)
@@ -855,21 +883,22 @@ void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* task
 			DMK_PSHAPES(32, OpF32)
 		}
 	}
+#undef DMK_PSHAPES_LONG
 #undef DMK_PSHAPES32
 #undef DMK_PSHAPES96
 #undef DMK_PSHAPES
 #undef DMK_PLAUNCH
 }
 
-template void launch_panel_gemm<double, double>(int, int, int, int, const PTask*, const PSeg*,
+template void launch_panel_gemm<double, double>(int, int, int, int, int, const PTask*, const PSeg*,
                                                 const Tile*, int, const double* const*,
                                                 const PStrides&, double*, int64_t, double*,
                                                 int64_t, int, int, hipStream_t);
-template void launch_panel_gemm<float, float>(int, int, int, int, const PTask*, const PSeg*,
+template void launch_panel_gemm<float, float>(int, int, int, int, int, const PTask*, const PSeg*,
                                               const Tile*, int, const float* const*,
                                               const PStrides&, float*, int64_t, float*, int64_t,
                                               int, int, hipStream_t);
-template void launch_panel_gemm<float, double>(int, int, int, int, const PTask*, const PSeg*,
+template void launch_panel_gemm<float, double>(int, int, int, int, int, const PTask*, const PSeg*,
                                                const Tile*, int, const float* const*,
                                                const PStrides&, double*, int64_t, float*, int64_t,
                                                int, int, hipStream_t);

@@ -27,7 +27,7 @@
 namespace dmk {
 
 template <typename T, typename TO>
-void launch_panel_gemm(int bm, int bn, int variant, int depth, const PTask* tasks,
+void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const PTask* tasks,
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
                        const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
                        int64_t wsstride, int nvec, int beta, hipStream_t stream);
@@ -273,7 +273,7 @@ void gemm_vbatch(bool cplx, char transa, char transb, const int* m, const int* n
 		launch_vb_copy(d_copies, int(copies.size()), ws, s);
 		const double*  bases[PBASE_COUNT] = { ws, ws, ws, ws, ws };
 		const PStrides vs { 0, 0, 0 };
-		launch_panel_gemm<double, double>(64, 64, 0, 2, d_tasks, d_segs, d_tiles,
+		launch_panel_gemm<double, double>(64, 64, 0, 2, 1, d_tasks, d_segs, d_tiles,
 		                                  int(tiles.size()), bases, vs, cw, 0, nullptr, 0, 1, 0, s);
 		DMK_HIP_CHECK(hipGetLastError());
 		launch_vb_scatter(d_outs, int(outs.size()), cw, alpha[0], cplx ? alpha[1] : 0.0, beta[0],

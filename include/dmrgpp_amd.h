@@ -72,9 +72,8 @@ typedef struct dmk_info {
 	int32_t pad2;
 	int64_t n_complex;         /* complex vector length (equals n for a real handle) */
 	double  ms_scan;           /* handle creation, host wall ms: block scan (nonzero, identity) */
-	double  ms_stage;          /*   copy of the uploaded blocks into pinned staging, chunk by
-	                              chunk, each chunk's upload queued as it is filled */
-	double  ms_upload;         /*   the rest of the upload and the panel packing on the device */
+	double  ms_stage;          /*   copy of the uploaded blocks into pinned staging */
+	double  ms_upload;         /*   upload and panel packing on the device */
 	double  raw_bytes;         /*   bytes uploaded */
 } dmk_info;
 
@@ -93,6 +92,9 @@ typedef struct dmk_tuning {
 	int32_t mfma32;            /* 1 = 32x32 MFMA blocks (FP32: 32x32x2, BF16x3: 32x32x16), 0 = 16x16 */
 	int32_t prefetch;          /* 1 or 2 K chunks ahead (0 = 1); BF16x3 honours 2 in the 32x32x16 form
 	                              only */
+	int32_t kchunk;            /* K chunk depth: 1 (0 = 1) = 16 (FP64) / 32 (FP32, BF16x3) elements
+	                              per LDS stage, 2 = twice that (FP64 / FP32, tiles with
+	                              tile_m + tile_n <= 128 in both stages) */
 } dmk_tuning;
 
 /* Replaces BatchedGemm<double>::BatchedGemm(noperator, npatches, left_patch_size,

@@ -234,6 +234,11 @@ TUNINGS = [
          xcd_order=0, mfma32=0, prefetch=1),
     dict(stage1_tile_m=32, stage1_tile_n=128, stage2_tile_m=32, stage2_tile_n=32, splitk=32,
          xcd_order=1, mfma32=0, prefetch=2),
+    # Double-depth K chunks (FP64 / FP32; BF16x3 rejects them), both MFMA forms, forced split-k.
+    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=48,
+         xcd_order=1, mfma32=1, prefetch=2, kchunk=2),
+    dict(stage1_tile_m=32, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=32, splitk=0,
+         xcd_order=0, mfma32=0, prefetch=1, kchunk=2),
 ]
 
 
@@ -246,7 +251,7 @@ def test_gpu_tunings_match_oracle(tuning, prec, tol):
     # is bitwise identical.
     shapes = (tuning["stage1_tile_m"], tuning["stage1_tile_n"], tuning["stage2_tile_m"],
               tuning["stage2_tile_n"])
-    if prec == "bf16x3" and (96 in shapes or 32 in shapes):
+    if prec == "bf16x3" and (96 in shapes or 32 in shapes or tuning.get("kchunk", 1) == 2):
         op = dm.BatchedGemm(problems.random_problem([5, 7], [6, 3], 2, seed=1), prec)
         with pytest.raises(dm.DmkError):
             op.set_tuning(**tuning)
@@ -260,7 +265,7 @@ def test_gpu_tunings_match_oracle(tuning, prec, tol):
         want = oracle.plugin_apply(p, x)
         op = dm.BatchedGemm(p, prec)
         op.set_tuning(**tuning)
-        assert op.tuning() == tuning
+        assert op.tuning() == dict({"kchunk": 1}, **tuning)
         if tuning["splitk"]:
             assert op.info()["split_tasks"] > 0
         dx, dy = dm.DeviceArray.from_host(x), dm.DeviceArray(p.n)
@@ -309,7 +314,7 @@ def test_gpu_tuning_belongs_to_its_precision():
     op = dm.BatchedGemm(prob, "fp64")
     t64 = op.tuning()
     big = dict(t64, stage1_tile_m=128, stage1_tile_n=128, stage2_tile_m=128, stage2_tile_n=128,
-               splitk=0, mfma32=1)
+               splitk=0, mfma32=1, kchunk=1)
     op.set_precision("fp32")
     op.set_tuning(**big)
     op.set_precision("fp64")
