@@ -43,14 +43,14 @@ class DmkInfo(ctypes.Structure):
         ("flops_stage2", ctypes.c_double), ("bytes_compulsory", ctypes.c_double),
         ("bytes_bx", ctypes.c_double), ("precision", ctypes.c_int32), ("pad", ctypes.c_int32),
         ("split_tasks", ctypes.c_int64), ("ws_elems", ctypes.c_int64),
-        ("is_complex", ctypes.c_int32), ("pad2", ctypes.c_int32), ("n_complex", ctypes.c_int64),
+        ("is_complex", ctypes.c_int32), ("device_blocks", ctypes.c_int32), ("n_complex", ctypes.c_int64),
         ("ms_scan", ctypes.c_double), ("ms_stage", ctypes.c_double),
         ("ms_upload", ctypes.c_double), ("raw_bytes", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_
-                if name not in ("pad", "pad2")}
+                if name != "pad"}
 
 
 class Tuning(ctypes.Structure):
@@ -497,12 +497,57 @@ class DeviceArray:
             pass
 
 
+def _device_plugin_args(prob: KronProblem, args):
+    """The plugin arguments of a real problem with every distinct block copied, column major,
+    into one device allocation: the same argument arrays with device pointers (and leading
+    dimension = rows). Returns (args, the DeviceArray holding the blocks)."""
+    if prob.dtype != np.float64:
+        raise DmkError("device-resident blocks: real (float64) problems only")
+    offs, host = {}, []
+    at = 0
+    for a, b in prob.blocks.values():
+        for m in (a, b):
+            if id(m) not in offs:
+                offs[id(m)] = at
+                host.append(np.asarray(m, dtype=np.float64).ravel(order="F"))
+                at += m.size
+    dev = DeviceArray(max(at, 1))
+    if host:
+        dev.upload(np.concatenate(host))
+    pa, pb = np.array(args._arrays[2]), np.array(args._arrays[3])
+    lda, ldb = np.array(args._arrays[0]), np.array(args._arrays[1])
+    for (I, J, k), (a, b) in prob.blocks.items():
+        i = prob.index(I, J, k)
+        pa[i] = dev.ptr + 8 * offs[id(a)]
+        pb[i] = dev.ptr + 8 * offs[id(b)]
+        lda[i] = a.shape[0]
+        ldb[i] = b.shape[0]
+
+    class _Args:
+        pass
+
+    d = _Args()
+    d.noperator, d.npatches, d.L, d.R = args.noperator, args.npatches, args.L, args.R
+    total = pa.size
+    d._arrays = (lda, ldb, pa, pb)
+    d.A = (ctypes.c_void_p * total).from_buffer(pa)
+    d.B = (ctypes.c_void_p * total).from_buffer(pb)
+    d.ldA = (ctypes.c_size_t * total).from_buffer(lda)
+    d.ldB = (ctypes.c_size_t * total).from_buffer(ldb)
+    return d, dev
+
+
 class BatchedGemm:
     """Mirror of ``BatchedGemm<T>`` (dmrg/GPUPlugin/BatchedGemm.h:11-56) over the C ABI, for
     T = double (dmk_create_d) and T = std::complex<double> (dmk_create_z, chosen by the
     problem's dtype)."""
 
-    def __init__(self, prob: KronProblem, precision: str | int = "fp64"):
+    def __init__(self, prob: KronProblem, precision: str | int = "fp64",
+                 device_blocks: bool = False):
+        """device_blocks: copy the distinct blocks into one device allocation first and hand the
+        plugin device pointers (the blocks of a caller that keeps its block operators in HBM:
+        the handle scans and packs them on the device, with no staging or upload). Real
+        problems only."""
         if prob.dtype == np.float64:
             create, self.is_complex = lib().dmk_create_d, False
         elif prob.dtype == np.complex128:
@@ -510,6 +555,9 @@ class BatchedGemm:
         else:
             raise DmkError(f"unsupported scalar type {prob.dtype}")
         args = prob.plugin_args()
+        self._blocks_dev = None
+        if device_blocks:
+            args, self._blocks_dev = _device_plugin_args(prob, args)
         h = ctypes.c_void_p()
         _check(create(args.noperator, args.npatches, ctypes.addressof(args.L),
                       ctypes.addressof(args.R), ctypes.addressof(args.A),
@@ -630,6 +678,9 @@ class BatchedGemm:
         if getattr(self, "handle", None):
             lib().dmk_destroy(self.handle)
             self.handle = None
+        if getattr(self, "_blocks_dev", None) is not None:
+            self._blocks_dev.free()
+            self._blocks_dev = None
 
     def __del__(self):
         try:

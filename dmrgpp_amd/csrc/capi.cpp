@@ -130,6 +130,11 @@ int dmk_create_z(size_t noperator, size_t npatches, const size_t* left_patch_siz
 		if (!handle)
 			throw std::invalid_argument("dmk_create_z: null handle pointer");
 		*handle = nullptr;
+		if (!Amatrix || !Bmatrix)
+			throw std::invalid_argument("dmk_create_z: null argument array");
+		if (dmk::blocks_on_device(int64_t(noperator), int64_t(npatches), Amatrix, Bmatrix))
+			throw std::invalid_argument("dmk_create_z: complex blocks must be host memory "
+			                            "(device-resident blocks: dmk_create_d)");
 		dmk::RealEmbedding e = dmk::embed_complex(int64_t(noperator), int64_t(npatches),
 		                                          left_patch_size, right_patch_size, Amatrix,
 		                                          ld_Amatrix, Bmatrix, ld_Bmatrix);
@@ -372,6 +377,7 @@ int dmk_get_info(const dmk_handle* handle, dmk_info* info)
 		info->split_tasks       = s.split_tasks;
 		info->ws_elems          = s.ws_elems;
 		info->is_complex        = handle->is_complex ? 1 : 0;
+		info->device_blocks     = handle->plan->device_blocks() ? 1 : 0;
 		info->n_complex         = handle->nc;
 		info->ms_scan           = s.ms_scan;
 		info->ms_stage          = s.ms_stage;

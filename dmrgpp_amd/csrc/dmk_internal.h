@@ -99,24 +99,44 @@ struct PrepPatch {
 	int32_t ldt;
 };
 
-// Device-side packing of the factors (initkron_kernels.hip): block (rows x cols, compact column
-// major at raw_off in the uploaded blocks) written as panel rows at dst with row stride lddst,
-// then `tail` (< 64) zeros after each row.
+// Device-side packing of the factors (initkron_kernels.hip): block (rows x cols, column major at
+// src with leading dimension ldsrc: in the uploaded copy of host blocks, or the caller's
+// device-resident block itself) written as panel rows at dst with row stride lddst, then `tail`
+// (< 64) zeros after each row.
 struct PackJob {
-	int64_t raw_off;
-	int64_t dst;
-	int32_t rows;
-	int32_t cols;
-	int32_t lddst;
-	int32_t tail;
-	int32_t to_b; // 0: A pack, 1: B pack
-	int32_t pad;
+	const double* src;
+	int64_t       ldsrc;
+	int64_t       dst;
+	int32_t       rows;
+	int32_t       cols;
+	int32_t       lddst;
+	int32_t       tail;
+	int32_t       to_b; // 0: A pack, 1: B pack
+	int32_t       pad;
 };
 struct PackTile {
 	int32_t job;
 	int32_t i0;
 	int32_t j0;
 	int32_t pad;
+};
+
+// Scan of a device-resident block (64 x 64 tiles listed as PackTile with job = block index):
+// flags[2 d] becomes nonzero when block d holds a nonzero element, flags[2 d + 1] when it departs
+// from the identity pattern (a diagonal element other than 1, an off-diagonal one other than 0).
+struct ScanBlock {
+	const double* p;
+	int64_t       ld;
+	int32_t       rows;
+	int32_t       cols;
+};
+
+// Gather of `count` diagonal elements p[t * (ld + 1)] to out + off.
+struct DiagJob {
+	const double* p;
+	int64_t       ld;
+	int64_t       off;
+	int64_t       count;
 };
 
 // Rows of K-major panels are padded to this many elements (16 bytes of bf16).

@@ -12,6 +12,7 @@
 #include "dmrgpp_amd.h"
 
 #include "dmk_internal.h"
+#include "kron_plan.h"
 #include "pool.h"
 
 #include <chrono>
@@ -207,6 +208,9 @@ int create(bool cplx, size_t noperator, size_t npatches, const size_t* L, const 
 			throw std::invalid_argument("dmk_facade_create: null or empty problem");
 		if (max_rank_stored < 0)
 			throw std::invalid_argument("dmk_facade_create: max_rank_stored must be >= 0");
+		// The stored matrix and the adjoint blocks are assembled on the host.
+		if (dmk::blocks_on_device(int64_t(noperator), int64_t(npatches), A, B))
+			throw std::invalid_argument("dmk_facade_create: the blocks must be host memory");
 		auto f      = std::make_unique<dmk_facade>();
 		f->complex  = cplx;
 		f->ldaggerl = ldaggerl != 0;

@@ -57,8 +57,11 @@ void launch_splitk_reduce(const Reduce* red, const ReduceChunk* chunks, int nchu
                           int64_t wsstride, TO* out, int64_t ystride, int nvec, int beta,
                           hipStream_t stream);
 void launch_cast_f64_to_f32(const double* x, float* y, int64_t n, hipStream_t stream);
-void launch_pack_panels(const PackJob* jobs, const PackTile* tiles, int ntiles, const double* raw,
-                        double* apack, double* bpack, hipStream_t stream);
+void launch_pack_panels(const PackJob* jobs, const PackTile* tiles, int ntiles, double* apack,
+                        double* bpack, hipStream_t stream);
+void launch_scan_blocks(const ScanBlock* blocks, const PackTile* tiles, int ntiles, int* flags,
+                        hipStream_t stream);
+void launch_gather_diag(const DiagJob* jobs, int njobs, double* out, hipStream_t stream);
 void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hipStream_t stream);
 
 namespace {
@@ -96,6 +99,27 @@ bool is_identity(const double* p, int64_t rows, int64_t cols, int64_t ld)
 				return false;
 	}
 	return true;
+}
+
+// Pooled device memory released when the scope ends (plan construction scratch).
+struct DeviceScratch {
+	void* p = nullptr;
+	DeviceScratch() = default;
+	DeviceScratch(const DeviceScratch&) = delete;
+	DeviceScratch& operator=(const DeviceScratch&) = delete;
+	~DeviceScratch() { pool_device_free(p); }
+};
+
+// Device memory or managed memory (both read by kernels in place); pageable and pinned host
+// memory are neither.
+bool device_resident(const void* p)
+{
+	hipPointerAttribute_t a {};
+	const hipError_t      e = hipPointerGetAttributes(&a, p);
+	(void)hipGetLastError(); // unregistered host memory makes the query fail
+	if (e != hipSuccess)
+		return false;
+	return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.isManaged;
 }
 
 template <typename X> X* dev_alloc_copy(const std::vector<X>& h)
@@ -223,6 +247,28 @@ std::unordered_map<std::string, Tuning> g_tune_cache;
 
 } // namespace
 
+bool blocks_on_device(int64_t noperator, int64_t npatches, const double* const* Amatrix,
+                      const double* const* Bmatrix)
+{
+	const size_t nb = size_t(npatches * npatches * noperator);
+	int          host = 0, dev = 0;
+	std::unordered_map<const void*, bool> seen;
+	for (size_t id = 0; id < nb; ++id) {
+		if (!Amatrix[id] || !Bmatrix[id])
+			continue;
+		for (const double* p : { Amatrix[id], Bmatrix[id] }) {
+			auto it = seen.find(p);
+			if (it == seen.end())
+				it = seen.emplace(p, device_resident(p)).first;
+			(it->second ? dev : host)++;
+		}
+	}
+	if (host && dev)
+		throw std::invalid_argument("dmk_create: the blocks must all be host memory or all device "
+		                            "(or managed) memory");
+	return dev > 0;
+}
+
 bool cached_tuning(const std::string& key, Tuning& t)
 {
 	std::lock_guard<std::mutex> lock(g_tune_mutex);
@@ -345,11 +391,40 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 		a_of[t]          = distinct(Amatrix[id], L[I], L[J], int64_t(ldA[id]));
 		b_of[t]          = distinct(Bmatrix[id], R[I], R[J], int64_t(ldB[id]));
 	}
-	parallel_for(blocks.size(), [&](size_t d) {
-		Distinct& b = blocks[d];
-		b.nonzero   = has_nonzero(b.p, b.rows, b.cols, b.ld) ? 1 : 0;
-		b.identity  = (b.nonzero && is_identity(b.p, b.rows, b.cols, b.ld)) ? 1 : 0;
-	});
+	device_blocks_ = blocks_on_device(nC, np, Amatrix, Bmatrix);
+	if (device_blocks_) {
+		// The same two tests on the device, over every element of every distinct block.
+		std::vector<ScanBlock> sb(blocks.size());
+		std::vector<PackTile>  st;
+		for (size_t d = 0; d < blocks.size(); ++d) {
+			const Distinct& b = blocks[d];
+			sb[d] = ScanBlock { b.p, b.ld, int32_t(b.rows), int32_t(b.cols) };
+			for (int32_t i0 = 0; i0 < b.rows; i0 += 64)
+				for (int32_t j0 = 0; j0 < b.cols; j0 += 64)
+					st.push_back(PackTile { int32_t(d), i0, j0, 0 });
+		}
+		std::vector<int> fl(2 * blocks.size(), 0);
+		DeviceScratch    dsb, dst, dfl;
+		dsb.p = dev_alloc_copy(sb);
+		dst.p = dev_alloc_copy(st);
+		dfl.p = dev_alloc_copy(fl);
+		launch_scan_blocks(static_cast<const ScanBlock*>(dsb.p),
+		                   static_cast<const PackTile*>(dst.p), int(st.size()),
+		                   static_cast<int*>(dfl.p), nullptr);
+		DMK_HIP_CHECK(hipGetLastError());
+		DMK_HIP_CHECK(hipMemcpy(fl.data(), dfl.p, fl.size() * sizeof(int), hipMemcpyDeviceToHost));
+		for (size_t d = 0; d < blocks.size(); ++d) {
+			Distinct& b = blocks[d];
+			b.nonzero   = fl[2 * d] ? 1 : 0;
+			b.identity  = (b.nonzero && !fl[2 * d + 1] && b.rows == b.cols) ? 1 : 0;
+		}
+	} else {
+		parallel_for(blocks.size(), [&](size_t d) {
+			Distinct& b = blocks[d];
+			b.nonzero   = has_nonzero(b.p, b.rows, b.cols, b.ld) ? 1 : 0;
+			b.identity  = (b.nonzero && is_identity(b.p, b.rows, b.cols, b.ld)) ? 1 : 0;
+		});
+	}
 	std::vector<char> scan(present.size(), NONE);
 	for (size_t t = 0; t < present.size(); ++t) {
 		const Distinct& a = blocks[size_t(a_of[t])];
@@ -400,6 +475,8 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	pL_.resize(size_t(np));
 	pR_.resize(size_t(np));
 	dg_begin_.assign(size_t(np + 1), 0);
+	std::vector<DiagJob> diag_jobs;
+	int64_t              ndiag = 0;
 	for (int64_t I = 0; I < np; ++I) {
 		pL_[size_t(I)]      = int32_t(L[I]);
 		pR_[size_t(I)]      = int32_t(R[I]);
@@ -408,14 +485,32 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			const size_t id = idx(I, I, k);
 			if (kind[id] == NONE)
 				continue;
-			dg_off_.push_back(int64_t(dg_vals_.size()));
-			for (int64_t l = 0; l < L[I]; ++l)
-				dg_vals_.push_back(Amatrix[id][l + l * int64_t(ldA[id])]);
-			for (int64_t r = 0; r < R[I]; ++r)
-				dg_vals_.push_back(Bmatrix[id][r + r * int64_t(ldB[id])]);
+			dg_off_.push_back(ndiag);
+			if (device_blocks_) {
+				diag_jobs.push_back(DiagJob { Amatrix[id], int64_t(ldA[id]), ndiag, L[I] });
+				diag_jobs.push_back(
+				    DiagJob { Bmatrix[id], int64_t(ldB[id]), ndiag + L[I], R[I] });
+			} else {
+				for (int64_t l = 0; l < L[I]; ++l)
+					dg_vals_.push_back(Amatrix[id][l + l * int64_t(ldA[id])]);
+				for (int64_t r = 0; r < R[I]; ++r)
+					dg_vals_.push_back(Bmatrix[id][r + r * int64_t(ldB[id])]);
+			}
+			ndiag += L[I] + R[I];
 		}
 	}
 	dg_begin_[size_t(np)] = int32_t(dg_off_.size());
+	if (device_blocks_ && ndiag > 0) {
+		dg_vals_.resize(size_t(ndiag));
+		DeviceScratch dj, dv;
+		dj.p = dev_alloc_copy(diag_jobs);
+		dv.p = pool_device_alloc(size_t(ndiag) * sizeof(double));
+		launch_gather_diag(static_cast<const DiagJob*>(dj.p), int(diag_jobs.size()),
+		                   static_cast<double*>(dv.p), nullptr);
+		DMK_HIP_CHECK(hipGetLastError());
+		DMK_HIP_CHECK(hipMemcpy(dg_vals_.data(), dv.p, size_t(ndiag) * sizeof(double),
+		                        hipMemcpyDeviceToHost));
+	}
 
 	// Packing, K-major (rows padded to kPanelPad elements, zero padding):
 	//  * A panel of patch I, the stage-2 left operand of Y_I^T: row n (< L_I) holds the K_I
@@ -607,26 +702,28 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	DMK_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 	for (auto& e : ev_)
 		DMK_HIP_CHECK(hipEventCreate(&e));
-	// Upload every distinct block a copy job reads, compact and column major, through pinned
-	// staging (filled in parallel), then build the panels on the device.
+	// Host blocks: upload every distinct block a copy job reads, compact and column major,
+	// through pinned staging (filled in parallel). Device-resident blocks are read in place.
+	// Then the panels are built on the device.
 	const auto t_stage0 = std::chrono::steady_clock::now();
 	int64_t    raw_elems = 0;
-	for (const Copy& c : copies) {
-		Distinct& b = blocks[size_t(c.src)];
-		if (b.raw_off < 0) {
-			b.raw_off = raw_elems;
-			raw_elems += b.rows * b.cols;
+	if (!device_blocks_)
+		for (const Copy& c : copies) {
+			Distinct& b = blocks[size_t(c.src)];
+			if (b.raw_off < 0) {
+				b.raw_off = raw_elems;
+				raw_elems += b.rows * b.cols;
+			}
 		}
-	}
 	std::vector<int32_t> uploaded;
 	for (size_t d = 0; d < blocks.size(); ++d)
 		if (blocks[d].raw_off >= 0)
 			uploaded.push_back(int32_t(d));
-	const size_t rbytes = size_t(std::max<int64_t>(raw_elems, 1)) * sizeof(double);
+	const size_t rbytes = size_t(raw_elems) * sizeof(double);
 	// One staging copy and one upload: filling the staging in chunks, each chunk's DMA queued as
 	// it was filled, measured slower on the m = 2048 ladder steps (median plan 37 ms against
 	// 15 ms: the DMA and the host copies compete for the host's memory bandwidth).
-	double*      hraw   = static_cast<double*>(pool_host_alloc(rbytes));
+	double* hraw = rbytes ? static_cast<double*>(pool_host_alloc(rbytes)) : nullptr;
 	parallel_for(uploaded.size(), [&](size_t u) {
 		const Distinct& b = blocks[size_t(uploaded[u])];
 		double*         d = hraw + b.raw_off;
@@ -636,24 +733,6 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 			for (int64_t j = 0; j < b.cols; ++j)
 				std::memcpy(d + j * b.rows, b.p + j * b.ld, size_t(b.rows) * sizeof(double));
 	});
-	std::vector<PackJob>  jobs;
-	std::vector<PackTile> ptiles;
-	jobs.reserve(copies.size());
-	for (const Copy& c : copies) {
-		PackJob j {};
-		j.raw_off = blocks[size_t(c.src)].raw_off;
-		j.dst     = c.dst;
-		j.rows    = int32_t(c.rows);
-		j.cols    = int32_t(c.cols);
-		j.lddst   = int32_t(c.lddst);
-		j.tail    = int32_t(c.tail);
-		j.to_b    = c.to_b;
-		const int32_t jid = int32_t(jobs.size());
-		jobs.push_back(j);
-		for (int32_t i0 = 0; i0 < j.rows; i0 += 64)
-			for (int32_t j0 = 0; j0 < std::max(j.cols, 1); j0 += 64)
-				ptiles.push_back(PackTile { jid, i0, j0, 0 });
-	}
 	stats_.ms_stage = ms_since(t_stage0);
 	const auto t_up0 = std::chrono::steady_clock::now();
 	const size_t abytes = size_t(std::max<int64_t>(asize, 1)) * sizeof(double);
@@ -661,12 +740,33 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	double*   draw   = nullptr;
 	PackJob*  djobs  = nullptr;
 	PackTile* dtiles = nullptr;
+	std::vector<PackTile> ptiles;
 	try {
 		d_apack_ = static_cast<double*>(pool_device_alloc(abytes));
 		d_bpack_ = static_cast<double*>(pool_device_alloc(bbytes));
-		draw     = static_cast<double*>(pool_device_alloc(rbytes));
-		djobs    = dev_alloc_copy(jobs);
-		dtiles   = dev_alloc_copy(ptiles);
+		if (rbytes)
+			draw = static_cast<double*>(pool_device_alloc(rbytes));
+		std::vector<PackJob> jobs;
+		jobs.reserve(copies.size());
+		for (const Copy& c : copies) {
+			const Distinct& b = blocks[size_t(c.src)];
+			PackJob         j {};
+			j.src   = device_blocks_ ? b.p : draw + b.raw_off;
+			j.ldsrc = device_blocks_ ? b.ld : b.rows;
+			j.dst   = c.dst;
+			j.rows  = int32_t(c.rows);
+			j.cols  = int32_t(c.cols);
+			j.lddst = int32_t(c.lddst);
+			j.tail  = int32_t(c.tail);
+			j.to_b  = c.to_b;
+			const int32_t jid = int32_t(jobs.size());
+			jobs.push_back(j);
+			for (int32_t i0 = 0; i0 < j.rows; i0 += 64)
+				for (int32_t j0 = 0; j0 < std::max(j.cols, 1); j0 += 64)
+					ptiles.push_back(PackTile { jid, i0, j0, 0 });
+		}
+		djobs  = dev_alloc_copy(jobs);
+		dtiles = dev_alloc_copy(ptiles);
 	} catch (...) {
 		// A throwing constructor runs no destructor: everything goes back to its pool here.
 		pool_device_release(d_apack_);
@@ -677,9 +777,10 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 		pool_device_release(dtiles);
 		throw;
 	}
-	hipError_t e1 = hipMemcpyAsync(draw, hraw, rbytes, hipMemcpyHostToDevice, stream_);
+	hipError_t e1 = rbytes ? hipMemcpyAsync(draw, hraw, rbytes, hipMemcpyHostToDevice, stream_)
+	                       : hipSuccess;
 	if (e1 == hipSuccess)
-		launch_pack_panels(djobs, dtiles, int(ptiles.size()), draw, d_apack_, d_bpack_, stream_);
+		launch_pack_panels(djobs, dtiles, int(ptiles.size()), d_apack_, d_bpack_, stream_);
 	hipError_t e2 = hipGetLastError();
 	hipError_t e3 = hipStreamSynchronize(stream_);
 	pool_host_free(hraw);

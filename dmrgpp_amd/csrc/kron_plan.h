@@ -85,13 +85,20 @@ struct StageTables {
 // The starting tuning of a precision mode (before any autotuning).
 Tuning default_tuning(int precision);
 
+// Whether the present blocks (both factors non-null) live in device or managed memory: true when
+// all do, false when all are host memory; throws std::invalid_argument when they are mixed.
+bool blocks_on_device(int64_t noperator, int64_t npatches, const double* const* Amatrix,
+                      const double* const* Bmatrix);
+
 // Process-wide autotuner cache (histogram signature -> tuning).
 bool cached_tuning(const std::string& key, Tuning& t);
 void store_tuning(const std::string& key, const Tuning& t);
 
 class Plan {
 public:
-	// Same inputs as BatchedGemm<double>::BatchedGemm (GPUPlugin/BatchedGemm.h:21-28).
+	// Same inputs as BatchedGemm<double>::BatchedGemm (GPUPlugin/BatchedGemm.h:21-28). The
+	// blocks are all host memory (scanned on the host, staged and uploaded) or all device /
+	// managed memory (scanned and packed on the device, read in place; no upload).
 	Plan(int64_t noperator, int64_t npatches, const size_t* left_patch_size,
 	     const size_t* right_patch_size, const double* const* Amatrix, const size_t* ld_Amatrix,
 	     const double* const* Bmatrix, const size_t* ld_Bmatrix);
@@ -135,6 +142,7 @@ public:
 	const PlanStats& stats() const { return stats_; }
 	hipStream_t      stream() const { return stream_; }
 	int64_t          n() const { return stats_.n; }
+	bool             device_blocks() const { return device_blocks_; }
 
 	// diag(H) into out (device, n): sum over the diagonal-patch terms (I, I, k) of
 	// diag(A_k(I,I)) (x) diag(B_k(I,I)). Enqueued on the plan's stream.
@@ -153,7 +161,8 @@ private:
 	void launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, int nvec, int beta,
 	            hipEvent_t mid, void* bx, void* ws, void* xin);
 
-	int         precision_ = PREC_FP64;
+	int         precision_     = PREC_FP64;
+	bool        device_blocks_ = false;
 	Tuning      tuning_;
 	Tuning      tuning_of_[3]; // per precision mode
 	PlanStats   stats_;

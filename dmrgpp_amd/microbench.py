@@ -11,8 +11,11 @@
   python -m dmrgpp_amd.microbench --replay F.npz --multi 1,2,4,8 [--iters 12]
   python -m dmrgpp_amd.microbench --density 1093,986,654,538 [--iters 5]
   python -m dmrgpp_amd.microbench --vbatch [--iters 30]
+  python -m dmrgpp_amd.microbench --replay F.npz --plan [--iters 10]
 
 
+With --plan, the replayed superblock's handle is created from host blocks and from the same
+blocks already in device memory, and the creation times are reported (scan, staging, upload).
 Each case reports the average matvec time from HIP events (one warm-up, then iters calls) and
 the algorithmic TFLOP/s in FP64, FP32 and BF16x3. With --multi, the replayed superblock is
 multiplied into several vectors per launch (dmk_apply_multi_d, the batched form of the Krylov
@@ -114,6 +117,49 @@ def run_vbatch(iters, log):
     return out
 
 
+def run_plan(prob, iters, log):
+    """Handle creation (dmk_create_d: scan, staging, upload, packing) from host blocks and from
+    the same blocks already in device memory (scan and packing on the device, nothing staged or
+    uploaded): wall clock of the create call and the handle's own breakdown, medians over iters
+    creations after one warm-up each. The copy of the blocks into device memory is the
+    caller's and is not timed."""
+    import ctypes
+
+    from . import DmkInfo, _check, _device_plugin_args, lib
+
+    out = {}
+    for mode in ("host", "device"):
+        args = prob.plugin_args()
+        dev = None
+        if mode == "device":
+            args, dev = _device_plugin_args(prob, args)
+        rec = []
+        for it in range(iters + 1):
+            h = ctypes.c_void_p()
+            t = time.perf_counter()
+            _check(lib().dmk_create_d(args.noperator, args.npatches, ctypes.addressof(args.L),
+                                      ctypes.addressof(args.R), ctypes.addressof(args.A),
+                                      ctypes.addressof(args.ldA), ctypes.addressof(args.B),
+                                      ctypes.addressof(args.ldB), ctypes.byref(h)),
+                   "dmk_create_d")
+            ms = 1e3 * (time.perf_counter() - t)
+            inf = DmkInfo()
+            _check(lib().dmk_get_info(h.value, ctypes.byref(inf)), "dmk_get_info")
+            info = inf.as_dict()
+            lib().dmk_destroy(h.value)
+            if it:
+                rec.append((ms, info["ms_scan"], info["ms_stage"], info["ms_upload"]))
+        if dev is not None:
+            dev.free()
+        r = np.median(np.array(rec), axis=0)
+        out[mode] = {"ms_create": round(float(r[0]), 2), "ms_scan": round(float(r[1]), 2),
+                     "ms_stage": round(float(r[2]), 2), "ms_upload": round(float(r[3]), 2),
+                     "raw_mib": round(info["raw_bytes"] / 2**20, 1),
+                     "device_blocks": info["device_blocks"]}
+        log(f"plan from {mode} blocks: {out[mode]}")
+    return out
+
+
 def run_density(sizes, iters, log):
     from . import density_diag
 
@@ -158,6 +204,9 @@ def main(argv=None):
                     help="autotune each case and precision (else the default launch shapes)")
     ap.add_argument("--density", default=None, nargs="?", const="1093,986,654,538",
                     help="comma-separated block sizes: time the truncation eigensolver instead")
+    ap.add_argument("--plan", action="store_true",
+                    help="time handle creation on the replayed superblock from host blocks and "
+                         "from device-resident blocks instead")
     a = ap.parse_args(argv)
     tunings = json.load(open(a.load_tuning)) if a.load_tuning else None
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
@@ -167,6 +216,14 @@ def main(argv=None):
     if a.density:
         sizes = [int(c) for c in a.density.split(",")]
         print(json.dumps(run_density(sizes, a.iters, log)), flush=True)
+        return 0
+    if a.plan:
+        from .problem import load_npz
+
+        if not a.replay:
+            ap.error("--plan needs --replay")
+        print(json.dumps(run_plan(load_npz(a.replay, name="superblock"), a.iters, log)),
+              flush=True)
         return 0
     if a.multi:
         from .problem import load_npz

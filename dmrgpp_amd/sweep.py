@@ -449,7 +449,8 @@ class GpuSolver:
                                           "stage2_tiles")}
         r["timing"] = {"plan_ms": 1e3 * (t1 - t0), "lanczos_ms": 1e3 * (t2 - t1),
                        "plan_scan_ms": info["ms_scan"], "plan_stage_ms": info["ms_stage"],
-                       "plan_upload_ms": info["ms_upload"]}
+                       "plan_upload_ms": info["ms_upload"],
+                       "plan_raw_mb": info["raw_bytes"] / 2**20}
         return r["energy"], z, r
 
 
@@ -556,7 +557,8 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
             f"terms={len(prob.blocks):5d} E={e:.12f} steps={rec.lanczos_steps:3d} "
             f"solve={rec.ms_lanczos:7.1f}ms (plan {tm.get('plan_ms', 0):6.1f} "
             f"[scan {tm.get('plan_scan_ms', 0):.1f} stage {tm.get('plan_stage_ms', 0):.1f} "
-            f"upload {tm.get('plan_upload_ms', 0):.1f}] lanczos "
+            f"upload {tm.get('plan_upload_ms', 0):.1f} raw {tm.get('plan_raw_mb', 0):.0f} MiB] "
+            f"lanczos "
             f"{tm.get('lanczos_ms', 0):6.1f}) grow={tm['grow_ms']:6.1f} kron={tm['kron_ms']:6.1f} "
             f"wft={tm['wft_ms']:6.1f} trunc={tm['truncate_ms']:6.1f} disc={disc:.2e}")
         return rec

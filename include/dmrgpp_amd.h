@@ -69,12 +69,15 @@ typedef struct dmk_info {
 	int64_t split_tasks;       /* stage-2 tasks cut into split-k pieces under the current tuning */
 	int64_t ws_elems;          /* split-k partial workspace (elements per vector) */
 	int32_t is_complex;        /* 1: created by dmk_create_z (n above is 2 n_complex) */
-	int32_t pad2;
+	int32_t device_blocks;     /* 1: the blocks were device (or managed) memory, scanned and packed
+	                              in place on the device (no staging, no upload) */
 	int64_t n_complex;         /* complex vector length (equals n for a real handle) */
-	double  ms_scan;           /* handle creation, host wall ms: block scan (nonzero, identity) */
-	double  ms_stage;          /*   copy of the uploaded blocks into pinned staging */
+	double  ms_scan;           /* handle creation, host wall ms: block scan (nonzero, identity;
+	                              on the device for device-resident blocks) */
+	double  ms_stage;          /*   copy of the uploaded blocks into pinned staging (0 for
+	                              device-resident blocks) */
 	double  ms_upload;         /*   upload and panel packing on the device */
-	double  raw_bytes;         /*   bytes uploaded */
+	double  raw_bytes;         /*   bytes uploaded (0 for device-resident blocks) */
 } dmk_info;
 
 /* Launch shape of a handle: output tile (64 or 128 each way; FP64 and FP32 also 96 x 96,
@@ -99,8 +102,11 @@ typedef struct dmk_tuning {
 
 /* Replaces BatchedGemm<double>::BatchedGemm(noperator, npatches, left_patch_size,
  * right_patch_size, Amatrix, ld_Amatrix, Bmatrix, ld_Bmatrix) — dmrg/GPUPlugin/BatchedGemm.h:21-28,
- * BatchedGemm.cpp:8-70. Blocks are host pointers; they are scanned, packed and copied to the
- * device, and may be freed after the call (as setup_sparse_batch.cpp:281-343 copies them). */
+ * BatchedGemm.cpp:8-70. Blocks are host pointers (scanned on the host, staged, uploaded and
+ * packed on the device) or, all of them, device / managed memory pointers, as from a caller that
+ * keeps its block operators in HBM (scanned and packed on the device, read in place, nothing
+ * uploaded; dmk_info.device_blocks = 1). Mixing host and device blocks is an error. Either way
+ * the blocks may be freed after the call (as setup_sparse_batch.cpp:281-343 copies them). */
 int dmk_create_d(size_t noperator, size_t npatches, const size_t* left_patch_size,
                  const size_t* right_patch_size, const double* const* Amatrix,
                  const size_t* ld_Amatrix, const double* const* Bmatrix, const size_t* ld_Bmatrix,
@@ -110,7 +116,8 @@ int dmk_create_d(size_t noperator, size_t npatches, const size_t* left_patch_siz
  * BatchedGemm.h:21-28, instantiated at BatchedGemm.cpp:119). Blocks are std::complex<double>
  * column-major host arrays passed as interleaved (re, im) doubles, leading dimensions in complex
  * elements. The handle holds the real embedding of H on 2n reals (real parts of all patches,
- * then imaginary parts); blocks with zero imaginary parts cost nothing extra. */
+ * then imaginary parts); blocks with zero imaginary parts cost nothing extra. The embedding is
+ * built on the host: device-resident blocks are rejected. */
 int dmk_create_z(size_t noperator, size_t npatches, const size_t* left_patch_size,
                  const size_t* right_patch_size, const double* const* Amatrix,
                  const size_t* ld_Amatrix, const double* const* Bmatrix, const size_t* ld_Bmatrix,
@@ -445,7 +452,8 @@ int dmk_davidson_ground_d(dmk_handle* handle, const double* init, double* z,
  * Kron plugin computes the product, and L^dagger comes from a second plugin handle built from
  * the adjoint blocks (the reference runs that adjoint on its CPU Kron path). Each call is timed
  * on the host and accumulated (DeltaClock matrixVectorProduct, MatrixVectorKron.h:139-166).
- * x and y are device vectors of n elements (interleaved complex for _z); synchronous. */
+ * The blocks are host memory (the stored matrix and the adjoint blocks are assembled on the
+ * host). x and y are device vectors of n elements (interleaved complex for _z); synchronous. */
 typedef struct dmk_facade dmk_facade;
 
 typedef struct dmk_facade_info {
