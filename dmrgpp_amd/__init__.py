@@ -176,6 +176,7 @@ SIGNATURES = {
                                  ctypes.POINTER(KrylovParams), _VP, ctypes.POINTER(KrylovStats)]),
     "dmk_diagonal_d": (_I, [_VP, _VP]),
     "dmk_density_diag_d": (_I, [_I, _VP, _VP, _VP]),
+    "dmk_density_diag_dev_d": (_I, [_I, _VP, _VP, _VP, _VP]),
     "dmk_density_svd_d": (_I, [_I, _VP, _VP, _VP, _VP, _VP]),
     "dmk_mlp_create": (_I, [ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_VP)]),
     "dmk_mlp_destroy": (None, [_VP]),
@@ -292,6 +293,58 @@ def density_diag(blocks):
         out.append((eigs[off:off + k].copy(), a))
         off += k
     return out
+
+
+def density_diag_device(blocks):
+    """Eigenpairs of symmetric blocks that are already in device memory (dmk_density_diag_dev_d):
+    blocks is a list of (device address, n) of compact column-major n x n matrices; returns
+    [(w ascending, v with eigenvectors in columns)] on the host."""
+    nb = len(blocks)
+    vecs = [np.empty((n, n), order="F") for _, n in blocks]
+    sizes = (ctypes.c_int64 * max(nb, 1))(*[n for _, n in blocks])
+    ptrs = (ctypes.c_void_p * max(nb, 1))(*[p for p, _ in blocks])
+    vptr = (ctypes.c_void_p * max(nb, 1))(*[v.ctypes.data for v in vecs])
+    eigs = np.zeros(max(1, sum(n for _, n in blocks)))
+    _check(lib().dmk_density_diag_dev_d(nb, ctypes.addressof(sizes), ctypes.addressof(ptrs),
+                                        ctypes.addressof(vptr), eigs.ctypes.data),
+           "dmk_density_diag_dev_d")
+    out, off = [], 0
+    for v in vecs:
+        k = v.shape[0]
+        out.append((eigs[off:off + k].copy(), v))
+        off += k
+    return out
+
+
+def density_matrices_device(z_ptr: int, offsets, R, L, side: str, scale: float = 1.0):
+    """The reduced density-matrix blocks of a device-resident superblock vector, formed on the
+    device: patch I's segment X_I (R_I x L_I, column major at z_ptr + 8 offsets[I]) gives
+    rho_I = scale X_I^T X_I (L_I x L_I) for the left block and scale X_I X_I^T (R_I x R_I) for
+    the right block (DensityMatrixLocal.h:201-204 forms them on the host), all in one batched
+    GEMM (dmk_gemm_vbatch_d). Returns (DeviceArray holding the blocks, [(address, n)])."""
+    R = [int(r) for r in R]
+    L = [int(c) for c in L]
+    ns = L if side == "left" else R
+    at, offs = 0, []
+    for n in ns:
+        offs.append(at)
+        at += n * n
+    dev = DeviceArray(max(at, 1))
+    groups = []
+    for I, n in enumerate(ns):
+        x = z_ptr + 8 * int(offsets[I])
+        c = dev.ptr + 8 * offs[I]
+        if side == "left":   # C = X^T X: m = n = L_I, k = R_I
+            groups.append((n, n, R[I], R[I], R[I], n, [(x, x, c)]))
+        else:                # C = X X^T: m = n = R_I, k = L_I
+            groups.append((n, n, L[I], R[I], R[I], n, [(x, x, c)]))
+    try:
+        gemm_vbatch("T" if side == "left" else "N", "N" if side == "left" else "T", groups,
+                    alpha=scale)
+    except BaseException:
+        dev.free()
+        raise
+    return dev, [(dev.ptr + 8 * o, n) for o, n in zip(offs, ns)]
 
 
 def density_svd(blocks):

@@ -44,6 +44,8 @@ struct dmk_handle {
 namespace dmk {
 
 void density_diag(int nblocks, const int64_t* n, double* const* blocks, double* eigs);
+void density_diag_device(int nblocks, const int64_t* n, const double* const* dblocks,
+                         double* const* vecs, double* eigs);
 void density_svd(int nblocks, const int64_t* rows, const int64_t* cols,
                  const double* const* blocks, double* const* u, double* s);
 void launch_copy(double* y, const double* x, int64_t n, hipStream_t s);
@@ -644,6 +646,15 @@ int dmk_density_diag_d(int nblocks, const int64_t* n, double* const* blocks, dou
 {
 	return guarded([&] {
 		dmk::density_diag(nblocks, n, blocks, eigs);
+		return DMK_OK;
+	});
+}
+
+int dmk_density_diag_dev_d(int nblocks, const int64_t* n, const double* const* blocks,
+                           double* const* vecs, double* eigs)
+{
+	return guarded([&] {
+		dmk::density_diag_device(nblocks, n, blocks, vecs, eigs);
 		return DMK_OK;
 	});
 }

@@ -193,6 +193,53 @@ class GpuDensityEig:
             out[i] = wv
         return out
 
+    # The same from a device-resident superblock vector: the density-matrix blocks are formed
+    # on the device (rho_device, one batched GEMM), the large ones stay there for the eigensolver
+    # and only the small ones are downloaded for the host LAPACK (solve_device).
+    def rho_device(self, dev_psi, side: str):
+        from . import density_matrices_device
+
+        z_ptr, offsets, R, L, scale = dev_psi
+        return density_matrices_device(z_ptr, offsets, R, L, side, scale)
+
+    def solve_device(self, rho) -> list:
+        import threading
+
+        from . import _check, density_diag_device, lib
+
+        dev, blocks = rho
+        try:
+            big = [i for i, (_, n) in enumerate(blocks) if n >= self.min_size]
+            out = [None] * len(blocks)
+            res, err = [], []
+
+            def device():
+                try:
+                    res.extend(density_diag_device([blocks[i] for i in big]))
+                except BaseException as e:  # re-raised in the caller's thread
+                    err.append(e)
+
+            th = threading.Thread(target=device) if big else None
+            if th is not None:
+                th.start()
+            try:
+                for i, (p, n) in enumerate(blocks):
+                    if n < self.min_size:
+                        r = np.empty((n, n), order="F")
+                        _check(lib().dmk_memcpy_d2h(r.ctypes.data, p, r.nbytes),
+                               "dmk_memcpy_d2h")
+                        out[i] = np.linalg.eigh(0.5 * (r + r.T))
+            finally:
+                if th is not None:
+                    th.join()
+            if err:
+                raise err[0]
+            for i, wv in zip(big, res):
+                out[i] = wv
+            return out
+        finally:
+            dev.free()
+
 
 class SvdDensityEig:
     """The SVD form of the truncation's eigenpairs (DensityMatrixSvd::diag,
@@ -324,7 +371,8 @@ class GpuBasisChange:
 
 
 def truncate(block: Block, psi_blocks: list, patches: list, side: str, m: int,
-             model: SiteModel, eig=None, times: dict | None = None, rotate=None):
+             model: SiteModel, eig=None, times: dict | None = None, rotate=None,
+             dev_psi=None):
     """Keeps the m largest eigenvalues of the block's reduced density matrix.
 
     psi_blocks[I] is the R_I x L_I ground-state segment of patch I. For the left block
@@ -333,26 +381,31 @@ def truncate(block: Block, psi_blocks: list, patches: list, side: str, m: int,
     large ones on the GPU; an eig with from_wavefunction set, SvdDensityEig, gets the
     wave-function blocks instead); rotate, when given, does the change of basis of H and the kept operators
     (GpuBasisChange runs it on the GPU; by default the host does it). times, when given, accumulates the milliseconds of the density
-    matrices (rho_ms), their eigenpairs (eig_ms) and the change of basis (basis_ms). Returns
-    (new block, discarded weight)."""
+    matrices (rho_ms), their eigenpairs (eig_ms) and the change of basis (basis_ms). dev_psi,
+    when given with an eig that has rho_device / solve_device (GpuDensityEig), is the device
+    copy of the ground state, (address, patch offsets, R, L, 1 / |z|^2): the density matrices
+    are then formed on the device and the large ones never leave it. Returns (new block,
+    discarded weight)."""
     t0 = time.perf_counter()
     cand = []
     vecs = {}
-    groups, rhos = [], []
-    for I, (gL, gR) in enumerate(patches):
-        X = psi_blocks[I]
-        if getattr(eig, "from_wavefunction", False):
-            g, rho = (gL, X.T) if side == "left" else (gR, X)
-        else:
-            if side == "left":
-                g, rho = gL, X.T @ X
+    groups = [gL if side == "left" else gR for gL, gR in patches]
+    if dev_psi is not None and hasattr(eig, "rho_device"):
+        rho = eig.rho_device(dev_psi, side)
+        t1 = time.perf_counter()
+        pairs = eig.solve_device(rho)
+    else:
+        rhos = []
+        for I in range(len(patches)):
+            X = psi_blocks[I]
+            if getattr(eig, "from_wavefunction", False):
+                rho = X.T if side == "left" else X
             else:
-                g, rho = gR, X @ X.T
-            rho = 0.5 * (rho + rho.T)
-        groups.append(g)
-        rhos.append(rho)
-    t1 = time.perf_counter()
-    pairs = (eig or host_eig)(rhos)
+                rho = X.T @ X if side == "left" else X @ X.T
+                rho = 0.5 * (rho + rho.T)
+            rhos.append(rho)
+        t1 = time.perf_counter()
+        pairs = (eig or host_eig)(rhos)
     t2 = time.perf_counter()
     for g, (w, v) in zip(groups, pairs):
         vecs[g] = v
@@ -515,17 +568,7 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
         rep = dict(rep)
         if init is not None:
             wft.free(init)
-        if wft:
-            old = prev.pop("z_dev", None)
-            if old is not None:
-                wft.free(old)
-            prev.update(phase=phase, l=l, Lb=Lb, Rb=Rb, prob=prob, patches=patches,
-                        z_dev=rep.pop("z_dev", None))
-            rep["wft_init"] = init is not None
-        elif "z_dev" in rep:
-            zd = rep.pop("z_dev")
-            if hasattr(zd, "free"):
-                zd.free()
+        zd = rep.pop("z_dev", None)
         timing = {"grow_ms": 1e3 * (tk0 - tg), "kron_ms": 1e3 * (tw - tk0),
                   "wft_ms": 1e3 * (t0 - tw)}
         off = prob.offsets
@@ -533,16 +576,27 @@ def run(model: SiteModel, lattice: Lattice, target_of, m_infinite: int, sweeps: 
                for I in range(prob.npatches)]
         nrm = np.linalg.norm(z)
         psi = [p / nrm for p in psi]
+        # The device copy of the ground state, for density matrices formed on the device.
+        dev_psi = (zd.ptr, off, prob.R, prob.L, 1.0 / nrm ** 2) \
+            if zd is not None and getattr(zd, "ptr", None) else None
         disc = 0.0
         tparts = {}
         if "left" in trunc:
             left[lsize + 1], d = truncate(Lb, psi, patches, "left", m, model, eig, tparts,
-                                          rotate)
+                                          rotate, dev_psi)
             disc = max(disc, d)
         if "right" in trunc:
             right[rsize + 1], d = truncate(Rb, psi, patches, "right", m, model, eig, tparts,
-                                           rotate)
+                                           rotate, dev_psi)
             disc = max(disc, d)
+        if wft:
+            old = prev.pop("z_dev", None)
+            if old is not None:
+                wft.free(old)
+            prev.update(phase=phase, l=l, Lb=Lb, Rb=Rb, prob=prob, patches=patches, z_dev=zd)
+            rep["wft_init"] = init is not None
+        elif zd is not None and hasattr(zd, "free"):
+            zd.free()
         energy = e
         timing["truncate_ms"] = 1e3 * (time.perf_counter() - t1)
         timing.update({"truncate_" + k: v for k, v in tparts.items()})

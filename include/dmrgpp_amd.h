@@ -328,11 +328,22 @@ int dmk_krylov_evolve_z(dmk_handle* handle, int nvec, const double* phi, size_t 
  * major, in host memory at blocks[b] with leading dimension n[b] (0 <= n[b] <= 32768); only its
  * upper triangle is read. It is overwritten by its orthonormal eigenvectors (column j belongs to
  * eigenvalue j), and its eigenvalues, ascending, go to eigs[o_b .. o_b + n[b]) with
- * o_b = n[0] + ... + n[b-1]. rocSOLVER dsyevd on the device, up to four blocks at once on
+ * o_b = n[0] + ... + n[b-1]. rocSOLVER dsyevd on the device: blocks of similar size (down to
+ * half the largest of their class, at most 16) as one strided batch, each padded to the class's
+ * size with a diagonal below its spectrum; classes and single blocks up to four at once on
  * their own streams and host threads (largest first); synchronous. The reference's
  * EnforcePhase (EnforcePhase.h:23-39) multiplies each eigenvector by the sign of a norm, which is
  * +1 for real entries, so no sign convention is imposed here either. */
 int dmk_density_diag_d(int nblocks, const int64_t* n, double* const* blocks, double* eigs);
+
+/* The same on blocks already in device memory, as the density matrices of a truncation that
+ * forms them on the device from the device-resident ground state (DensityMatrixLocal.h:201-204
+ * builds them on the host): block b is n[b] x n[b], compact column major at blocks[b] (device;
+ * read, not modified; only its upper triangle is used), its eigenvectors go to the host array
+ * vecs[b] (n[b] x n[b], column major) and its eigenvalues, ascending, to eigs as above.
+ * Synchronous. */
+int dmk_density_diag_dev_d(int nblocks, const int64_t* n, const double* const* blocks,
+                           double* const* vecs, double* eigs);
 
 /* The SVD form of the same step: replaces DensityMatrixSvd::diag (DensityMatrixSvd.h:517-554),
  * whose ParallelSvd::doTask (DensityMatrixSvd.h:382-410) calls PsimagLite::Svd with jobz 'A' on

@@ -56,6 +56,71 @@ def test_gpu_density_diag_matches_lapack():
 
 
 @pytest.mark.gpu
+def test_gpu_density_diag_padded_batch_classes():
+    # Blocks of similar size are solved as one padded strided batch (sizes down to half the
+    # class's largest): density blocks, a rank-deficient one, an exactly degenerate one and a
+    # general symmetric block with negative eigenvalues (the padding must stay below every
+    # spectrum), next to a second class and a single block. Every block against LAPACK.
+    rng = np.random.default_rng(11)
+    q, _ = np.linalg.qr(rng.standard_normal((256, 256)))
+    degenerate = q @ np.diag(np.repeat([0.5, 0.25, 0.125, 0.0625], 64)) @ q.T
+    general = rng.uniform(-1, 1, (300, 300))
+    general = 3.0 * (general + general.T)
+    blocks = [_density_like(400, 400, 1), _density_like(390, 60, 2), degenerate, general,
+              _density_like(201, 201, 3), _density_like(200, 200, 4),
+              _density_like(120, 30, 5), _density_like(90, 90, 6), _density_like(17, 17, 7)]
+    out = dm.density_diag(blocks)
+    for a, (w, v) in zip(blocks, out):
+        n = a.shape[0]
+        w0, v0 = np.linalg.eigh(a)
+        scale = max(1.0, np.max(np.abs(w0)))
+        assert np.max(np.abs(w - w0)) < 1e-13 * scale, n
+        assert np.linalg.norm(a @ v - v * w) < 1e-12 * scale * np.sqrt(n), n
+        assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-11 * np.sqrt(n), n
+    # The degenerate block's eigenspaces.
+    (w, v), a = out[2], blocks[2]
+    w0, v0 = np.linalg.eigh(a)
+    for lam in (0.5, 0.25, 0.125, 0.0625):
+        s = np.abs(w0 - lam) < 1e-10
+        assert s.sum() == 64
+        assert np.linalg.norm(v[:, s] @ v[:, s].T - v0[:, s] @ v0[:, s].T) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_density_matrices_formed_on_the_device():
+    # rho_I = s X_I^T X_I (left) and s X_I X_I^T (right) from a device vector holding the
+    # patches' R_I x L_I segments, against numpy; then their eigenpairs from device memory
+    # (dmk_density_diag_dev_d) against the host-memory path and LAPACK.
+    rng = np.random.default_rng(8)
+    R, L = [3, 70, 1, 300, 129], [40, 1, 90, 260, 128]
+    off = np.concatenate([[0], np.cumsum(np.array(R) * np.array(L))])
+    z = rng.standard_normal(off[-1])
+    dz = dm.DeviceArray.from_host(z)
+    scale = 1.0 / np.dot(z, z)
+    for side in ("left", "right"):
+        dev, blocks = dm.density_matrices_device(dz.ptr, off, R, L, side, scale)
+        want = []
+        for I in range(len(R)):
+            X = z[off[I]:off[I + 1]].reshape((R[I], L[I]), order="F")
+            want.append(scale * (X.T @ X if side == "left" else X @ X.T))
+        for (p, n), w in zip(blocks, want):
+            got = np.empty((n, n), order="F")
+            dm._check(dm.lib().dmk_memcpy_d2h(got.ctypes.data, p, got.nbytes), "d2h")
+            assert np.max(np.abs(got - w)) < 1e-14 * max(1.0, np.max(np.abs(w)))
+        pairs = dm.density_diag_device(blocks)
+        host = dm.density_diag([0.5 * (w + w.T) for w in want])
+        for w, (ev, v), (eh, _) in zip(want, pairs, host):
+            a = 0.5 * (w + w.T)
+            n = a.shape[0]
+            assert np.max(np.abs(ev - eh)) < 1e-14
+            assert np.max(np.abs(ev - np.linalg.eigvalsh(a))) < 1e-14
+            assert np.linalg.norm(a @ v - v * ev) < 1e-13 * np.sqrt(n)
+            assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-11 * np.sqrt(n)
+        dev.free()
+    dz.free()
+
+
+@pytest.mark.gpu
 def test_gpu_density_diag_degenerate_spectrum():
     # Exactly degenerate eigenvalues (a Kronecker structure gives each one several times): the
     # eigenspace projectors agree with LAPACK's even though the vectors inside them may not.
