@@ -61,7 +61,10 @@ def log(*a):
 
 
 def shape_key(t):
-    """The stage-2 launch shape of a tuning (the key of TRAFFIC)."""
+    """The stage-2 launch shape of a tuning (the key of TRAFFIC); a balanced persistent stage 2
+    has no measured entry."""
+    if t.get("persist", 0):
+        return None
     return (t["stage2_tile_m"], t["stage2_tile_n"], t["splitk"], t["xcd_order"])
 
 

@@ -58,7 +58,7 @@ class Tuning(ctypes.Structure):
                 ("stage2_tile_m", ctypes.c_int32), ("stage2_tile_n", ctypes.c_int32),
                 ("splitk", ctypes.c_int32), ("xcd_order", ctypes.c_int32),
                 ("mfma32", ctypes.c_int32), ("prefetch", ctypes.c_int32),
-                ("kchunk", ctypes.c_int32)]
+                ("kchunk", ctypes.c_int32), ("persist", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -401,6 +401,7 @@ void to_c(const dmk::Tuning& t, dmk_tuning* c)
 	c->mfma32        = t.mfma32;
 	c->prefetch      = t.depth;
 	c->kchunk        = t.kchunk;
+	c->persist       = t.persist;
 }
 } // namespace
 
@@ -429,6 +430,7 @@ int dmk_set_tuning(dmk_handle* handle, const dmk_tuning* tuning)
 		t.mfma32 = tuning->mfma32;
 		t.depth  = tuning->prefetch == 0 ? 1 : tuning->prefetch;
 		t.kchunk = tuning->kchunk == 0 ? 1 : tuning->kchunk;
+		t.persist = tuning->persist;
 		plan_of(handle).retune(t);
 		return DMK_OK;
 	});

@@ -36,6 +36,39 @@ struct Tile {
 	int32_t pad;
 };
 
+// Balanced persistent launch (a grid of as many workgroups as the chip holds at once, each
+// given an equal share of the launch's K chunks): workgroup w runs items [bins[w], bins[w+1]).
+// An item is chunks [c0, c0 + nch) of tile (task, m0, n0); its result goes to the task's output
+// (slot < 0: the item is the whole tile) or to partial slot `slot` (a BM x BN tile of the compute
+// type, element (r, c) at r * BN + c), which the fix-up kernel sums for the tile.
+struct SkItem {
+	int32_t task;
+	int32_t m0;
+	int32_t n0;
+	int32_t c0;
+	int32_t nch;
+	int32_t slot;
+};
+// A tile cut between workgroups: its partials are slots [first, first + count), summed in that
+// order (the workgroup order), so the result does not depend on timing.
+struct SkFix {
+	int32_t task;
+	int32_t m0;
+	int32_t n0;
+	int32_t first;
+	int32_t count;
+	int32_t pad;
+};
+
+// Extra arguments of a panel-kernel launch: the item list and the workgroups' item ranges of a
+// balanced persistent launch (nullptr: one tile per workgroup); occupancy, when non-null,
+// receives the kernel's resident workgroups per CU instead of a launch.
+struct PanelLaunchExtra {
+	const SkItem*  items     = nullptr;
+	const int32_t* bins      = nullptr;
+	int*           occupancy = nullptr;
+};
+
 // Descriptors of the matvec kernels on K-major panels (panel_kernels.hip). Operand bases:
 enum PBaseId : int32_t
 {

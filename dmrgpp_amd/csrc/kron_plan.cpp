@@ -40,12 +40,18 @@ template <typename T, typename TO>
 void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const PTask* tasks,
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
                        const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
-                       int64_t wsstride, int nvec, int beta, hipStream_t stream);
+                       int64_t wsstride, int nvec, int beta, hipStream_t stream,
+                       const PanelLaunchExtra& ex = PanelLaunchExtra {});
 void launch_panel_bf16x3(int bm, int bn, int variant, int depth, const PTask* tasks, const PSeg* segs,
                          const Tile* tiles, int ntiles, const void* const* bases_hi,
                          const void* const* bases_lo, const PStrides& vs, double* out_y,
                          int64_t ystride, float* out_f, void* out_h, void* out_l, int64_t wsstride,
-                         bool pair, int nvec, int beta, hipStream_t stream);
+                         bool pair, int nvec, int beta, hipStream_t stream,
+                         const PanelLaunchExtra& ex = PanelLaunchExtra {});
+template <typename T>
+void launch_sk_fixup(const SkFix* fixes, int nfix, const PTask* tasks, const T* part,
+                     int64_t wsstride, double* out_y, int64_t ystride, int bm, int bn, int nvec,
+                     int beta, hipStream_t stream);
 template <typename E>
 void launch_prep_x(const PrepTile* tiles, int ntiles, const PrepPatch* patches, const double* x,
                    int64_t ldx, E* xp, E* xt, const PStrides& vs, int nvec, hipStream_t stream);
@@ -849,6 +855,9 @@ Plan::~Plan()
 	pool_device_free(d_prep_patches_);
 	pool_device_free(d_reduce_);
 	pool_device_free(d_chunks_);
+	pool_device_free(d_sk_items_);
+	pool_device_free(d_sk_bins_);
+	pool_device_free(d_sk_fixes_);
 	pool_device_free(d_dg_);
 	st1_.release();
 	st2_.release();
@@ -950,6 +959,8 @@ void Plan::retune(const Tuning& t)
 		throw std::invalid_argument("dmk tuning: prefetch must be 1 or 2");
 	if (t.kchunk != 1 && t.kchunk != 2)
 		throw std::invalid_argument("dmk tuning: kchunk must be 1 or 2");
+	if (t.persist < 0 || t.persist > 8)
+		throw std::invalid_argument("dmk tuning: persist must be 0 .. 8");
 	if (t.kchunk == 2
 	    && (precision_ == PREC_BF16X3 || t.bm1 + t.bn1 > 128 || t.bm2 + t.bn2 > 128
 	        || t.bm1 == 96 || t.bn1 == 96 || t.bm2 == 96 || t.bn2 == 96))
@@ -988,8 +999,8 @@ void Plan::retune(const Tuning& t)
 		int64_t ktot = 0;
 		for (int s = tk.seg_begin; s < tk.seg_end; ++s)
 			ktot = (ktot + 15) / 16 * 16 + t2_segs_[size_t(s)].k;
-		int nsplit = 1;
-		if (t.splitk > 0 && ktot >= 2 * int64_t(t.splitk))
+		int nsplit = 1; // (the balanced launch cuts tiles itself)
+		if (!t.persist && t.splitk > 0 && ktot >= 2 * int64_t(t.splitk))
 			nsplit = int(std::min<int64_t>(16, ktot / t.splitk));
 		if (nsplit <= 1) {
 			PTask c     = tk;
@@ -1047,6 +1058,7 @@ void Plan::retune(const Tuning& t)
 	}
 	stats_.ws_elems = ws;
 	build_tiles(st2_, cost, t.xcd);
+	build_balanced(t);
 	chunks_.clear();
 	for (size_t r = 0; r < reduce_.size(); ++r)
 		for (int64_t s0 = 0; s0 < reduce_[r].count; s0 += kReduceChunk)
@@ -1059,11 +1071,118 @@ void Plan::retune(const Tuning& t)
 	d_chunks_ = dev_alloc_copy(chunks_);
 	st1_.upload();
 	st2_.upload();
+	pool_device_release(d_sk_items_);
+	pool_device_release(d_sk_bins_);
+	pool_device_release(d_sk_fixes_);
+	d_sk_items_ = dev_alloc_copy(sk_items_);
+	d_sk_bins_  = dev_alloc_copy(sk_bins_);
+	d_sk_fixes_ = dev_alloc_copy(sk_fixes_);
 	stats_.stage1_tiles = int64_t(st1_.tiles.size());
 	stats_.stage2_tiles = int64_t(st2_.tiles.size());
 	ws_bytes_           = 0; // force the workspace re-check
 	pool_device_release(d_ws_);
 	ensure_scratch(1);
+}
+
+int Plan::stage2_occupancy(const Tuning& t) const
+{
+	int occ = 0;
+	PanelLaunchExtra q;
+	q.occupancy = &occ;
+	const PStrides vs {};
+	if (precision_ == PREC_FP64)
+		launch_panel_gemm<double, double>(t.bm2, t.bn2, 0, t.depth, t.kchunk, nullptr, nullptr,
+		                                  nullptr, 0, nullptr, vs, nullptr, 0, nullptr, 0, 0, 0,
+		                                  nullptr, q);
+	else if (precision_ == PREC_FP32)
+		launch_panel_gemm<float, double>(t.bm2, t.bn2, t.mfma32, t.depth, t.kchunk, nullptr,
+		                                 nullptr, nullptr, 0, nullptr, vs, nullptr, 0, nullptr, 0,
+		                                 0, 0, nullptr, q);
+	else
+		launch_panel_bf16x3(t.bm2, t.bn2, t.mfma32, t.depth, nullptr, nullptr, nullptr, 0, nullptr,
+		                    nullptr, vs, nullptr, 0, nullptr, nullptr, nullptr, 0, false, 0, 0,
+		                    nullptr, q);
+	return std::max(occ, 1);
+}
+
+// Balanced persistent stage 2: as many workgroups as the chip holds at once (the kernel's
+// occupancy times the CU count), kXcds-strided as the XCD-aware order places them, and on each
+// XCD the K chunks of that XCD's tiles (in list order, so neighbouring tiles that share operand
+// panels stay together) cut into equal contiguous shares. A tile whose chunks fall into two or
+// more shares leaves one partial per share; the fix-up sums them in share order.
+void Plan::build_balanced(const Tuning& t)
+{
+	sk_items_.clear();
+	sk_bins_.clear();
+	sk_fixes_.clear();
+	if (!t.persist)
+		return;
+	const int bk = (precision_ == PREC_FP64 ? 16 : 32) * t.kchunk;
+	int       dev = 0, ncu = 0;
+	DMK_HIP_CHECK(hipGetDevice(&dev));
+	DMK_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	// t.persist workgroups per CU, at most as many as the kernel's occupancy lets stay resident.
+	const int per_cu  = std::min(t.persist, stage2_occupancy(t));
+	const int per_xcd = std::max(1, ncu * per_cu / kXcds);
+	std::vector<int64_t> tchunks(st2_.tasks.size(), 0);
+	for (size_t ti = 0; ti < st2_.tasks.size(); ++ti)
+		for (int s = st2_.tasks[ti].seg_begin; s < st2_.tasks[ti].seg_end; ++s)
+			if (st2_.segs[size_t(s)].k > 0)
+				tchunks[ti] += (st2_.segs[size_t(s)].k + bk - 1) / bk;
+	std::vector<std::vector<SkItem>> bins(size_t(per_xcd) * kXcds);
+	int32_t                          nslots = 0;
+	for (int x = 0; x < kXcds; ++x) {
+		std::vector<Tile> list;
+		for (size_t i = size_t(x); i < st2_.tiles.size(); i += kXcds)
+			if (st2_.tiles[i].task >= 0)
+				list.push_back(st2_.tiles[i]);
+		int64_t total = 0;
+		for (const Tile& tl : list)
+			total += tchunks[size_t(tl.task)];
+		// Share j of this XCD covers chunks [end(j - 1), end(j)).
+		auto end_of = [&](int j) { return total * int64_t(j + 1) / per_xcd; };
+		int     j   = 0;
+		int64_t pos = 0;
+		for (const Tile& tl : list) {
+			const int64_t c = tchunks[size_t(tl.task)];
+			while (j + 1 < per_xcd && end_of(j) <= pos)
+				++j;
+			if (c == 0) { // a task without products: one item writes its zeros
+				bins[size_t(j) * kXcds + size_t(x)].push_back(
+				    SkItem { tl.task, tl.m0, tl.n0, 0, 0, -1 });
+				continue;
+			}
+			// The shares this tile's chunks [pos, pos + c) fall into.
+			std::vector<std::pair<int, int64_t>> cuts; // (share, first chunk of the piece)
+			for (int64_t a = pos; a < pos + c;) {
+				while (j + 1 < per_xcd && end_of(j) <= a)
+					++j;
+				const int64_t b = std::min(pos + c, j + 1 < per_xcd ? end_of(j) : pos + c);
+				cuts.emplace_back(j, a);
+				a = b;
+			}
+			const int32_t first = cuts.size() > 1 ? nslots : -1;
+			for (size_t q = 0; q < cuts.size(); ++q) {
+				const int64_t a = cuts[q].second;
+				const int64_t b = q + 1 < cuts.size() ? cuts[q + 1].second : pos + c;
+				bins[size_t(cuts[q].first) * kXcds + size_t(x)].push_back(
+				    SkItem { tl.task, tl.m0, tl.n0, int32_t(a - pos), int32_t(b - a),
+				             first < 0 ? -1 : first + int32_t(q) });
+			}
+			if (first >= 0) {
+				sk_fixes_.push_back(
+				    SkFix { tl.task, tl.m0, tl.n0, first, int32_t(cuts.size()), 0 });
+				nslots += int32_t(cuts.size());
+			}
+			pos += c;
+		}
+	}
+	sk_bins_.push_back(0);
+	for (const auto& b : bins) {
+		sk_items_.insert(sk_items_.end(), b.begin(), b.end());
+		sk_bins_.push_back(int32_t(sk_items_.size()));
+	}
+	stats_.ws_elems = int64_t(nslots) * t.bm2 * t.bn2;
 }
 
 void Plan::ensure_scratch(int nvec)
@@ -1171,6 +1290,15 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 	const int      v1  = tuning_.mfma32;
 	const int      dp  = tuning_.depth;
 	const int      kc  = tuning_.kchunk;
+	// Stage 2: one tile per workgroup, or the balanced persistent launch and its fix-up.
+	PanelLaunchExtra sk;
+	int              n2 = int(st2_.tiles.size());
+	if (tuning_.persist) {
+		sk.items = d_sk_items_;
+		sk.bins  = d_sk_bins_;
+		n2       = int(sk_bins_.size()) - 1;
+	}
+	const int nfix = int(sk_fixes_.size());
 	if (precision_ == PREC_FP64) {
 		double* bx = static_cast<double*>(d_bx);
 		double* ws = static_cast<double*>(d_ws);
@@ -1185,10 +1313,12 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
 		launch_panel_gemm<double, double>(st2_.bm, st2_.bn, 0, dp, kc, st2_.d_tasks, st2_.d_segs,
-		                                  st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
-		                                  ldout, ws, wss, nvec, beta, stream_);
+		                                  st2_.d_tiles, n2, bases, vs, vout, ldout, ws, wss, nvec,
+		                                  beta, stream_, sk);
 		launch_splitk_reduce<double, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                     vout, ldout, nvec, beta, stream_);
+		launch_sk_fixup<double>(d_sk_fixes_, nfix, st2_.d_tasks, ws, wss, vout, ldout, st2_.bm,
+		                        st2_.bn, nvec, beta, stream_);
 	} else if (precision_ == PREC_BF16X3) {
 		// (hi, lo) bf16 arrays: hi parts of all vectors, then the lo parts.
 		uint16_t*     xph = static_cast<uint16_t*>(d_xin);
@@ -1209,11 +1339,13 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
 		float* ws = static_cast<float*>(d_ws);
-		launch_panel_bf16x3(st2_.bm, st2_.bn, v1, dp, st2_.d_tasks, st2_.d_segs, st2_.d_tiles,
-		                    int(st2_.tiles.size()), hi, lo, vs, vout, ldout, ws, nullptr, nullptr,
-		                    wss, false, nvec, beta, stream_);
+		launch_panel_bf16x3(st2_.bm, st2_.bn, v1, dp, st2_.d_tasks, st2_.d_segs, st2_.d_tiles, n2,
+		                    hi, lo, vs, vout, ldout, ws, nullptr, nullptr, wss, false, nvec, beta,
+		                    stream_, sk);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                    vout, ldout, nvec, beta, stream_);
+		launch_sk_fixup<float>(d_sk_fixes_, nfix, st2_.d_tasks, ws, wss, vout, ldout, st2_.bm,
+		                       st2_.bn, nvec, beta, stream_);
 	} else {
 		float* bx = static_cast<float*>(d_bx);
 		float* ws = static_cast<float*>(d_ws);
@@ -1228,10 +1360,12 @@ void Plan::launch(const double* vin, int64_t ldin, double* vout, int64_t ldout, 
 		if (mid)
 			DMK_HIP_CHECK(hipEventRecord(mid, stream_));
 		launch_panel_gemm<float, double>(st2_.bm, st2_.bn, v1, dp, kc, st2_.d_tasks, st2_.d_segs,
-		                                 st2_.d_tiles, int(st2_.tiles.size()), bases, vs, vout,
-		                                 ldout, ws, wss, nvec, beta, stream_);
+		                                 st2_.d_tiles, n2, bases, vs, vout, ldout, ws, wss, nvec,
+		                                 beta, stream_, sk);
 		launch_splitk_reduce<float, double>(d_reduce_, d_chunks_, int(chunks_.size()), ws, wss,
 		                                    vout, ldout, nvec, beta, stream_);
+		launch_sk_fixup<float>(d_sk_fixes_, nfix, st2_.d_tasks, ws, wss, vout, ldout, st2_.bm,
+		                       st2_.bn, nvec, beta, stream_);
 	}
 	DMK_HIP_CHECK(hipGetLastError());
 }
@@ -1400,6 +1534,23 @@ Tuning Plan::autotune(const double* vin, double* vout, int iters)
 		for (int x : { 0, 1 }) {
 			Tuning t = best;
 			t.xcd    = x;
+			c.push_back(t);
+		}
+		try_set(c);
+		c.clear();
+		// The balanced persistent stage 2, also with the larger tiles (which lose most to the
+		// tail of a launch of one tile per workgroup).
+		for (int p : { 0, 8, 4, 2 }) { // 8: as many as fit (the kernel's occupancy)
+			Tuning t  = best;
+			t.persist = p;
+			c.push_back(t);
+		}
+		for (const auto& sh : { std::make_pair(128, 128), std::make_pair(128, 64),
+		                        std::make_pair(64, 128), std::make_pair(64, 64) }) {
+			Tuning t  = best;
+			t.persist = 8;
+			t.bm2     = sh.first;
+			t.bn2     = sh.second;
 			c.push_back(t);
 		}
 		try_set(c);

@@ -61,11 +61,13 @@ struct Tuning {
 	int mfma32 = 0; // 1: 32x32 MFMA blocks (FP32 32x32x2, BF16x3 32x32x16); FP64 ignores it
 	int depth  = 1; // global loads issued this many K chunks ahead (1 or 2)
 	int kchunk = 1; // K chunk depth in units of the mode's base chunk (1 or 2)
+	int persist = 0; // k > 0: stage 2 as a balanced persistent launch, k workgroups per CU at most
 
 	bool operator==(const Tuning& o) const
 	{
 		return bm1 == o.bm1 && bn1 == o.bn1 && bm2 == o.bm2 && bn2 == o.bn2 && splitk == o.splitk
-		       && xcd == o.xcd && mfma32 == o.mfma32 && depth == o.depth && kchunk == o.kchunk;
+		       && xcd == o.xcd && mfma32 == o.mfma32 && depth == o.depth && kchunk == o.kchunk
+		       && persist == o.persist;
 	}
 };
 
@@ -186,6 +188,18 @@ private:
 	std::vector<ReduceChunk> chunks_;
 	Reduce*                  d_reduce_ = nullptr;
 	ReduceChunk*             d_chunks_ = nullptr;
+	// Balanced persistent stage 2 (Tuning::persist): items, workgroup item ranges, cut tiles.
+	std::vector<SkItem>  sk_items_;
+	std::vector<int32_t> sk_bins_;
+	std::vector<SkFix>   sk_fixes_;
+	SkItem*              d_sk_items_ = nullptr;
+	int32_t*             d_sk_bins_  = nullptr;
+	SkFix*               d_sk_fixes_ = nullptr;
+	// Resident stage-2 workgroups per CU of a tuning in the current precision (occupancy query
+	// of the kernel the tuning selects).
+	int stage2_occupancy(const Tuning& t) const;
+	// Builds the balanced item lists of stage 2 from st2_'s XCD-ordered tiles.
+	void build_balanced(const Tuning& t);
 
 	std::vector<int64_t> off_;
 	double*              d_apack_   = nullptr;

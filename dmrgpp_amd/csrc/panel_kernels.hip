@@ -37,6 +37,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <type_traits>
 
@@ -252,9 +253,15 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 // FP64 / FP32: T operands, TO output for tasks writing the caller's vector (the FP32 mode writes
 // its stage-2 output in FP64 for the FP64 Lanczos recurrence). Workspace tasks (stage-1 panels,
 // split-k partials) store T and always overwrite.
+//
+// With bins != nullptr the launch is persistent and balanced: workgroup b runs the items
+// [bins[b], bins[b + 1]) one after another (SkItem: a chunk range of one tile, its result to the
+// task's output or to a partial slot of out_ws for the fix-up); else workgroup b runs tile b
+// whole.
 template <typename TO, int BM, int BN, int BK, int PF, typename Op>
 __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK == 32 ? 5 : 1)) void panel_gemm_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
+    const SkItem* __restrict__ items, const int32_t* __restrict__ bins,
     PBasesT<typename Op::T> bases, PStrides vs, TO* __restrict__ out_y, int64_t ystride,
     typename Op::T* __restrict__ out_ws, int64_t wsstride, int beta)
 {
@@ -273,12 +280,8 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 	__shared__ vec_t As[2][BM * VPR];
 	__shared__ vec_t Bs[2][BN * VPR];
 
-	const Tile tl = tiles[blockIdx.x];
-	if (tl.task < 0)
-		return; // padding slot of the XCD-aware order
-	const PTask tk  = tasks[tl.task];
-	const int   tid = threadIdx.x;
-	const int   vec = blockIdx.y;
+	const int tid = threadIdx.x;
+	const int vec = blockIdx.y;
 
 	const T* b0   = bases.p[PBASE_APACK];
 	const T* b1   = bases.p[PBASE_BPACK];
@@ -298,156 +301,204 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 	const int g    = lane / Op::kM; // lane group: k slice g * NF .. g * NF + NF - 1
 	const int lr   = lane % Op::kM;
 
-	acc_t acc[FI][FJ];
-#pragma unroll
-	for (int i = 0; i < FI; ++i)
-#pragma unroll
-		for (int j = 0; j < FJ; ++j)
-			acc[i][j] = acc_t {};
-
-	int seg = tk.seg_begin;
-	while (seg < tk.seg_end && segs[seg].k <= 0)
-		++seg;
-
-	if (seg < tk.seg_end) {
-		// Load cursor: the next chunk whose global loads are issued (segment seg, offset ck0).
-		// Loads are issued unconditionally: past the tile's last chunk (cvalid false) the last
-		// chunk is loaded again and never stored.
-		LA   la;
-		LB   lb;
-		PSeg s      = segs[seg];
-		int  ck0    = 0;
-		int  lastk  = 0;
-		bool cvalid = true;
-		la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
-		lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
-		auto issue = [&](vec_t (&xa)[LA::NV], vec_t (&xb)[LB::NV], int& lim) {
-			const int kk = cvalid ? ck0 : lastk;
-			la.load(xa, kk, s.k, tid);
-			lb.load(xb, kk, s.k, tid);
-			lim   = s.k - kk;
-			lastk = kk;
-			if (cvalid) {
-				bool newseg;
-				cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
-				if (cvalid && newseg) {
-					s = segs[seg];
-					la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
-					lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
-				}
-			}
-		};
-		auto compute = [&](int b) {
-			// Fragments: NF consecutive k of one LDS row per operand block.
-			T fa[FI][NF], fb[FJ][NF];
-#pragma unroll
-			for (int i = 0; i < FI; ++i) {
-				const int r = wm + Op::kM * i + lr;
-#pragma unroll
-				for (int v = 0; v < NF / VW; ++v) {
-					const vec_t x = As[b][r * VPR + LA::slot(r, g * (NF / VW) + v)];
-#pragma unroll
-					for (int e = 0; e < VW; ++e)
-						fa[i][v * VW + e] = x[e];
-				}
-			}
-#pragma unroll
-			for (int j = 0; j < FJ; ++j) {
-				const int r = wn + Op::kM * j + lr;
-#pragma unroll
-				for (int v = 0; v < NF / VW; ++v) {
-					const vec_t x = Bs[b][r * VPR + LB::slot(r, g * (NF / VW) + v)];
-#pragma unroll
-					for (int e = 0; e < VW; ++e)
-						fb[j][v * VW + e] = x[e];
-				}
-			}
-#pragma unroll
-			for (int st = 0; st < NF; ++st)
-#pragma unroll
-				for (int i = 0; i < FI; ++i)
-#pragma unroll
-					for (int j = 0; j < FJ; ++j)
-						acc[i][j] = Op::mma(fa[i][st], fb[j][st], acc[i][j]);
-		};
-		vec_t a0[LA::NV], b0[LB::NV];
-		int   lim0, buf = 0;
-		// The loops are counted and leave only at the top, so that every path into the loop
-		// header has the same loads in flight and the compiler waits for one register set only.
-		const int nch = chunk_count<BK>(segs, seg, tk.seg_end);
-		if constexpr (PF == 1) {
-			// One chunk ahead: the loads of chunk c+1 fly during the MFMAs of chunk c.
-			issue(a0, b0, lim0);
-			la.store(As[0], a0, tid, lim0);
-			lb.store(Bs[0], b0, tid, lim0);
-			__syncthreads();
-			for (int c = 0; c + 1 < nch; ++c) {
-				issue(a0, b0, lim0);
-				compute(buf);
-				la.store(As[buf ^ 1], a0, tid, lim0);
-				lb.store(Bs[buf ^ 1], b0, tid, lim0);
-				__syncthreads();
-				buf ^= 1;
-			}
-			compute(buf);
+	const int it_end = bins ? bins[blockIdx.x + 1] : 1;
+	for (int it = bins ? bins[blockIdx.x] : 0; it < it_end; ++it) {
+		Tile tl;
+		int  skip = 0, nch_item = -1, slot = -1; // chunks to skip, chunks to run (-1: all), slot
+		if (bins) {
+			const SkItem w = items[it];
+			tl             = Tile { w.task, w.m0, w.n0, 0 };
+			skip           = w.c0;
+			nch_item       = w.nch;
+			slot           = w.slot;
 		} else {
-			// Two chunks ahead, two register sets: at the top of each iteration, LDS[buf] holds
-			// chunk c and set 1 holds chunk c+1 (or a repeat of the last chunk); the loads of
-			// chunk c+2 go to set 0 and are stored one chunk later.
-			vec_t a1[LA::NV], b1[LB::NV];
-			int   lim1;
-			issue(a0, b0, lim0);
-			issue(a1, b1, lim1);
-			la.store(As[0], a0, tid, lim0);
-			lb.store(Bs[0], b0, tid, lim0);
-			__syncthreads();
-			int c = 0;
-			for (; c + 2 < nch; c += 2) {
-				issue(a0, b0, lim0);
-				compute(buf);
-				la.store(As[buf ^ 1], a1, tid, lim1);
-				lb.store(Bs[buf ^ 1], b1, tid, lim1);
-				__syncthreads();
-				buf ^= 1;
-				issue(a1, b1, lim1);
-				compute(buf);
-				la.store(As[buf ^ 1], a0, tid, lim0);
-				lb.store(Bs[buf ^ 1], b0, tid, lim0);
-				__syncthreads();
-				buf ^= 1;
-			}
-			// One or two chunks remain; the second one is in set 1.
-			compute(buf);
-			if (c + 1 < nch) {
-				la.store(As[buf ^ 1], a1, tid, lim1);
-				lb.store(Bs[buf ^ 1], b1, tid, lim1);
-				__syncthreads();
-				compute(buf ^ 1);
-			}
+			tl = tiles[blockIdx.x];
 		}
-	}
+		if (tl.task < 0)
+			continue; // padding slot of the XCD-aware order
+		const PTask tk = tasks[tl.task];
 
-	const bool to_y = tk.out_is_y != 0;
+		acc_t acc[FI][FJ];
 #pragma unroll
-	for (int i = 0; i < FI; ++i)
+		for (int i = 0; i < FI; ++i)
 #pragma unroll
-		for (int j = 0; j < FJ; ++j)
-#pragma unroll
-			for (int r = 0; r < Op::kRegs; ++r) {
-				const int row = tl.m0 + wm + Op::kM * i + Op::row(lane, r);
-				const int col = tl.n0 + wn + Op::kM * j + lr;
-				if (row < tk.m && col < tk.n) {
-					const int64_t addr = out_addr(tk, row, col);
-					if (to_y) {
-						TO v = TO(acc[i][j][r]);
-						if (beta)
-							v += out_y[addr];
-						out_y[addr] = v;
-					} else {
-						out_ws[addr] = acc[i][j][r];
+			for (int j = 0; j < FJ; ++j)
+				acc[i][j] = acc_t {};
+
+		// The item's first chunk: `skip` whole chunks into the task's segments (with k > 0).
+		int seg = tk.seg_begin;
+		while (seg < tk.seg_end && segs[seg].k <= 0)
+			++seg;
+		int kstart = 0;
+		while (seg < tk.seg_end) {
+			const int nc = (segs[seg].k + BK - 1) / BK;
+			if (skip < nc) {
+				kstart = skip * BK;
+				break;
+			}
+			skip -= nc;
+			do
+				++seg;
+			while (seg < tk.seg_end && segs[seg].k <= 0);
+		}
+
+		if (seg < tk.seg_end && nch_item != 0) {
+			// Load cursor: the next chunk whose global loads are issued (segment seg, offset ck0).
+			// Loads are issued unconditionally: past the tile's last chunk (cvalid false) the last
+			// chunk is loaded again and never stored (past an item's last chunk, the task's next
+			// chunk is loaded and never stored).
+			LA   la;
+			LB   lb;
+			PSeg s      = segs[seg];
+			int  ck0    = kstart;
+			int  lastk  = kstart;
+			bool cvalid = true;
+			la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
+			lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
+			auto issue = [&](vec_t (&xa)[LA::NV], vec_t (&xb)[LB::NV], int& lim) {
+				const int kk = cvalid ? ck0 : lastk;
+				la.load(xa, kk, s.k, tid);
+				lb.load(xb, kk, s.k, tid);
+				lim   = s.k - kk;
+				lastk = kk;
+				if (cvalid) {
+					bool newseg;
+					cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
+					if (cvalid && newseg) {
+						s = segs[seg];
+						la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
+						lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
 					}
 				}
+			};
+			auto compute = [&](int b) {
+				// Fragments: NF consecutive k of one LDS row per operand block.
+				T fa[FI][NF], fb[FJ][NF];
+#pragma unroll
+				for (int i = 0; i < FI; ++i) {
+					const int r = wm + Op::kM * i + lr;
+#pragma unroll
+					for (int v = 0; v < NF / VW; ++v) {
+						const vec_t x = As[b][r * VPR + LA::slot(r, g * (NF / VW) + v)];
+#pragma unroll
+						for (int e = 0; e < VW; ++e)
+							fa[i][v * VW + e] = x[e];
+					}
+				}
+#pragma unroll
+				for (int j = 0; j < FJ; ++j) {
+					const int r = wn + Op::kM * j + lr;
+#pragma unroll
+					for (int v = 0; v < NF / VW; ++v) {
+						const vec_t x = Bs[b][r * VPR + LB::slot(r, g * (NF / VW) + v)];
+#pragma unroll
+						for (int e = 0; e < VW; ++e)
+							fb[j][v * VW + e] = x[e];
+					}
+				}
+#pragma unroll
+				for (int st = 0; st < NF; ++st)
+#pragma unroll
+					for (int i = 0; i < FI; ++i)
+#pragma unroll
+						for (int j = 0; j < FJ; ++j)
+							acc[i][j] = Op::mma(fa[i][st], fb[j][st], acc[i][j]);
+			};
+			vec_t a0[LA::NV], b0[LB::NV];
+			int   lim0, buf = 0;
+			// The loops are counted and leave only at the top, so that every path into the loop
+			// header has the same loads in flight and the compiler waits for one register set only.
+			const int nch =
+			    nch_item > 0 ? nch_item : chunk_count<BK>(segs, seg, tk.seg_end) - kstart / BK;
+			if constexpr (PF == 1) {
+				// One chunk ahead: the loads of chunk c+1 fly during the MFMAs of chunk c.
+				issue(a0, b0, lim0);
+				la.store(As[0], a0, tid, lim0);
+				lb.store(Bs[0], b0, tid, lim0);
+				__syncthreads();
+				for (int c = 0; c + 1 < nch; ++c) {
+					issue(a0, b0, lim0);
+					compute(buf);
+					la.store(As[buf ^ 1], a0, tid, lim0);
+					lb.store(Bs[buf ^ 1], b0, tid, lim0);
+					__syncthreads();
+					buf ^= 1;
+				}
+				compute(buf);
+			} else {
+				// Two chunks ahead, two register sets: at the top of each iteration, LDS[buf] holds
+				// chunk c and set 1 holds chunk c+1 (or a repeat of the last chunk); the loads of
+				// chunk c+2 go to set 0 and are stored one chunk later.
+				vec_t a1[LA::NV], b1[LB::NV];
+				int   lim1;
+				issue(a0, b0, lim0);
+				issue(a1, b1, lim1);
+				la.store(As[0], a0, tid, lim0);
+				lb.store(Bs[0], b0, tid, lim0);
+				__syncthreads();
+				int c = 0;
+				for (; c + 2 < nch; c += 2) {
+					issue(a0, b0, lim0);
+					compute(buf);
+					la.store(As[buf ^ 1], a1, tid, lim1);
+					lb.store(Bs[buf ^ 1], b1, tid, lim1);
+					__syncthreads();
+					buf ^= 1;
+					issue(a1, b1, lim1);
+					compute(buf);
+					la.store(As[buf ^ 1], a0, tid, lim0);
+					lb.store(Bs[buf ^ 1], b0, tid, lim0);
+					__syncthreads();
+					buf ^= 1;
+				}
+				// One or two chunks remain; the second one is in set 1.
+				compute(buf);
+				if (c + 1 < nch) {
+					la.store(As[buf ^ 1], a1, tid, lim1);
+					lb.store(Bs[buf ^ 1], b1, tid, lim1);
+					__syncthreads();
+					compute(buf ^ 1);
+				}
 			}
+		}
+
+		if (slot >= 0) {
+			// A partial of a tile cut between workgroups: the whole BM x BN tile, tile-local.
+			T* part = out_ws + int64_t(slot) * (BM * BN);
+#pragma unroll
+			for (int i = 0; i < FI; ++i)
+#pragma unroll
+				for (int j = 0; j < FJ; ++j)
+#pragma unroll
+					for (int r = 0; r < Op::kRegs; ++r)
+						part[(wm + Op::kM * i + Op::row(lane, r)) * BN + wn + Op::kM * j + lr] =
+						    acc[i][j][r];
+		} else {
+			const bool to_y = tk.out_is_y != 0;
+#pragma unroll
+			for (int i = 0; i < FI; ++i)
+#pragma unroll
+				for (int j = 0; j < FJ; ++j)
+#pragma unroll
+					for (int r = 0; r < Op::kRegs; ++r) {
+						const int row = tl.m0 + wm + Op::kM * i + Op::row(lane, r);
+						const int col = tl.n0 + wn + Op::kM * j + lr;
+						if (row < tk.m && col < tk.n) {
+							const int64_t addr = out_addr(tk, row, col);
+							if (to_y) {
+								TO v = TO(acc[i][j][r]);
+								if (beta)
+									v += out_y[addr];
+								out_y[addr] = v;
+							} else {
+								out_ws[addr] = acc[i][j][r];
+							}
+						}
+					}
+		}
+		// The next item's first stores go to LDS buffer 0, which the last chunk may have used.
+		__syncthreads();
+	}
 }
 
 // BF16x3: every operand value v is held as hi = bf16(v), lo = bf16(v - hi) in two arrays of the
@@ -458,6 +509,7 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 template <int BM, int BN, int BK, int PF, bool PAIR, typename Op>
 __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
+    const SkItem* __restrict__ items, const int32_t* __restrict__ bins,
     PBasesT<__bf16> hi, PBasesT<__bf16> lo, PStrides vs, double* __restrict__ out_y,
     int64_t ystride, float* __restrict__ out_f, __bf16* __restrict__ out_h,
     __bf16* __restrict__ out_l, int64_t wsstride, int beta)
@@ -476,12 +528,8 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 	__shared__ bf8_t Bh[2][BN * VPR];
 	__shared__ bf8_t Bl[2][BN * VPR];
 
-	const Tile tl = tiles[blockIdx.x];
-	if (tl.task < 0)
-		return;
-	const PTask tk  = tasks[tl.task];
-	const int   tid = threadIdx.x;
-	const int   vec = blockIdx.y;
+	const int tid = threadIdx.x;
+	const int vec = blockIdx.y;
 
 	const int64_t vo[PBASE_COUNT] = { 0, 0, vec * vs.xp, vec * vs.xt, vec * vs.bx };
 	auto pick = [&](const PBasesT<__bf16>& b, int id) -> const __bf16* {
@@ -504,160 +552,234 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 	const int g    = lane / Op::kM;
 	const int lr   = lane % Op::kM;
 
-	acc_t acc[FI][FJ];
-#pragma unroll
-	for (int i = 0; i < FI; ++i)
-#pragma unroll
-		for (int j = 0; j < FJ; ++j)
-			acc[i][j] = acc_t {};
+	const int it_end = bins ? bins[blockIdx.x + 1] : 1;
+	for (int it = bins ? bins[blockIdx.x] : 0; it < it_end; ++it) {
+		// The work item, as in panel_gemm_kernel.
+		Tile tl;
+		int  skip = 0, nch_item = -1, slot = -1;
+		if (bins) {
+			const SkItem w = items[it];
+			tl             = Tile { w.task, w.m0, w.n0, 0 };
+			skip           = w.c0;
+			nch_item       = w.nch;
+			slot           = w.slot;
+		} else {
+			tl = tiles[blockIdx.x];
+		}
+		if (tl.task < 0)
+			continue;
+		const PTask tk = tasks[tl.task];
 
-	int seg = tk.seg_begin;
-	while (seg < tk.seg_end && segs[seg].k <= 0)
-		++seg;
+		acc_t acc[FI][FJ];
+#pragma unroll
+		for (int i = 0; i < FI; ++i)
+#pragma unroll
+			for (int j = 0; j < FJ; ++j)
+				acc[i][j] = acc_t {};
 
-	if (seg < tk.seg_end) {
-		// One register set: the hi and lo vectors of both operands of one chunk, and the
-		// chunk's extent inside its segment.
-		struct Set {
-			bf8_t ah[LA::NV], al[LA::NV], bh[LB::NV], bl[LB::NV];
-			int   lim;
-		};
-		LA   lah, lal;
-		LB   lbh, lbl;
-		PSeg s      = segs[seg];
-		int  ck0    = 0;
-		int  lastk  = 0;
-		bool cvalid = true;
-		auto setup  = [&](const PSeg& sg) {
-                        lah.setup(pick(hi, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
-                        lal.setup(pick(lo, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
-                        lbh.setup(pick(hi, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
-                        lbl.setup(pick(lo, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
-		};
-		// Unconditional loads of the cursor's chunk (past the last chunk the last one again,
-		// never stored), then the cursor moves on.
-		auto issue = [&](Set& x) {
-			const int kk = cvalid ? ck0 : lastk;
-			lah.load(x.ah, kk, s.k, tid);
-			lal.load(x.al, kk, s.k, tid);
-			lbh.load(x.bh, kk, s.k, tid);
-			lbl.load(x.bl, kk, s.k, tid);
-			x.lim = s.k - kk;
-			lastk = kk;
-			if (cvalid) {
-				bool newseg;
-				cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
-				if (cvalid && newseg) {
-					s = segs[seg];
-					setup(s);
-				}
+		int seg = tk.seg_begin;
+		while (seg < tk.seg_end && segs[seg].k <= 0)
+			++seg;
+		int kstart = 0;
+		while (seg < tk.seg_end) {
+			const int nc = (segs[seg].k + BK - 1) / BK;
+			if (skip < nc) {
+				kstart = skip * BK;
+				break;
 			}
-		};
-		auto store = [&](int b, const Set& x) {
-			lah.store(Ah[b], x.ah, tid, x.lim);
-			lal.store(Al[b], x.al, tid, x.lim);
-			lbh.store(Bh[b], x.bh, tid, x.lim);
-			lbl.store(Bl[b], x.bl, tid, x.lim);
-		};
-		auto compute = [&](int b) {
-#pragma unroll
-			for (int st = 0; st < NST; ++st) {
-				// Step st covers k in [st * kK, st * kK + kK): lane group g the 8 at 8 g.
-				const int kv = st * NG + g;
-				bf8_t     ah[FI], al[FI], bh[FJ], bl[FJ];
-#pragma unroll
-				for (int i = 0; i < FI; ++i) {
-					const int r = wm + Op::kM * i + lr;
-					ah[i]       = Ah[b][r * VPR + LA::slot(r, kv)];
-					al[i]       = Al[b][r * VPR + LA::slot(r, kv)];
+			skip -= nc;
+			do
+				++seg;
+			while (seg < tk.seg_end && segs[seg].k <= 0);
+		}
+
+		if (seg < tk.seg_end && nch_item != 0) {
+			// One register set: the hi and lo vectors of both operands of one chunk, and the
+			// chunk's extent inside its segment.
+			struct Set {
+				bf8_t ah[LA::NV], al[LA::NV], bh[LB::NV], bl[LB::NV];
+				int   lim;
+			};
+			LA   lah, lal;
+			LB   lbh, lbl;
+			PSeg s      = segs[seg];
+			int  ck0    = kstart;
+			int  lastk  = kstart;
+			bool cvalid = true;
+			auto setup  = [&](const PSeg& sg) {
+	                        lah.setup(pick(hi, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
+	                        lal.setup(pick(lo, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
+	                        lbh.setup(pick(hi, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
+	                        lbl.setup(pick(lo, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
+			};
+			// Unconditional loads of the cursor's chunk (past the last chunk the last one again,
+			// never stored), then the cursor moves on.
+			auto issue = [&](Set& x) {
+				const int kk = cvalid ? ck0 : lastk;
+				lah.load(x.ah, kk, s.k, tid);
+				lal.load(x.al, kk, s.k, tid);
+				lbh.load(x.bh, kk, s.k, tid);
+				lbl.load(x.bl, kk, s.k, tid);
+				x.lim = s.k - kk;
+				lastk = kk;
+				if (cvalid) {
+					bool newseg;
+					cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
+					if (cvalid && newseg) {
+						s = segs[seg];
+						setup(s);
+					}
 				}
+			};
+			auto store = [&](int b, const Set& x) {
+				lah.store(Ah[b], x.ah, tid, x.lim);
+				lal.store(Al[b], x.al, tid, x.lim);
+				lbh.store(Bh[b], x.bh, tid, x.lim);
+				lbl.store(Bl[b], x.bl, tid, x.lim);
+			};
+			auto compute = [&](int b) {
 #pragma unroll
-				for (int j = 0; j < FJ; ++j) {
-					const int r = wn + Op::kM * j + lr;
-					bh[j]       = Bh[b][r * VPR + LB::slot(r, kv)];
-					bl[j]       = Bl[b][r * VPR + LB::slot(r, kv)];
-				}
+				for (int st = 0; st < NST; ++st) {
+					// Step st covers k in [st * kK, st * kK + kK): lane group g the 8 at 8 g.
+					const int kv = st * NG + g;
+					bf8_t     ah[FI], al[FI], bh[FJ], bl[FJ];
 #pragma unroll
-				for (int i = 0; i < FI; ++i)
+					for (int i = 0; i < FI; ++i) {
+						const int r = wm + Op::kM * i + lr;
+						ah[i]       = Ah[b][r * VPR + LA::slot(r, kv)];
+						al[i]       = Al[b][r * VPR + LA::slot(r, kv)];
+					}
 #pragma unroll
 					for (int j = 0; j < FJ; ++j) {
-						acc[i][j] = Op::mma(al[i], bh[j], acc[i][j]);
-						acc[i][j] = Op::mma(ah[i], bl[j], acc[i][j]);
-						acc[i][j] = Op::mma(ah[i], bh[j], acc[i][j]);
+						const int r = wn + Op::kM * j + lr;
+						bh[j]       = Bh[b][r * VPR + LB::slot(r, kv)];
+						bl[j]       = Bl[b][r * VPR + LB::slot(r, kv)];
 					}
-			}
-		};
-		setup(s);
-		// Counted loops, left only at the top (see panel_gemm_kernel).
-		const int nch = chunk_count<BK>(segs, seg, tk.seg_end);
-		Set       x0;
-		int       buf = 0;
-		if constexpr (PF == 1) {
-			issue(x0);
-			store(0, x0);
-			__syncthreads();
-			for (int c = 0; c + 1 < nch; ++c) {
+#pragma unroll
+					for (int i = 0; i < FI; ++i)
+#pragma unroll
+						for (int j = 0; j < FJ; ++j) {
+							acc[i][j] = Op::mma(al[i], bh[j], acc[i][j]);
+							acc[i][j] = Op::mma(ah[i], bl[j], acc[i][j]);
+							acc[i][j] = Op::mma(ah[i], bh[j], acc[i][j]);
+						}
+				}
+			};
+			setup(s);
+			// Counted loops, left only at the top (see panel_gemm_kernel).
+			const int nch =
+			    nch_item > 0 ? nch_item : chunk_count<BK>(segs, seg, tk.seg_end) - kstart / BK;
+			Set       x0;
+			int       buf = 0;
+			if constexpr (PF == 1) {
 				issue(x0);
-				compute(buf);
-				store(buf ^ 1, x0);
+				store(0, x0);
 				__syncthreads();
-				buf ^= 1;
-			}
-			compute(buf);
-		} else {
-			Set x1;
-			issue(x0);
-			issue(x1);
-			store(0, x0);
-			__syncthreads();
-			int c = 0;
-			for (; c + 2 < nch; c += 2) {
+				for (int c = 0; c + 1 < nch; ++c) {
+					issue(x0);
+					compute(buf);
+					store(buf ^ 1, x0);
+					__syncthreads();
+					buf ^= 1;
+				}
+				compute(buf);
+			} else {
+				Set x1;
 				issue(x0);
-				compute(buf);
-				store(buf ^ 1, x1);
-				__syncthreads();
-				buf ^= 1;
 				issue(x1);
+				store(0, x0);
+				__syncthreads();
+				int c = 0;
+				for (; c + 2 < nch; c += 2) {
+					issue(x0);
+					compute(buf);
+					store(buf ^ 1, x1);
+					__syncthreads();
+					buf ^= 1;
+					issue(x1);
+					compute(buf);
+					store(buf ^ 1, x0);
+					__syncthreads();
+					buf ^= 1;
+				}
 				compute(buf);
-				store(buf ^ 1, x0);
-				__syncthreads();
-				buf ^= 1;
-			}
-			compute(buf);
-			if (c + 1 < nch) {
-				store(buf ^ 1, x1);
-				__syncthreads();
-				compute(buf ^ 1);
-			}
-		}
-	}
-
-	const bool to_y = tk.out_is_y != 0;
-#pragma unroll
-	for (int i = 0; i < FI; ++i)
-#pragma unroll
-		for (int j = 0; j < FJ; ++j)
-#pragma unroll
-			for (int r = 0; r < Op::kRegs; ++r) {
-				const int row = tl.m0 + wm + Op::kM * i + Op::row(lane, r);
-				const int col = tl.n0 + wn + Op::kM * j + lr;
-				if (row < tk.m && col < tk.n) {
-					const int64_t addr = out_addr(tk, row, col);
-					const float   v    = acc[i][j][r];
-					if (to_y) {
-						double y = double(v);
-						if (beta)
-							y += out_y[addr];
-						out_y[addr] = y;
-					} else if (PAIR) {
-						const __bf16 h = __bf16(v);
-						out_h[addr]    = h;
-						out_l[addr]    = __bf16(v - float(h));
-					} else {
-						out_f[addr] = v;
-					}
+				if (c + 1 < nch) {
+					store(buf ^ 1, x1);
+					__syncthreads();
+					compute(buf ^ 1);
 				}
 			}
+		}
+
+		if (slot >= 0) {
+			// A partial of a tile cut between workgroups (FP32, tile-local).
+			float* part = out_f + int64_t(slot) * (BM * BN);
+#pragma unroll
+			for (int i = 0; i < FI; ++i)
+#pragma unroll
+				for (int j = 0; j < FJ; ++j)
+#pragma unroll
+					for (int r = 0; r < Op::kRegs; ++r)
+						part[(wm + Op::kM * i + Op::row(lane, r)) * BN + wn + Op::kM * j + lr] =
+						    acc[i][j][r];
+		} else {
+			const bool to_y = tk.out_is_y != 0;
+#pragma unroll
+			for (int i = 0; i < FI; ++i)
+#pragma unroll
+				for (int j = 0; j < FJ; ++j)
+#pragma unroll
+					for (int r = 0; r < Op::kRegs; ++r) {
+						const int row = tl.m0 + wm + Op::kM * i + Op::row(lane, r);
+						const int col = tl.n0 + wn + Op::kM * j + lr;
+						if (row < tk.m && col < tk.n) {
+							const int64_t addr = out_addr(tk, row, col);
+							const float   v    = acc[i][j][r];
+							if (to_y) {
+								double y = double(v);
+								if (beta)
+									y += out_y[addr];
+								out_y[addr] = y;
+							} else if (PAIR) {
+								const __bf16 h = __bf16(v);
+								out_h[addr]    = h;
+								out_l[addr]    = __bf16(v - float(h));
+							} else {
+								out_f[addr] = v;
+							}
+						}
+					}
+		}
+		__syncthreads(); // LDS buffer 0 is reused by the next item
+	}
+}
+
+// Fix-up of a balanced launch: the partials of each tile cut between workgroups, summed in slot
+// order (the workgroups' order, so the result does not depend on timing) in FP64 into the
+// task's output (the caller's vector; beta adds to it). One workgroup per cut tile and vector.
+template <typename T>
+__global__ __launch_bounds__(256) void sk_fixup_kernel(const SkFix* __restrict__ fixes,
+                                                       const PTask* __restrict__ tasks,
+                                                       const T* __restrict__ part,
+                                                       int64_t wsstride, double* __restrict__ out_y,
+                                                       int64_t ystride, int bm, int bn, int beta)
+{
+	const SkFix   f   = fixes[blockIdx.x];
+	const PTask   tk  = tasks[f.task];
+	const int64_t tsz = int64_t(bm) * bn;
+	part += int64_t(blockIdx.y) * wsstride + int64_t(f.first) * tsz;
+	out_y += int64_t(blockIdx.y) * ystride;
+	for (int e = threadIdx.x; e < bm * bn; e += blockDim.x) {
+		const int row = f.m0 + e / bn;
+		const int col = f.n0 + e % bn;
+		if (row >= tk.m || col >= tk.n)
+			continue;
+		double v = 0.0;
+		for (int q = 0; q < f.count; ++q)
+			v += double(part[q * tsz + e]);
+		const int64_t addr = out_addr(tk, row, col);
+		out_y[addr]        = beta ? v + out_y[addr] : v;
+	}
 }
 
 // Input preparation, one 64 x 64 tile of one patch per workgroup: X_J (R x L, column major at
@@ -780,24 +902,30 @@ template <typename T, typename TO>
 void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const PTask* tasks,
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
                        const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
-                       int64_t wsstride, int nvec, int beta, hipStream_t stream)
+                       int64_t wsstride, int nvec, int beta, hipStream_t stream,
+                       const PanelLaunchExtra& ex)
 {
-	if (ntiles <= 0 || nvec <= 0)
+	if (!ex.occupancy && (ntiles <= 0 || nvec <= 0))
 		return;
 	PBasesT<T> b;
 	for (int i = 0; i < PBASE_COUNT; ++i)
-		b.p[i] = bases[i];
-	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
+		b.p[i] = bases ? bases[i] : nullptr;
+	const dim3 grid { unsigned(std::max(ntiles, 1)), unsigned(std::max(nvec, 1)), 1u };
+	auto go = [&](auto kern) {
+		if (ex.occupancy) {
+			DMK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(ex.occupancy, kern,
+			                                                           kPThreads, 0));
+			return;
+		}
+		hipLaunchKernelGGL(kern, grid, dim3(kPThreads), 0, stream, tasks, segs, tiles, ex.items,
+		                   ex.bins, b, vs, out_y, ystride, out_ws, wsstride, beta);
+	};
 #define DMK_PLAUNCH(BM, BN, BK, OP)                                                                \
 	do {                                                                                       \
 		if (depth == 2)                                                                    \
-			hipLaunchKernelGGL((panel_gemm_kernel<TO, BM, BN, BK, 2, OP>), grid,         \
-			                   dim3(kPThreads), 0, stream, tasks, segs, tiles, b, vs,    \
-			                   out_y, ystride, out_ws, wsstride, beta);                  \
+			go(panel_gemm_kernel<TO, BM, BN, BK, 2, OP>);                             \
 		else                                                                               \
-			hipLaunchKernelGGL((panel_gemm_kernel<TO, BM, BN, BK, 1, OP>), grid,         \
-			                   dim3(kPThreads), 0, stream, tasks, segs, tiles, b, vs,    \
-			                   out_y, ystride, out_ws, wsstride, beta);                  \
+			go(panel_gemm_kernel<TO, BM, BN, BK, 1, OP>);                             \
 	} while (0)
 #define DMK_PSHAPES(BK, OP)                                                                        \
 	if (bm == 64 && bn == 64)                                                                  \
@@ -893,15 +1021,18 @@ void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const
 template void launch_panel_gemm<double, double>(int, int, int, int, int, const PTask*, const PSeg*,
                                                 const Tile*, int, const double* const*,
                                                 const PStrides&, double*, int64_t, double*,
-                                                int64_t, int, int, hipStream_t);
+                                                int64_t, int, int, hipStream_t,
+                                                const PanelLaunchExtra&);
 template void launch_panel_gemm<float, float>(int, int, int, int, int, const PTask*, const PSeg*,
                                               const Tile*, int, const float* const*,
                                               const PStrides&, float*, int64_t, float*, int64_t,
-                                              int, int, hipStream_t);
+                                              int, int, hipStream_t,
+                                                const PanelLaunchExtra&);
 template void launch_panel_gemm<float, double>(int, int, int, int, int, const PTask*, const PSeg*,
                                                const Tile*, int, const float* const*,
                                                const PStrides&, double*, int64_t, float*, int64_t,
-                                               int, int, hipStream_t);
+                                               int, int, hipStream_t,
+                                                const PanelLaunchExtra&);
 
 // variant 1 selects the 32x32x16 bf16 MFMA form; depth 2 loads two chunks ahead (32x32x16
 // form only, the 16x16x32 form runs one ahead).
@@ -910,22 +1041,29 @@ void launch_panel_bf16x3(int bm, int bn, int variant, int depth, const PTask* ta
                          const Tile* tiles, int ntiles, const void* const* bases_hi,
                          const void* const* bases_lo, const PStrides& vs, double* out_y,
                          int64_t ystride, float* out_f, void* out_h, void* out_l, int64_t wsstride,
-                         bool pair, int nvec, int beta, hipStream_t stream)
+                         bool pair, int nvec, int beta, hipStream_t stream,
+                         const PanelLaunchExtra& ex)
 {
-	if (ntiles <= 0 || nvec <= 0)
+	if (!ex.occupancy && (ntiles <= 0 || nvec <= 0))
 		return;
 	PBasesT<__bf16> h, l;
 	for (int i = 0; i < PBASE_COUNT; ++i) {
-		h.p[i] = static_cast<const __bf16*>(bases_hi[i]);
-		l.p[i] = static_cast<const __bf16*>(bases_lo[i]);
+		h.p[i] = bases_hi ? static_cast<const __bf16*>(bases_hi[i]) : nullptr;
+		l.p[i] = bases_lo ? static_cast<const __bf16*>(bases_lo[i]) : nullptr;
 	}
 	__bf16*    oh = static_cast<__bf16*>(out_h);
 	__bf16*    ol = static_cast<__bf16*>(out_l);
-	const dim3 grid { unsigned(ntiles), unsigned(nvec), 1u };
-#define DMK_BLAUNCH_PF(BM, BN, PF, PAIR, OP)                                                       \
-	hipLaunchKernelGGL((panel_bf16x3_kernel<BM, BN, 32, PF, PAIR, OP>), grid, dim3(kPThreads),  \
-	                   0, stream, tasks, segs, tiles, h, l, vs, out_y, ystride, out_f, oh, ol,   \
-	                   wsstride, beta)
+	const dim3 grid { unsigned(std::max(ntiles, 1)), unsigned(std::max(nvec, 1)), 1u };
+	auto go = [&](auto kern) {
+		if (ex.occupancy) {
+			DMK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(ex.occupancy, kern,
+			                                                           kPThreads, 0));
+			return;
+		}
+		hipLaunchKernelGGL(kern, grid, dim3(kPThreads), 0, stream, tasks, segs, tiles, ex.items,
+		                   ex.bins, h, l, vs, out_y, ystride, out_f, oh, ol, wsstride, beta);
+	};
+#define DMK_BLAUNCH_PF(BM, BN, PF, PAIR, OP) go(panel_bf16x3_kernel<BM, BN, 32, PF, PAIR, OP>)
 #define DMK_BLAUNCH(BM, BN, PAIR, OP)                                                              \
 	do {                                                                                       \
 		if (depth == 2 && OP::kM == 32)                                                    \
@@ -961,6 +1099,21 @@ void launch_panel_bf16x3(int bm, int bn, int variant, int depth, const PTask* ta
 #undef DMK_BLAUNCH
 #undef DMK_BLAUNCH_PF
 }
+
+template <typename T>
+void launch_sk_fixup(const SkFix* fixes, int nfix, const PTask* tasks, const T* part,
+                     int64_t wsstride, double* out_y, int64_t ystride, int bm, int bn, int nvec,
+                     int beta, hipStream_t stream)
+{
+	if (nfix <= 0 || nvec <= 0)
+		return;
+	hipLaunchKernelGGL((sk_fixup_kernel<T>), dim3(unsigned(nfix), unsigned(nvec)), dim3(256), 0,
+	                   stream, fixes, tasks, part, wsstride, out_y, ystride, bm, bn, beta);
+}
+template void launch_sk_fixup<double>(const SkFix*, int, const PTask*, const double*, int64_t,
+                                      double*, int64_t, int, int, int, int, hipStream_t);
+template void launch_sk_fixup<float>(const SkFix*, int, const PTask*, const float*, int64_t,
+                                     double*, int64_t, int, int, int, int, hipStream_t);
 
 template <typename E>
 void launch_prep_x(const PrepTile* tiles, int ntiles, const PrepPatch* patches, const double* x,

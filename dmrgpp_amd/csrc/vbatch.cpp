@@ -30,7 +30,8 @@ template <typename T, typename TO>
 void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const PTask* tasks,
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
                        const PStrides& vs, TO* out_y, int64_t ystride, T* out_ws,
-                       int64_t wsstride, int nvec, int beta, hipStream_t stream);
+                       int64_t wsstride, int nvec, int beta, hipStream_t stream,
+                       const PanelLaunchExtra& ex = PanelLaunchExtra {});
 
 namespace {
 

@@ -98,6 +98,11 @@ typedef struct dmk_tuning {
 	int32_t kchunk;            /* K chunk depth: 1 (0 = 1) = 16 (FP64) / 32 (FP32, BF16x3) elements
 	                              per LDS stage, 2 = twice that (FP64 / FP32, tiles with
 	                              tile_m + tile_n <= 128 in both stages) */
+	int32_t persist;           /* k = 1..8: stage 2 as one balanced persistent launch of k
+	                              workgroups per CU (at most as many as stay resident), each given
+	                              an equal share of the stage's K chunks (tiles cut between
+	                              workgroups are summed in a fixed order afterwards; splitk is
+	                              then ignored); 0: one tile per workgroup */
 } dmk_tuning;
 
 /* Replaces BatchedGemm<double>::BatchedGemm(noperator, npatches, left_patch_size,

@@ -239,6 +239,18 @@ TUNINGS = [
          xcd_order=1, mfma32=1, prefetch=2, kchunk=2),
     dict(stage1_tile_m=32, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=32, splitk=0,
          xcd_order=0, mfma32=0, prefetch=1, kchunk=2),
+    # The balanced persistent stage 2 (on these small problems most tiles are cut between
+    # workgroups, so nearly every output goes through the partials and the fix-up).
+    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=64, stage2_tile_n=64, splitk=0,
+         xcd_order=1, mfma32=1, prefetch=2, persist=8),
+    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=128, stage2_tile_n=128, splitk=0,
+         xcd_order=0, mfma32=0, prefetch=1, persist=1),
+    dict(stage1_tile_m=64, stage1_tile_n=32, stage2_tile_m=128, stage2_tile_n=64, splitk=0,
+         xcd_order=1, mfma32=1, prefetch=1, persist=2),
+    dict(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=96, stage2_tile_n=64, splitk=0,
+         xcd_order=1, mfma32=0, prefetch=2, persist=3),
+    dict(stage1_tile_m=32, stage1_tile_n=32, stage2_tile_m=32, stage2_tile_n=64, splitk=0,
+         xcd_order=0, mfma32=0, prefetch=2, kchunk=2, persist=5),
 ]
 
 
@@ -265,9 +277,11 @@ def test_gpu_tunings_match_oracle(tuning, prec, tol):
         want = oracle.plugin_apply(p, x)
         op = dm.BatchedGemm(p, prec)
         op.set_tuning(**tuning)
-        assert op.tuning() == dict({"kchunk": 1}, **tuning)
+        assert op.tuning() == dict({"kchunk": 1, "persist": 0}, **tuning)
         if tuning["splitk"]:
             assert op.info()["split_tasks"] > 0
+        if tuning.get("persist"):
+            assert op.info()["split_tasks"] == 0 and op.info()["ws_elems"] > 0
         dx, dy = dm.DeviceArray.from_host(x), dm.DeviceArray(p.n)
         op.apply_Htarget(dx, dy)
         op.sync()
@@ -281,6 +295,37 @@ def test_gpu_tunings_match_oracle(tuning, prec, tol):
         op.sync()
         assert rel_err(dy.download(), 2 * want) < tol
         op.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_gpu_balanced_launch_multi_vector(prec):
+    # Several vectors in one balanced persistent launch (each vector's partials in its own part
+    # of the workspace) equal the vectors one at a time, bitwise, with beta = 0 and 1.
+    left = {-2: 70, 0: 130, 2: 70}
+    prob = problems.qn_problem(left, left, 0, problems.heisenberg_links(2), seed=21)
+    op = dm.BatchedGemm(prob, prec)
+    op.set_tuning(stage1_tile_m=64, stage1_tile_n=64, stage2_tile_m=128, stage2_tile_n=64,
+                  splitk=0, xcd_order=1, mfma32=0, prefetch=1, persist=8)
+    nvec, n = 3, prob.n
+    X = np.random.default_rng(4).uniform(-1, 1, nvec * n)
+    Y0 = np.random.default_rng(5).uniform(-1, 1, nvec * n)
+    dX, dY = dm.DeviceArray.from_host(X), dm.DeviceArray.from_host(Y0)
+    op.apply_multi(dX, dY, nvec, beta=1)
+    op.sync()
+    multi = dY.download()
+    for v in range(nvec):
+        dx = dm.DeviceArray.from_host(X[v * n:(v + 1) * n])
+        dy = dm.DeviceArray.from_host(Y0[v * n:(v + 1) * n])
+        op.apply_multi(dx, dy, 1, beta=1)
+        op.sync()
+        assert np.array_equal(dy.download(), multi[v * n:(v + 1) * n]), v
+        want = Y0[v * n:(v + 1) * n] + oracle.plugin_apply(prob, X[v * n:(v + 1) * n])
+        tol = {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 1e-4}[prec]
+        assert rel_err(multi[v * n:(v + 1) * n], want) < tol
+        dx.free()
+        dy.free()
+    op.close()
 
 
 @pytest.mark.gpu
