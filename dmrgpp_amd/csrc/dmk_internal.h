@@ -172,9 +172,15 @@ struct DiagJob {
 	int64_t       count;
 };
 
-// Rows of K-major panels are padded to this many elements (16 bytes of bf16).
-constexpr int kPanelPad = 8;
+// Rows of K-major panels are zero padded to a multiple of this many elements, and split-k pieces
+// start on multiples of it: every K chunk of a segment (at most 64 elements, from a multiple of
+// its size) then lies inside the segment's rows, which the LDS-DMA loads of the FP64 / FP32
+// panel kernel rely on (they have no masks).
+constexpr int kPanelPad = 64;
 inline int64_t panel_ld(int64_t k) { return (k + kPanelPad - 1) / kPanelPad * kPanelPad; }
+// The panel kernels address the rows of one tile (at most 128) by 32-bit byte offsets from the
+// tile's first row: leading dimensions up to 2^21 elements keep them below 2^31.
+inline bool panel_ld_fits(int64_t ld) { return ld >= 0 && ld <= (int64_t(1) << 21); }
 
 // Error reporting for the C ABI.
 void        set_last_error(const std::string& msg);

@@ -518,7 +518,7 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 		                        hipMemcpyDeviceToHost));
 	}
 
-	// Packing, K-major (rows padded to kPanelPad elements, zero padding):
+	// Packing, K-major (rows zero padded to a multiple of kPanelPad elements):
 	//  * A panel of patch I, the stage-2 left operand of Y_I^T: row n (< L_I) holds the K_I
 	//    columns (J outer, k inner, as the reference's A panels) of its general terms;
 	//  * identity-B terms: the A block row major (stage-2 left operand);
@@ -704,6 +704,10 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 	stats_.bpack_elems  = bsize;
 	if (stats_.n > (int64_t(1) << 31) * 64)
 		throw std::runtime_error("dmk_create: superblock too large");
+	for (const auto* v : { &t1_segs_, &t2_segs_ })
+		for (const PSeg& s : *v)
+			if (!panel_ld_fits(s.ldl) || !panel_ld_fits(s.ldr))
+				throw std::runtime_error("dmk_create: a block row is too long for the panel kernels");
 
 	DMK_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 	for (auto& e : ev_)
@@ -981,7 +985,8 @@ void Plan::retune(const Tuning& t)
 
 	// Stage 2: tasks with a long contraction are cut into nsplit pieces along K; each piece
 	// writes a partial of the whole output patch to the workspace, and a reduction sums the
-	// pieces in order. Piece boundaries are multiples of the 16-deep K chunk.
+	// pieces in order. Piece boundaries are multiples of kPanelPad, so that every piece's K
+	// chunks lie inside its rows.
 	st2_.release();
 	st2_.bm = t.bm2;
 	st2_.bn = t.bn2;
@@ -993,12 +998,12 @@ void Plan::retune(const Tuning& t)
 	stats_.split_tasks     = 0;
 	for (size_t ti = 0; ti < t2_tasks_.size(); ++ti) {
 		const PTask& tk = t2_tasks_[ti];
-		// The task's segments laid end to end along K, each starting at a multiple of 16, so that
-		// cuts at multiples of 16 fall at multiples of 16 within every segment (piece rows stay
-		// 16-byte aligned and every piece but a segment's last ends on a whole vector).
+		// The task's segments laid end to end along K, each starting at a multiple of kPanelPad,
+		// so that cuts at multiples of kPanelPad fall at multiples of kPanelPad within every
+		// segment (every piece but a segment's last is a whole number of K chunks long).
 		int64_t ktot = 0;
 		for (int s = tk.seg_begin; s < tk.seg_end; ++s)
-			ktot = (ktot + 15) / 16 * 16 + t2_segs_[size_t(s)].k;
+			ktot = panel_ld(ktot) + t2_segs_[size_t(s)].k;
 		int nsplit = 1; // (the balanced launch cuts tiles itself)
 		if (!t.persist && t.splitk > 0 && ktot >= 2 * int64_t(t.splitk))
 			nsplit = int(std::min<int64_t>(16, ktot / t.splitk));
@@ -1022,10 +1027,10 @@ void Plan::retune(const Tuning& t)
 		rd.count  = count;
 		rd.nsplit = nsplit;
 		reduce_.push_back(rd);
-		// Cut the laid-out K range [0, ktot) at multiples of 16.
+		// Cut the laid-out K range [0, ktot) at multiples of kPanelPad.
 		std::vector<int64_t> cuts(size_t(nsplit) + 1, 0);
 		for (int p = 1; p < nsplit; ++p)
-			cuts[size_t(p)] = ((ktot * p / nsplit) / 16) * 16;
+			cuts[size_t(p)] = ((ktot * p / nsplit) / kPanelPad) * kPanelPad;
 		cuts[size_t(nsplit)] = ktot;
 		for (int p = 0; p < nsplit; ++p) {
 			PTask c     = tk;
@@ -1037,7 +1042,7 @@ void Plan::retune(const Tuning& t)
 			int64_t end = 0; // end of the previous segment in the layout
 			for (int s = tk.seg_begin; s < tk.seg_end; ++s) {
 				const PSeg&   sg   = t2_segs_[size_t(s)];
-				const int64_t base = (end + 15) / 16 * 16; // this segment's start
+				const int64_t base = panel_ld(end); // this segment's start
 				const int64_t a    = std::max(cuts[size_t(p)], base);
 				const int64_t b  = std::min(cuts[size_t(p) + 1], base + sg.k);
 				if (b > a) {

@@ -11,20 +11,24 @@
 // This is the two-stage evaluation of GPUPlugin/apply_Htarget_sparse_not_ready.cpp:199-449 with
 // the identity factors of H_L (x) I and I (x) H_R contracted directly.
 //
-// Every operand is K-major with rows padded to a multiple of 8 elements and zero padding:
+// Every operand is K-major with rows zero padded to a multiple of kPanelPad (64) elements:
 // element (m, k) of a left operand at loff + k + m * ldl, element (k, n) of a right operand at
 // roff + k + n * ldr. The planner packs the factors that way, the input vector is rewritten that
 // way on every call (prep_x_kernel: X_J column major with padded columns, and X_J transposed),
-// and stage 1 writes BX_I that way. So every global load is a 16-byte vector along k, every LDS
-// store and fragment read a 16-byte vector, and the only edge handling is a row clamp (rows past
-// the task edge are computed and dropped) and a vector mask at the end of a segment's K range.
+// and stage 1 writes BX_I that way; split-k pieces start on multiples of kPanelPad. So every K
+// chunk of a segment (BK <= 64 elements from a multiple of BK) lies inside its rows, every
+// global load is a 16-byte vector along k, every LDS store and fragment read a 16-byte vector,
+// and the only edge handling is a row clamp (rows past the task edge are computed and dropped).
 //
 // Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a BM x BN output tile and
 // each wave a (BM/2) x (BN/2) sub-tile of MFMA blocks. K is walked in chunks of BK staged
-// through LDS, double buffered: the global loads of chunk c+1 (PF = 1), or of c+1 and c+2 into
-// two register sets (PF = 2), are in flight during the MFMAs of chunk c. The loads carry no
-// branches and the chunk loops are counted and leave only at the top, so that the compiler can
-// count the loads in flight and wait only for the register set an LDS store needs.
+// through LDS. FP64 / FP32 (panel_gemm_kernel): the chunks move from global memory straight into
+// a ring of two or three LDS stages (LDS-DMA), with no staging registers and no address
+// arithmetic per chunk, and chunk c+1 (or c+1 and c+2) is in flight during the MFMAs of chunk
+// c. BF16x3 (panel_bf16x3_kernel): the loads of chunk c+1 go to registers during the MFMAs of
+// chunk c and are stored to the other LDS buffer after them; the loads carry no branches and the
+// chunk loops are counted and leave only at the top, so that the compiler can count the loads
+// in flight and wait only for the register set an LDS store needs.
 // LDS rows hold the BK elements of one operand row as 16-byte slots, XOR-swizzled by
 // row (slot kv of row r at kv ^ swz(r)) so that the 16 rows one 16-lane pass reads fall on
 // distinct banks, without padding.
@@ -147,6 +151,18 @@ __device__ inline int64_t out_addr(const PTask& tk, int r, int c)
 	return tk.coff + int64_t(rr) * tk.sm + int64_t(blk) * tk.n + c;
 }
 
+// Element i of a descriptor array, read through the constant address space: scalar loads, which
+// an LDS-DMA load in flight (a vector memory operation that writes memory) does not make wait.
+template <typename S> __device__ inline S load_const(const S* p, int i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	typedef const __attribute__((address_space(4))) S* cptr;
+	return ((cptr)p)[i];
+#else
+	return p[i]; // the host pass only checks the kernels
+#endif
+}
+
 // Advances (seg, k0) to the next K chunk of the task, with K the current segment's length (held
 // in registers, so the common step reads no descriptor); false past the last chunk. newseg is
 // set when the chunk starts a new segment.
@@ -162,7 +178,7 @@ __device__ inline bool next_chunk(const PSeg* __restrict__ segs, int seg_end, in
 	newseg = true;
 	do
 		++seg;
-	while (seg < seg_end && segs[seg].k <= 0);
+	while (seg < seg_end && load_const(segs, seg).k <= 0);
 	return seg < seg_end;
 }
 
@@ -172,7 +188,7 @@ __device__ inline int chunk_count(const PSeg* __restrict__ segs, int seg_begin, 
 {
 	int n = 0;
 	for (int q = seg_begin; q < seg_end; ++q)
-		n += max(0, (segs[q].k + BK - 1) / BK);
+		n += max(0, (load_const(segs, q).k + BK - 1) / BK);
 	return n;
 }
 
@@ -250,16 +266,92 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 	}
 };
 
+// LDS address-space view of a __shared__ pointer (the destination operand of an LDS-DMA load).
+[[maybe_unused]] __device__ inline __attribute__((address_space(3))) void* lds_ptr(void* p)
+{
+	return (__attribute__((address_space(3))) void*)p;
+}
+
+// s_waitcnt vmcnt(N) lgkmcnt(0): this wave's vector memory operations but the last N issued have
+// completed, and its LDS reads too (gfx9 encoding: vmcnt bits [3:0] and [15:14], expcnt [6:4],
+// lgkmcnt [11:8]).
+template <int N> __device__ inline void wait_vm_lgkm0()
+{
+	static_assert(N >= 0 && N < 64, "vmcnt range");
+	__builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+}
+
+// A workgroup barrier without the memory fence of __syncthreads (which would wait for every
+// LDS-DMA load in flight, the prefetched chunks included); the memory clobber keeps the
+// compiler from moving LDS accesses across it. Callers wait for what they need first.
+__device__ inline void barrier_no_fence() { asm volatile("s_barrier" ::: "memory"); }
+
+// Direct global -> LDS loads (LDS-DMA, global_load_lds_dwordx4) of one operand tile of ROWS x
+// BK elements. The 16-byte LDS slots of the tile are filled 64 at a time: instruction q of wave
+// w writes slots (4 q + w) * 64 ... + 63, lane l slot (4 q + w) * 64 + l, and the slot's row r
+// and vector kv follow from the XOR swizzle of TileLoad (slot kv ^ f(r) of row r holds vector
+// kv; the map is an involution per row). Every lane's source is the uniform segment base plus
+// a per-lane 32-bit offset fixed for the segment, so a chunk costs no vector ALU work. Rows past
+// the task edge are clamped to its last row (computed and dropped). Every chunk lies inside its
+// rows: rows are zero padded to a multiple of kPanelPad >= BK elements and split-k cuts fall on
+// multiples of kPanelPad.
+template <typename E, int ROWS, int BK> struct TileDma {
+	using TL                 = TileLoad<E, ROWS, BK>;
+	using vec_t              = typename TL::vec_t;
+	static constexpr int VW  = TL::VW;
+	static constexpr int VPR = TL::VPR;
+	static constexpr int NI  = ROWS * VPR / kPThreads; // instructions per wave per chunk
+	static_assert(NI >= 1 && ROWS * VPR % kPThreads == 0, "tile / thread mismatch");
+	static_assert(kPanelPad % BK == 0, "chunks must fit the row padding");
+	const E* base;    // the segment's operand at the tile's first row (wave-uniform)
+	uint32_t off[NI]; // byte offsets of the lane's vectors from base
+
+	__device__ inline void setup(const E* seg, int ld, int row0, int nrows, int lane, int wave)
+	{
+		base = seg + int64_t(row0) * ld;
+#pragma unroll
+		for (int q = 0; q < NI; ++q) {
+			const int s  = (q * 4 + wave) * 64 + lane;
+			const int r  = s / VPR;
+			const int kv = TL::slot(r, s % VPR);
+			const int dr = min(row0 + r, nrows - 1) - row0;
+			off[q]       = uint32_t((dr * ld + kv * VW) * int(sizeof(E)));
+		}
+	}
+	__device__ inline void dma(vec_t* stage, int k0, int wave) const
+	{
+		const char* b = reinterpret_cast<const char*>(base + k0);
+#pragma unroll
+		for (int q = 0; q < NI; ++q) {
+			// The builtin exists for the device only; the host pass sees the kernel's signature.
+#if defined(__HIP_DEVICE_COMPILE__)
+			__builtin_amdgcn_global_load_lds(b + off[q], lds_ptr(stage + (q * 4 + wave) * 64), 16,
+			                                 0, 0);
+#else
+			(void)b;
+			(void)stage;
+#endif
+		}
+	}
+};
+
 // FP64 / FP32: T operands, TO output for tasks writing the caller's vector (the FP32 mode writes
 // its stage-2 output in FP64 for the FP64 Lanczos recurrence). Workspace tasks (stage-1 panels,
 // split-k partials) store T and always overwrite.
+//
+// The K chunks go through a ring of NS LDS stages filled by LDS-DMA: with NS = 2 chunk c+1 lands
+// during the MFMAs of chunk c, with NS = 3 chunks c+1 and c+2. Each wave waits for its own loads
+// of the next chunk (vmcnt) and then meets the others at a barrier, after which the chunk is in
+// LDS for all of them; a stage is refilled only after the barrier that follows its last reads.
+// The stages are distinct __shared__ arrays and the ring is unrolled NS times, so that the
+// compiler can tell the stage being read from the stages being filled.
 //
 // With bins != nullptr the launch is persistent and balanced: workgroup b runs the items
 // [bins[b], bins[b + 1]) one after another (SkItem: a chunk range of one tile, its result to the
 // task's output or to a partial slot of out_ws for the fix-up); else workgroup b runs tile b
 // whole.
-template <typename TO, int BM, int BN, int BK, int PF, typename Op>
-__global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK == 32 ? 5 : 1)) void panel_gemm_kernel(
+template <typename TO, int BM, int BN, int BK, int NS, typename Op>
+__global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
     const SkItem* __restrict__ items, const int32_t* __restrict__ bins,
     PBasesT<typename Op::T> bases, PStrides vs, TO* __restrict__ out_y, int64_t ystride,
@@ -267,18 +359,22 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 {
 	using T     = typename Op::T;
 	using acc_t = typename Op::acc_t;
-	using LA    = TileLoad<T, BM, BK>;
-	using LB    = TileLoad<T, BN, BK>;
+	using LA    = TileDma<T, BM, BK>;
+	using LB    = TileDma<T, BN, BK>;
+	using SA    = TileLoad<T, BM, BK>;
+	using SB    = TileLoad<T, BN, BK>;
 	using vec_t = typename LA::vec_t;
 	constexpr int VW  = LA::VW;
 	constexpr int VPR = LA::VPR; // 16-byte slots per LDS row
 	constexpr int WM = BM / 2, WN = BN / 2;
 	constexpr int FI = WM / Op::kM, FJ = WN / Op::kM;
-	constexpr int NF = BK / Op::kK; // consecutive k per lane per chunk
+	constexpr int NF = BK / Op::kK;          // consecutive k per lane per chunk
+	constexpr int NW = LA::NI + LB::NI;      // LDS-DMA instructions per wave per chunk
 	static_assert(NF % VW == 0, "fragment / vector mismatch");
+	static_assert(NS == 2 || NS == 3, "two or three stages");
 
-	__shared__ vec_t As[2][BM * VPR];
-	__shared__ vec_t Bs[2][BN * VPR];
+	__shared__ vec_t A0[BM * VPR], B0[BN * VPR], A1[BM * VPR], B1[BN * VPR];
+	__shared__ vec_t A2[NS == 3 ? BM * VPR : 1], B2[NS == 3 ? BN * VPR : 1];
 
 	const int tid = threadIdx.x;
 	const int vec = blockIdx.y;
@@ -295,7 +391,7 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 	out_ws += vec * wsstride;
 
 	const int lane = tid & 63;
-	const int wave = tid >> 6;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int wm   = (wave & 1) * WM;
 	const int wn   = (wave >> 1) * WN;
 	const int g    = lane / Op::kM; // lane group: k slice g * NF .. g * NF + NF - 1
@@ -316,7 +412,7 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 		}
 		if (tl.task < 0)
 			continue; // padding slot of the XCD-aware order
-		const PTask tk = tasks[tl.task];
+		const PTask tk = load_const(tasks, tl.task);
 
 		acc_t acc[FI][FJ];
 #pragma unroll
@@ -327,11 +423,11 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 
 		// The item's first chunk: `skip` whole chunks into the task's segments (with k > 0).
 		int seg = tk.seg_begin;
-		while (seg < tk.seg_end && segs[seg].k <= 0)
+		while (seg < tk.seg_end && load_const(segs, seg).k <= 0)
 			++seg;
 		int kstart = 0;
 		while (seg < tk.seg_end) {
-			const int nc = (segs[seg].k + BK - 1) / BK;
+			const int nc = (load_const(segs, seg).k + BK - 1) / BK;
 			if (skip < nc) {
 				kstart = skip * BK;
 				break;
@@ -339,127 +435,120 @@ __global__ __launch_bounds__(kPThreads, (BM * BN <= 4096 && Op::kM == 32 && BK =
 			skip -= nc;
 			do
 				++seg;
-			while (seg < tk.seg_end && segs[seg].k <= 0);
+			while (seg < tk.seg_end && load_const(segs, seg).k <= 0);
 		}
 
 		if (seg < tk.seg_end && nch_item != 0) {
-			// Load cursor: the next chunk whose global loads are issued (segment seg, offset ck0).
-			// Loads are issued unconditionally: past the tile's last chunk (cvalid false) the last
-			// chunk is loaded again and never stored (past an item's last chunk, the task's next
-			// chunk is loaded and never stored).
+			// Load cursor: the next chunk to fetch (segment seg, offset ck0). Past the tile's last
+			// chunk the last one is fetched again, into a stage that is not read any more, so
+			// that every step has the same loads in flight (past an item's last chunk, the
+			// task's next chunk is fetched and never read).
 			LA   la;
 			LB   lb;
-			PSeg s      = segs[seg];
+			PSeg s      = load_const(segs, seg);
 			int  ck0    = kstart;
 			int  lastk  = kstart;
 			bool cvalid = true;
-			la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
-			lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
-			auto issue = [&](vec_t (&xa)[LA::NV], vec_t (&xb)[LB::NV], int& lim) {
+			la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, lane, wave);
+			lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, lane, wave);
+			auto issue = [&](vec_t* sa, vec_t* sb) {
 				const int kk = cvalid ? ck0 : lastk;
-				la.load(xa, kk, s.k, tid);
-				lb.load(xb, kk, s.k, tid);
-				lim   = s.k - kk;
+				la.dma(sa, kk, wave);
+				lb.dma(sb, kk, wave);
 				lastk = kk;
 				if (cvalid) {
 					bool newseg;
 					cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
 					if (cvalid && newseg) {
-						s = segs[seg];
-						la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, tid);
-						lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, tid);
+						s = load_const(segs, seg);
+						la.setup(pick(s.lbase) + s.loff, s.ldl, tl.m0, tk.m, lane, wave);
+						lb.setup(pick(s.rbase) + s.roff, s.ldr, tl.n0, tk.n, lane, wave);
 					}
 				}
 			};
-			auto compute = [&](int b) {
-				// Fragments: NF consecutive k of one LDS row per operand block.
-				T fa[FI][NF], fb[FJ][NF];
+			auto compute = [&](const vec_t* sa, const vec_t* sb) {
+				// A lane's NF consecutive k of one LDS row per operand block, one 16-byte vector
+				// (VW of the NF MFMA steps) at a time, so that only VW steps of fragments are live.
 #pragma unroll
-				for (int i = 0; i < FI; ++i) {
-					const int r = wm + Op::kM * i + lr;
+				for (int v = 0; v < NF / VW; ++v) {
+					vec_t xa[FI], xb[FJ];
 #pragma unroll
-					for (int v = 0; v < NF / VW; ++v) {
-						const vec_t x = As[b][r * VPR + LA::slot(r, g * (NF / VW) + v)];
-#pragma unroll
-						for (int e = 0; e < VW; ++e)
-							fa[i][v * VW + e] = x[e];
+					for (int i = 0; i < FI; ++i) {
+						const int r = wm + Op::kM * i + lr;
+						xa[i]       = sa[r * VPR + SA::slot(r, g * (NF / VW) + v)];
 					}
-				}
 #pragma unroll
-				for (int j = 0; j < FJ; ++j) {
-					const int r = wn + Op::kM * j + lr;
-#pragma unroll
-					for (int v = 0; v < NF / VW; ++v) {
-						const vec_t x = Bs[b][r * VPR + LB::slot(r, g * (NF / VW) + v)];
-#pragma unroll
-						for (int e = 0; e < VW; ++e)
-							fb[j][v * VW + e] = x[e];
+					for (int j = 0; j < FJ; ++j) {
+						const int r = wn + Op::kM * j + lr;
+						xb[j]       = sb[r * VPR + SB::slot(r, g * (NF / VW) + v)];
 					}
+#pragma unroll
+					for (int e = 0; e < VW; ++e)
+#pragma unroll
+						for (int i = 0; i < FI; ++i)
+#pragma unroll
+							for (int j = 0; j < FJ; ++j)
+								acc[i][j] = Op::mma(T(xa[i][e]), T(xb[j][e]), acc[i][j]);
 				}
-#pragma unroll
-				for (int st = 0; st < NF; ++st)
-#pragma unroll
-					for (int i = 0; i < FI; ++i)
-#pragma unroll
-						for (int j = 0; j < FJ; ++j)
-							acc[i][j] = Op::mma(fa[i][st], fb[j][st], acc[i][j]);
 			};
-			vec_t a0[LA::NV], b0[LB::NV];
-			int   lim0, buf = 0;
-			// The loops are counted and leave only at the top, so that every path into the loop
-			// header has the same loads in flight and the compiler waits for one register set only.
 			const int nch =
 			    nch_item > 0 ? nch_item : chunk_count<BK>(segs, seg, tk.seg_end) - kstart / BK;
-			if constexpr (PF == 1) {
-				// One chunk ahead: the loads of chunk c+1 fly during the MFMAs of chunk c.
-				issue(a0, b0, lim0);
-				la.store(As[0], a0, tid, lim0);
-				lb.store(Bs[0], b0, tid, lim0);
-				__syncthreads();
-				for (int c = 0; c + 1 < nch; ++c) {
-					issue(a0, b0, lim0);
-					compute(buf);
-					la.store(As[buf ^ 1], a0, tid, lim0);
-					lb.store(Bs[buf ^ 1], b0, tid, lim0);
-					__syncthreads();
-					buf ^= 1;
-				}
-				compute(buf);
-			} else {
-				// Two chunks ahead, two register sets: at the top of each iteration, LDS[buf] holds
-				// chunk c and set 1 holds chunk c+1 (or a repeat of the last chunk); the loads of
-				// chunk c+2 go to set 0 and are stored one chunk later.
-				vec_t a1[LA::NV], b1[LB::NV];
-				int   lim1;
-				issue(a0, b0, lim0);
-				issue(a1, b1, lim1);
-				la.store(As[0], a0, tid, lim0);
-				lb.store(Bs[0], b0, tid, lim0);
-				__syncthreads();
+			if constexpr (NS == 2) {
+				// At the top of each step stage 0 holds chunk c; chunk c+1 goes to stage 1.
+				issue(A0, B0);
+				wait_vm_lgkm0<0>();
+				barrier_no_fence();
 				int c = 0;
-				for (; c + 2 < nch; c += 2) {
-					issue(a0, b0, lim0);
-					compute(buf);
-					la.store(As[buf ^ 1], a1, tid, lim1);
-					lb.store(Bs[buf ^ 1], b1, tid, lim1);
-					__syncthreads();
-					buf ^= 1;
-					issue(a1, b1, lim1);
-					compute(buf);
-					la.store(As[buf ^ 1], a0, tid, lim0);
-					lb.store(Bs[buf ^ 1], b0, tid, lim0);
-					__syncthreads();
-					buf ^= 1;
+				for (; c + 2 <= nch; c += 2) {
+					issue(A1, B1);
+					compute(A0, B0);
+					wait_vm_lgkm0<0>();
+					barrier_no_fence();
+					if (c + 2 == nch) {
+						compute(A1, B1);
+						break;
+					}
+					issue(A0, B0);
+					compute(A1, B1);
+					wait_vm_lgkm0<0>();
+					barrier_no_fence();
 				}
-				// One or two chunks remain; the second one is in set 1.
-				compute(buf);
-				if (c + 1 < nch) {
-					la.store(As[buf ^ 1], a1, tid, lim1);
-					lb.store(Bs[buf ^ 1], b1, tid, lim1);
-					__syncthreads();
-					compute(buf ^ 1);
+				if (c + 1 == nch)
+					compute(A0, B0);
+			} else {
+				// At the top of each step stage 0 holds chunk c and chunk c+1 is in flight to
+				// stage 1; chunk c+2 goes to stage 2, which was last read one step earlier.
+				issue(A0, B0);
+				issue(A1, B1);
+				wait_vm_lgkm0<NW>();
+				barrier_no_fence();
+				int c = 0;
+				for (; c + 3 <= nch; c += 3) {
+					issue(A2, B2);
+					compute(A0, B0);
+					wait_vm_lgkm0<NW>();
+					barrier_no_fence();
+					issue(A0, B0);
+					compute(A1, B1);
+					wait_vm_lgkm0<NW>();
+					barrier_no_fence();
+					issue(A1, B1);
+					compute(A2, B2);
+					wait_vm_lgkm0<NW>();
+					barrier_no_fence();
+				}
+				// Zero, one or two chunks remain, in stages 0 and 1.
+				if (c < nch) {
+					compute(A0, B0);
+					if (c + 1 < nch) {
+						wait_vm_lgkm0<0>();
+						barrier_no_fence();
+						compute(A1, B1);
+					}
 				}
 			}
+			// Every load has landed before the stages are reused or the workgroup ends.
+			wait_vm_lgkm0<0>();
 		}
 
 		if (slot >= 0) {
@@ -896,8 +985,8 @@ void launch_split_f64_bf16x2(const double* x, void* hi, void* lo, int64_t n, hip
 
 // Host-side launchers -------------------------------------------------------------------------
 
-// kchunk 2 doubles the K chunk (FP64 32, FP32 64 elements per LDS stage: twice the MFMAs per
-// barrier, 64 KB of LDS for tiles with bm + bn <= 128).
+// depth 1: two LDS stages (one chunk in flight), depth 2: three (two chunks in flight). kchunk 2
+// doubles the K chunk (FP64 32, FP32 64 elements per stage: twice the MFMAs per barrier).
 template <typename T, typename TO>
 void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const PTask* tasks,
                        const PSeg* segs, const Tile* tiles, int ntiles, const T* const* bases,
@@ -923,9 +1012,9 @@ void launch_panel_gemm(int bm, int bn, int variant, int depth, int kchunk, const
 #define DMK_PLAUNCH(BM, BN, BK, OP)                                                                \
 	do {                                                                                       \
 		if (depth == 2)                                                                    \
-			go(panel_gemm_kernel<TO, BM, BN, BK, 2, OP>);                             \
+			go(panel_gemm_kernel<TO, BM, BN, BK, 3, OP>);                             \
 		else                                                                               \
-			go(panel_gemm_kernel<TO, BM, BN, BK, 1, OP>);                             \
+			go(panel_gemm_kernel<TO, BM, BN, BK, 2, OP>);                             \
 	} while (0)
 #define DMK_PSHAPES(BK, OP)                                                                        \
 	if (bm == 64 && bn == 64)                                                                  \

@@ -6,8 +6,8 @@
 // kernel (two for complex scalars), after its operands are staged into one workspace in the
 // panel kernel's K-major layout: the task computes C^T = op(B)^T op(A)^T, whose left operand is
 // op(B) column major (each column contiguous along k) and whose right operand is op(A)^T column
-// major (each row of op(A) contiguous along k), both with leading dimensions padded to 8 and
-// zero filled; C^T written by rows is C by columns. Staging makes transposes, conjugation and
+// major (each row of op(A) contiguous along k), both with leading dimensions padded to a
+// multiple of kPanelPad and zero filled; C^T written by rows is C by columns. Staging makes transposes, conjugation and
 // complex scalars a property of the copy, so the GEMM launch only ever sees real K-major
 // operands, addressed as offsets into one allocation. Complex products run as real ones:
 //   Re C = Ar Br + (-Ai) Bi,   Im C = Ar Bi + Ai Br   (two segments per task).
@@ -63,7 +63,7 @@ int trans_code(char t, const char* which)
 constexpr int64_t kChunk = 8192; // elements per staging / scatter workgroup (32 per thread)
 
 // Appends copy jobs staging op(X) (rows x cols) from src into the workspace at dst, column
-// major with the leading dimension padded to 8 (panel_ld(rows)) and the padding zero filled.
+// major with the leading dimension padded (panel_ld(rows)) and the padding zero filled.
 // Returns the staged size in elements.
 int64_t stage(std::vector<VbCopy>& jobs, const double* src, int ld, int rows, int cols,
               int64_t dst, int flags)
@@ -258,6 +258,9 @@ void gemm_vbatch(bool cplx, char transa, char transb, const int* m, const int* n
 	if (tiles.size() > size_t(0x7fffffff) || copies.size() > size_t(0x7fffffff)
 	    || outs.size() > size_t(0x7fffffff))
 		throw std::runtime_error("gemm_vbatch: batch too large for one launch");
+	for (const PSeg& s : segs)
+		if (!panel_ld_fits(s.ldl) || !panel_ld_fits(s.ldr))
+			throw std::runtime_error("gemm_vbatch: a contraction is too long for the panel kernel");
 
 	std::vector<void*> owned;
 	try {

@@ -258,7 +258,8 @@ TUNINGS = [
 @pytest.mark.parametrize("tuning", TUNINGS)
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-4)])
 def test_gpu_tunings_match_oracle(tuning, prec, tol):
-    # Every tile shape (64 / 128, 96-wide, 32-wide), forced split-k (pieces of 16-48 along K, ragged last piece), both
+    # Every tile shape (64 / 128, 96-wide, 32-wide), forced split-k (splitk 16-64: cuts on
+    # multiples of 64 along K, so some pieces are empty and the last is ragged), both
     # workgroup orders and both FP32 MFMA forms give the oracle's result; the same tuning twice
     # is bitwise identical.
     shapes = (tuning["stage1_tile_m"], tuning["stage1_tile_n"], tuning["stage2_tile_m"],
