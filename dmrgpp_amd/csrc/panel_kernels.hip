@@ -594,8 +594,9 @@ __global__ __launch_bounds__(kPThreads) void panel_gemm_kernel(
 // same layout, and a product is lo_a*hi_b + hi_a*lo_b + hi_a*hi_b on bf16 MFMAs accumulating in
 // FP32 (lo*lo, ~2^-16 relative, is dropped). PAIR: workspace tasks store a (hi, lo) pair
 // (stage-1 panels) instead of FP32 (split-k partials). Stage-2 output to the caller is FP64.
-// PF: chunks loaded ahead (1 or 2), as in panel_gemm_kernel.
-template <int BM, int BN, int BK, int PF, bool PAIR, typename Op>
+// The K chunks (hi and lo of both operands) go through a ring of NS LDS stages filled by
+// LDS-DMA, as in panel_gemm_kernel.
+template <int BM, int BN, int BK, int NS, bool PAIR, typename Op>
 __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
     const PTask* __restrict__ tasks, const PSeg* __restrict__ segs, const Tile* __restrict__ tiles,
     const SkItem* __restrict__ items, const int32_t* __restrict__ bins,
@@ -604,18 +605,25 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
     __bf16* __restrict__ out_l, int64_t wsstride, int beta)
 {
 	using acc_t = typename Op::acc_t;
-	using LA    = TileLoad<__bf16, BM, BK>;
-	using LB    = TileLoad<__bf16, BN, BK>;
+	using LA    = TileDma<__bf16, BM, BK>;
+	using LB    = TileDma<__bf16, BN, BK>;
+	using SA    = TileLoad<__bf16, BM, BK>;
+	using SB    = TileLoad<__bf16, BN, BK>;
 	constexpr int VPR = LA::VPR;
 	constexpr int WM = BM / 2, WN = BN / 2;
 	constexpr int FI = WM / Op::kM, FJ = WN / Op::kM;
-	constexpr int NST = BK / Op::kK;     // MFMA steps per chunk
-	constexpr int NG  = 64 / Op::kM;     // lane groups, 8 consecutive k each per step
+	constexpr int NST = BK / Op::kK;         // MFMA steps per chunk
+	constexpr int NG  = 64 / Op::kM;         // lane groups, 8 consecutive k each per step
+	constexpr int NW  = 2 * (LA::NI + LB::NI); // LDS-DMA instructions per wave per chunk
+	static_assert(NS == 2 || NS == 3, "two or three stages");
 
-	__shared__ bf8_t Ah[2][BM * VPR];
-	__shared__ bf8_t Al[2][BM * VPR];
-	__shared__ bf8_t Bh[2][BN * VPR];
-	__shared__ bf8_t Bl[2][BN * VPR];
+	// One stage: the hi and lo parts of both operand tiles of a chunk.
+	struct Stage {
+		bf8_t ah[BM * VPR], al[BM * VPR], bh[BN * VPR], bl[BN * VPR];
+	};
+	__shared__ Stage S0, S1;
+	__shared__ bf8_t S2raw[NS == 3 ? sizeof(Stage) / sizeof(bf8_t) : 1];
+	Stage& S2 = *reinterpret_cast<Stage*>(S2raw); // used only when NS == 3
 
 	const int tid = threadIdx.x;
 	const int vec = blockIdx.y;
@@ -635,7 +643,7 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 	out_l += vec * wsstride;
 
 	const int lane = tid & 63;
-	const int wave = tid >> 6;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int wm   = (wave & 1) * WM;
 	const int wn   = (wave >> 1) * WN;
 	const int g    = lane / Op::kM;
@@ -657,7 +665,7 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 		}
 		if (tl.task < 0)
 			continue;
-		const PTask tk = tasks[tl.task];
+		const PTask tk = load_const(tasks, tl.task);
 
 		acc_t acc[FI][FJ];
 #pragma unroll
@@ -667,11 +675,11 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 				acc[i][j] = acc_t {};
 
 		int seg = tk.seg_begin;
-		while (seg < tk.seg_end && segs[seg].k <= 0)
+		while (seg < tk.seg_end && load_const(segs, seg).k <= 0)
 			++seg;
 		int kstart = 0;
 		while (seg < tk.seg_end) {
-			const int nc = (segs[seg].k + BK - 1) / BK;
+			const int nc = (load_const(segs, seg).k + BK - 1) / BK;
 			if (skip < nc) {
 				kstart = skip * BK;
 				break;
@@ -679,70 +687,57 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 			skip -= nc;
 			do
 				++seg;
-			while (seg < tk.seg_end && segs[seg].k <= 0);
+			while (seg < tk.seg_end && load_const(segs, seg).k <= 0);
 		}
 
 		if (seg < tk.seg_end && nch_item != 0) {
-			// One register set: the hi and lo vectors of both operands of one chunk, and the
-			// chunk's extent inside its segment.
-			struct Set {
-				bf8_t ah[LA::NV], al[LA::NV], bh[LB::NV], bl[LB::NV];
-				int   lim;
-			};
 			LA   lah, lal;
 			LB   lbh, lbl;
-			PSeg s      = segs[seg];
+			PSeg s      = load_const(segs, seg);
 			int  ck0    = kstart;
 			int  lastk  = kstart;
 			bool cvalid = true;
 			auto setup  = [&](const PSeg& sg) {
-	                        lah.setup(pick(hi, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
-	                        lal.setup(pick(lo, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, tid);
-	                        lbh.setup(pick(hi, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
-	                        lbl.setup(pick(lo, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, tid);
+				lah.setup(pick(hi, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, lane, wave);
+				lal.setup(pick(lo, sg.lbase) + sg.loff, sg.ldl, tl.m0, tk.m, lane, wave);
+				lbh.setup(pick(hi, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, lane, wave);
+				lbl.setup(pick(lo, sg.rbase) + sg.roff, sg.ldr, tl.n0, tk.n, lane, wave);
 			};
-			// Unconditional loads of the cursor's chunk (past the last chunk the last one again,
-			// never stored), then the cursor moves on.
-			auto issue = [&](Set& x) {
+			// LDS-DMA of the cursor's chunk into stage st (past the last chunk the last one
+			// again, into a stage that is not read any more), then the cursor moves on.
+			auto issue = [&](Stage& st) {
 				const int kk = cvalid ? ck0 : lastk;
-				lah.load(x.ah, kk, s.k, tid);
-				lal.load(x.al, kk, s.k, tid);
-				lbh.load(x.bh, kk, s.k, tid);
-				lbl.load(x.bl, kk, s.k, tid);
-				x.lim = s.k - kk;
+				lah.dma(st.ah, kk, wave);
+				lal.dma(st.al, kk, wave);
+				lbh.dma(st.bh, kk, wave);
+				lbl.dma(st.bl, kk, wave);
 				lastk = kk;
 				if (cvalid) {
 					bool newseg;
 					cvalid = next_chunk<BK>(segs, tk.seg_end, s.k, seg, ck0, newseg);
 					if (cvalid && newseg) {
-						s = segs[seg];
+						s = load_const(segs, seg);
 						setup(s);
 					}
 				}
 			};
-			auto store = [&](int b, const Set& x) {
-				lah.store(Ah[b], x.ah, tid, x.lim);
-				lal.store(Al[b], x.al, tid, x.lim);
-				lbh.store(Bh[b], x.bh, tid, x.lim);
-				lbl.store(Bl[b], x.bl, tid, x.lim);
-			};
-			auto compute = [&](int b) {
+			auto compute = [&](const Stage& st) {
 #pragma unroll
-				for (int st = 0; st < NST; ++st) {
-					// Step st covers k in [st * kK, st * kK + kK): lane group g the 8 at 8 g.
-					const int kv = st * NG + g;
+				for (int stp = 0; stp < NST; ++stp) {
+					// Step stp covers k in [stp * kK, stp * kK + kK): lane group g the 8 at 8 g.
+					const int kv = stp * NG + g;
 					bf8_t     ah[FI], al[FI], bh[FJ], bl[FJ];
 #pragma unroll
 					for (int i = 0; i < FI; ++i) {
 						const int r = wm + Op::kM * i + lr;
-						ah[i]       = Ah[b][r * VPR + LA::slot(r, kv)];
-						al[i]       = Al[b][r * VPR + LA::slot(r, kv)];
+						ah[i]       = st.ah[r * VPR + SA::slot(r, kv)];
+						al[i]       = st.al[r * VPR + SA::slot(r, kv)];
 					}
 #pragma unroll
 					for (int j = 0; j < FJ; ++j) {
 						const int r = wn + Op::kM * j + lr;
-						bh[j]       = Bh[b][r * VPR + LB::slot(r, kv)];
-						bl[j]       = Bl[b][r * VPR + LB::slot(r, kv)];
+						bh[j]       = st.bh[r * VPR + SB::slot(r, kv)];
+						bl[j]       = st.bl[r * VPR + SB::slot(r, kv)];
 					}
 #pragma unroll
 					for (int i = 0; i < FI; ++i)
@@ -755,49 +750,59 @@ __global__ __launch_bounds__(kPThreads) void panel_bf16x3_kernel(
 				}
 			};
 			setup(s);
-			// Counted loops, left only at the top (see panel_gemm_kernel).
 			const int nch =
 			    nch_item > 0 ? nch_item : chunk_count<BK>(segs, seg, tk.seg_end) - kstart / BK;
-			Set       x0;
-			int       buf = 0;
-			if constexpr (PF == 1) {
-				issue(x0);
-				store(0, x0);
-				__syncthreads();
-				for (int c = 0; c + 1 < nch; ++c) {
-					issue(x0);
-					compute(buf);
-					store(buf ^ 1, x0);
-					__syncthreads();
-					buf ^= 1;
-				}
-				compute(buf);
-			} else {
-				Set x1;
-				issue(x0);
-				issue(x1);
-				store(0, x0);
-				__syncthreads();
+			if constexpr (NS == 2) {
+				issue(S0);
+				wait_vm_lgkm0<0>();
+				barrier_no_fence();
 				int c = 0;
-				for (; c + 2 < nch; c += 2) {
-					issue(x0);
-					compute(buf);
-					store(buf ^ 1, x1);
-					__syncthreads();
-					buf ^= 1;
-					issue(x1);
-					compute(buf);
-					store(buf ^ 1, x0);
-					__syncthreads();
-					buf ^= 1;
+				for (; c + 2 <= nch; c += 2) {
+					issue(S1);
+					compute(S0);
+					wait_vm_lgkm0<0>();
+					barrier_no_fence();
+					if (c + 2 == nch) {
+						compute(S1);
+						break;
+					}
+					issue(S0);
+					compute(S1);
+					wait_vm_lgkm0<0>();
+					barrier_no_fence();
 				}
-				compute(buf);
-				if (c + 1 < nch) {
-					store(buf ^ 1, x1);
-					__syncthreads();
-					compute(buf ^ 1);
+				if (c + 1 == nch)
+					compute(S0);
+			} else {
+				issue(S0);
+				issue(S1);
+				wait_vm_lgkm0<NW>();
+				barrier_no_fence();
+				int c = 0;
+				for (; c + 3 <= nch; c += 3) {
+					issue(S2);
+					compute(S0);
+					wait_vm_lgkm0<NW>();
+					barrier_no_fence();
+					issue(S0);
+					compute(S1);
+					wait_vm_lgkm0<NW>();
+					barrier_no_fence();
+					issue(S1);
+					compute(S2);
+					wait_vm_lgkm0<NW>();
+					barrier_no_fence();
+				}
+				if (c < nch) {
+					compute(S0);
+					if (c + 1 < nch) {
+						wait_vm_lgkm0<0>();
+						barrier_no_fence();
+						compute(S1);
+					}
 				}
 			}
+			wait_vm_lgkm0<0>();
 		}
 
 		if (slot >= 0) {
@@ -1123,8 +1128,8 @@ template void launch_panel_gemm<float, double>(int, int, int, int, int, const PT
                                                int, int, hipStream_t,
                                                 const PanelLaunchExtra&);
 
-// variant 1 selects the 32x32x16 bf16 MFMA form; depth 2 loads two chunks ahead (32x32x16
-// form only, the 16x16x32 form runs one ahead).
+// variant 1 selects the 32x32x16 bf16 MFMA form; depth 1: two LDS stages (one chunk in
+// flight), depth 2: three (two chunks in flight).
 void launch_panel_bf16x3(int bm, int bn, int variant, int depth, const PTask* tasks,
                          const PSeg* segs,
                          const Tile* tiles, int ntiles, const void* const* bases_hi,
@@ -1152,13 +1157,13 @@ void launch_panel_bf16x3(int bm, int bn, int variant, int depth, const PTask* ta
 		hipLaunchKernelGGL(kern, grid, dim3(kPThreads), 0, stream, tasks, segs, tiles, ex.items,
 		                   ex.bins, h, l, vs, out_y, ystride, out_f, oh, ol, wsstride, beta);
 	};
-#define DMK_BLAUNCH_PF(BM, BN, PF, PAIR, OP) go(panel_bf16x3_kernel<BM, BN, 32, PF, PAIR, OP>)
+#define DMK_BLAUNCH_PF(BM, BN, NS, PAIR, OP) go(panel_bf16x3_kernel<BM, BN, 32, NS, PAIR, OP>)
 #define DMK_BLAUNCH(BM, BN, PAIR, OP)                                                              \
 	do {                                                                                       \
-		if (depth == 2 && OP::kM == 32)                                                    \
-			DMK_BLAUNCH_PF(BM, BN, 2, PAIR, OpBf16x32);                                \
+		if (depth == 2)                                                                    \
+			DMK_BLAUNCH_PF(BM, BN, 3, PAIR, OP);                                       \
 		else                                                                               \
-			DMK_BLAUNCH_PF(BM, BN, 1, PAIR, OP);                                       \
+			DMK_BLAUNCH_PF(BM, BN, 2, PAIR, OP);                                       \
 	} while (0)
 #define DMK_BSHAPES(PAIR, OP)                                                                      \
 	if (bm == 64 && bn == 64)                                                                  \
