@@ -817,13 +817,13 @@ Plan::Plan(int64_t noperator, int64_t npatches, const size_t* Lsz, const size_t*
 }
 
 // The autotuner's usual choice per mode on the m = 2048 ladder superblocks: 64 x 64 tiles in the
-// XCD-aware order with split-k 2048 and loads two chunks ahead (FP32 on 32 x 32 MFMA blocks);
-// BF16x3 with 128 x 64 stage-2 tiles, split-k 1024.
+// XCD-aware order with split-k 2048 and two LDS stages (FP32 on 32 x 32 MFMA blocks); BF16x3
+// with 128 x 64 stage-2 tiles, split-k 1024.
 Tuning default_tuning(int precision)
 {
 	Tuning t;
 	t.splitk = 2048;
-	t.depth  = 2;
+	t.depth  = 1;
 	t.xcd    = 1;
 	if (precision == PREC_FP32)
 		t.mfma32 = 1;

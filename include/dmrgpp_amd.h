@@ -86,15 +86,15 @@ typedef struct dmk_info {
  * granularity of stage 2 (target contraction length per piece, 0 = never split; pieces are
  * summed in a fixed order, so results stay deterministic),
  * the workgroup order (0 = by decreasing task cost, 1 = XCD-aware regions), the MFMA block form
- * and the load pipeline depth (global loads issued 1 or 2 K chunks ahead; 0 means 1). */
+ * and the load pipeline depth (1 or 2 K chunks in flight from global memory to LDS; 0 means 1). */
 typedef struct dmk_tuning {
 	int32_t stage1_tile_m, stage1_tile_n;
 	int32_t stage2_tile_m, stage2_tile_n;
 	int32_t splitk;
 	int32_t xcd_order;
 	int32_t mfma32;            /* 1 = 32x32 MFMA blocks (FP32: 32x32x2, BF16x3: 32x32x16), 0 = 16x16 */
-	int32_t prefetch;          /* 1 or 2 K chunks ahead (0 = 1); BF16x3 honours 2 in the 32x32x16 form
-	                              only */
+	int32_t prefetch;          /* K chunks in flight during the MFMAs of a chunk, in a ring of
+	                              prefetch + 1 LDS stages: 1 or 2 (0 = 1) */
 	int32_t kchunk;            /* K chunk depth: 1 (0 = 1) = 16 (FP64) / 32 (FP32, BF16x3) elements
 	                              per LDS stage, 2 = twice that (FP64 / FP32, tiles with
 	                              tile_m + tile_n <= 128 in both stages) */
