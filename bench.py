@@ -43,20 +43,21 @@ LX = 32
 # copy-kernel calibration factor 2.0, WRITE by 1.0. Keyed by precision and the stage-2 launch
 # shape (tile m, tile n, splitk, xcd order) the counters were taken with; the MFMA form and the
 # load pipeline depth change the loads' timing and the instruction mix, not the bytes (the two
-# bf16x3 forms on 128x64 tiles measured within 3 %). A run whose autotuner picks another shape
-# reports None. The first three entries are the shapes of the current profile (LDS-DMA
-# kernels); the others were measured with the same method on the earlier register-staged
-# kernels, which read the same chunks.
+# bf16x3 forms on 128x64 tiles measured within 3 %). The bytes belong to one superblock,
+# TRAFFIC_SUPERBLOCK (n, patches, terms): the grown superblock can differ from run to run (the
+# order of degenerate density-matrix eigenvalues moves with roundoff), and a run on another
+# superblock, or whose autotuner picks another shape, reports None.
+TRAFFIC_SUPERBLOCK = (2858152, 15, 116)
 TRAFFIC = {
     ("fp64", (32, 32, 4096, 1)): 2.4442e9 + 0.0258e9,
+    ("fp64", (32, 32, 2048, 1)): 2.0231e9 + 0.0653e9,
+    ("fp64", (64, 32, 2048, 1)): 1.7916e9 + 0.0650e9,
     ("fp32", (64, 64, 2048, 1)): 0.5997e9 + 0.0420e9,
+    ("fp32", (64, 32, 2048, 1)): 0.7202e9 + 0.0345e9,
+    ("fp32", (32, 32, 4096, 1)): 0.7558e9 + 0.0241e9,
     ("bf16x3", (128, 64, 2048, 1)): 0.6202e9 + 0.0331e9,
-    ("fp64", (64, 64, 2048, 1)): 1.2904e9 + 0.0715e9,
-    ("fp32", (32, 32, 4096, 1)): 0.9982e9 + 0.0250e9,
-    ("bf16x3", (128, 128, 1024, 1)): 0.5675e9 + 0.0745e9,
-    ("bf16x3", (128, 64, 1024, 1)): 0.5472e9 + 0.0675e9,
-    ("fp64", (64, 64, 2048, 0)): 3.3655e9 + 0.0706e9,
-    ("fp32", (64, 64, 2048, 0)): 1.6939e9 + 0.0341e9,
+    ("bf16x3", (64, 128, 0, 1)): 0.4910e9 + 0.0233e9,
+    ("bf16x3", (128, 64, 0, 1)): 0.4953e9 + 0.0232e9,
 }
 
 
@@ -321,7 +322,9 @@ def main():
                 "bf16x3": "panel_bf16x3_kernel"}[prec]
         out = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[prec],
                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[prec], 4),
-               "traffic": TRAFFIC.get((prec, shape_key(tunings[prec]))),
+               "traffic": (TRAFFIC.get((prec, shape_key(tunings[prec])))
+                           if (prob.n, prob.npatches, len(prob.blocks)) == TRAFFIC_SUPERBLOCK
+                           else None),
                "kernel": f"{kern} ({kname})",
                "ms_per_launch": round(kms, 4), "flops_per_launch": kflops}
         if prec == "bf16x3":
