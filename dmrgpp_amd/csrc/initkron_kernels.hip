@@ -26,12 +26,18 @@ __global__ __launch_bounds__(256) void pack_panels_kernel(const PackJob* __restr
 	const PackJob  jb  = jobs[pt.job];
 	double*        dst = (jb.to_b ? bpack : apack) + jb.dst;
 	const int      tx  = threadIdx.x & 63, ty = threadIdx.x >> 6;
-	// Read the tile column by column (i fastest).
-	for (int j = ty; j < 64; j += 4) {
-		const int i  = pt.i0 + tx;
-		const int jj = pt.j0 + j;
-		t[j][tx]     = (i < jb.rows && jj < jb.cols) ? jb.src[i + int64_t(jj) * jb.ldsrc] : 0.0;
+	// Read the tile column by column (i fastest), all 16 loads of a thread in flight before the
+	// first LDS store.
+	const int ir = pt.i0 + tx;
+	double    v[16];
+#pragma unroll
+	for (int q = 0; q < 16; ++q) {
+		const int jj = pt.j0 + ty + 4 * q;
+		v[q]         = (ir < jb.rows && jj < jb.cols) ? jb.src[ir + int64_t(jj) * jb.ldsrc] : 0.0;
 	}
+#pragma unroll
+	for (int q = 0; q < 16; ++q)
+		t[ty + 4 * q][tx] = v[q];
 	__syncthreads();
 	// Write panel rows (j fastest); the tile holding the last columns also zeroes the padding.
 	const int  jend = min(jb.cols, pt.j0 + 64);

@@ -903,14 +903,22 @@ __global__ __launch_bounds__(256) void prep_x_kernel(const PrepTile* __restrict_
 	xp += v * vs.xp + pp.pofs;
 	xt += v * vs.xt + pp.tofs;
 	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-	// Column-major read of the tile (m = m0 + tx fastest), XP written the same way.
-	for (int c = ty; c < 64; c += 4) {
-		const int    m   = pt.m0 + tx;
-		const int    col = pt.c0 + c;
-		const double val = (m < pp.R && col < pp.L) ? x[m + int64_t(col) * pp.R] : 0.0;
-		t[c][tx]         = val;
+	// Column-major read of the tile (m = m0 + tx fastest), all 16 loads of a thread in flight
+	// before the first store; XP written the same way.
+	const int m = pt.m0 + tx;
+	double    val[16];
+#pragma unroll
+	for (int q = 0; q < 16; ++q) {
+		const int col = pt.c0 + ty + 4 * q;
+		val[q]        = (m < pp.R && col < pp.L) ? x[m + int64_t(col) * pp.R] : 0.0;
+	}
+#pragma unroll
+	for (int q = 0; q < 16; ++q) {
+		const int c   = ty + 4 * q;
+		const int col = pt.c0 + c;
+		t[c][tx]      = val[q];
 		if (m < pp.ldp && col < pp.L)
-			put(xp, m + int64_t(col) * pp.ldp, val);
+			put(xp, m + int64_t(col) * pp.ldp, val[q]);
 	}
 	__syncthreads();
 	// XT: c fastest.
@@ -935,16 +943,23 @@ __global__ __launch_bounds__(256) void prep_x_bf16x2_kernel(
 	x += v * ldx + pp.xoff;
 	const int64_t po = v * vs.xp + pp.pofs, to = v * vs.xt + pp.tofs;
 	const int     tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-	for (int c = ty; c < 64; c += 4) {
-		const int    m   = pt.m0 + tx;
-		const int    col = pt.c0 + c;
-		const double val = (m < pp.R && col < pp.L) ? x[m + int64_t(col) * pp.R] : 0.0;
-		t[c][tx]         = val;
+	const int     m  = pt.m0 + tx;
+	double        val[16]; // all 16 loads of a thread in flight before the first store
+#pragma unroll
+	for (int q = 0; q < 16; ++q) {
+		const int col = pt.c0 + ty + 4 * q;
+		val[q]        = (m < pp.R && col < pp.L) ? x[m + int64_t(col) * pp.R] : 0.0;
+	}
+#pragma unroll
+	for (int q = 0; q < 16; ++q) {
+		const int c   = ty + 4 * q;
+		const int col = pt.c0 + c;
+		t[c][tx]      = val[q];
 		if (m < pp.ldp && col < pp.L) {
-			const __bf16  h = __bf16(float(val));
+			const __bf16  h = __bf16(float(val[q]));
 			const int64_t i = po + m + int64_t(col) * pp.ldp;
 			xph[i]          = h;
-			xpl[i]          = __bf16(float(val - double(float(h))));
+			xpl[i]          = __bf16(float(val[q] - double(float(h))));
 		}
 	}
 	__syncthreads();
