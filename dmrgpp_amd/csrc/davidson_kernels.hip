@@ -105,6 +105,8 @@ __global__ __launch_bounds__(kDT) void multidot_kernel(const double* __restrict_
 {
 	const double* v = V + ldv * int64_t(blockIdx.y);
 	double        s = 0;
+	// Unrolled so that four pairs of loads are in flight at once (the sum order is unchanged).
+#pragma unroll 4
 	for (int64_t i = int64_t(blockIdx.x) * kDT + threadIdx.x; i < n; i += int64_t(kDBlk) * kDT)
 		s += v[i] * t[i];
 	s = dv_block_sum(s);
@@ -133,6 +135,8 @@ __global__ __launch_bounds__(kDT) void sub_combine_kernel(double* __restrict__ t
 {
 	for (int64_t i = int64_t(blockIdx.x) * kDT + threadIdx.x; i < n; i += int64_t(gridDim.x) * kDT) {
 		double s = t[i];
+		// Unrolled so that eight basis loads are in flight at once (the sum order is unchanged).
+#pragma unroll 8
 		for (int j = 0; j < k; ++j)
 			s -= c[j] * V[int64_t(j) * ldv + i];
 		t[i] = s;
@@ -150,6 +154,9 @@ __global__ __launch_bounds__(kDT) void ritz_kernel(const double* __restrict__ V,
 	double rr = 0, uu = 0;
 	for (int64_t i = int64_t(blockIdx.x) * kDT + threadIdx.x; i < n; i += int64_t(kDBlk) * kDT) {
 		double a = 0, b = 0;
+		// Unrolled so that eight pairs of basis loads are in flight at once (the sums' order is
+		// unchanged).
+#pragma unroll 8
 		for (int j = 0; j < k; ++j) {
 			a += sv[j] * V[int64_t(j) * ld + i];
 			b += sv[j] * W[int64_t(j) * ld + i];
