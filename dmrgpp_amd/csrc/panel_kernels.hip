@@ -22,13 +22,10 @@
 //
 // Tiling: 256 threads = 4 wave64s in a 2x2 grid; a workgroup owns a BM x BN output tile and
 // each wave a (BM/2) x (BN/2) sub-tile of MFMA blocks. K is walked in chunks of BK staged
-// through LDS. FP64 / FP32 (panel_gemm_kernel): the chunks move from global memory straight into
-// a ring of two or three LDS stages (LDS-DMA), with no staging registers and no address
-// arithmetic per chunk, and chunk c+1 (or c+1 and c+2) is in flight during the MFMAs of chunk
-// c. BF16x3 (panel_bf16x3_kernel): the loads of chunk c+1 go to registers during the MFMAs of
-// chunk c and are stored to the other LDS buffer after them; the loads carry no branches and the
-// chunk loops are counted and leave only at the top, so that the compiler can count the loads
-// in flight and wait only for the register set an LDS store needs.
+// through LDS. In both kernels (panel_gemm_kernel for FP64 / FP32, panel_bf16x3_kernel for the
+// hi and lo parts of BF16x3) the chunks move from global memory straight into a ring of two or
+// three LDS stages (LDS-DMA), with no staging registers and no address arithmetic per chunk,
+// and chunk c+1 (or c+1 and c+2) is in flight during the MFMAs of chunk c.
 // LDS rows hold the BK elements of one operand row as 16-byte slots, XOR-swizzled by
 // row (slot kv of row r at kv ^ swz(r)) so that the 16 rows one 16-lane pass reads fall on
 // distinct banks, without padding.
@@ -192,40 +189,13 @@ __device__ inline int chunk_count(const PSeg* __restrict__ segs, int seg_begin, 
 	return n;
 }
 
-// Per-thread load geometry of one operand tile (ROWS x BK elements, VW-element vectors): thread
-// t loads vectors t + 256 q, i.e. rows t / VPR + (256 / VPR) q at vector column t % VPR.
+// LDS layout of one operand tile (ROWS x BK elements, VW-element 16-byte vectors, VPR vectors
+// per row).
 template <typename E, int ROWS, int BK> struct TileLoad {
-	using vec_t                = typename Vec16<E>::type;
-	static constexpr int VW    = 16 / int(sizeof(E));
-	static constexpr int VPR   = BK / VW;
-	static constexpr int NV    = ROWS * VPR / kPThreads;
-	static constexpr int RSTEP = kPThreads / VPR;
-	static_assert(NV >= 1 && ROWS * VPR % kPThreads == 0, "tile / thread mismatch");
-	const E* p[NV]; // row pointers of the current segment (column kv * VW)
-
-	__device__ inline void setup(const E* base, int ld, int row0, int nrows, int tid)
-	{
-		const int r  = tid / VPR;
-		const int kv = tid % VPR;
-#pragma unroll
-		for (int q = 0; q < NV; ++q) {
-			const int row = min(row0 + r + RSTEP * q, nrows - 1); // clamp: edge rows are dropped
-			p[q]          = base + int64_t(row) * ld + kv * VW;
-		}
-	}
-	// Issues the loads of the chunk at k0 of a segment K elements long, without branches (so the
-	// compiler can count the loads in flight and wait only for the set a store needs). A vector
-	// past K is replaced by the chunk's first vector of the same row, which is inside the row, and
-	// zeroed when stored (rows are zero padded past K, so a vector that starts before K needs no
-	// mask).
-	__device__ inline void load(vec_t (&v)[NV], int k0, int K, int tid) const
-	{
-		const int kvw = (tid % VPR) * VW;
-		const int off = k0 + kvw < K ? k0 : k0 - kvw;
-#pragma unroll
-		for (int q = 0; q < NV; ++q)
-			v[q] = *reinterpret_cast<const vec_t*>(p[q] + off);
-	}
+	using vec_t              = typename Vec16<E>::type;
+	static constexpr int VW  = 16 / int(sizeof(E));
+	static constexpr int VPR = BK / VW;
+	static_assert(ROWS * VPR % kPThreads == 0, "tile / thread mismatch");
 	// Slot of vector kv of LDS row r. A 16-byte LDS access is served 8 lanes (32 banks) or 16
 	// lanes (64 banks) per pass; the fragment reads put consecutive rows on consecutive lanes,
 	// so the XOR mask f(r) keeps any 8 consecutive rows, and any 16 consecutive rows that start
@@ -242,27 +212,6 @@ template <typename E, int ROWS, int BK> struct TileLoad {
 		else
 			f = r & (VPR - 1);
 		return kv ^ f;
-	}
-	// Stores a chunk whose first lim elements lie inside its segment (lim >= BK: all of it).
-	__device__ inline void store(vec_t* __restrict__ s, const vec_t (&v)[NV], int tid,
-	                             int lim) const
-	{
-		const int r  = tid / VPR;
-		const int kv = tid % VPR;
-		if (lim >= BK) {
-#pragma unroll
-			for (int q = 0; q < NV; ++q) {
-				const int row               = r + RSTEP * q;
-				s[row * VPR + slot(row, kv)] = v[q];
-			}
-		} else {
-			const bool ok = kv * VW < lim;
-#pragma unroll
-			for (int q = 0; q < NV; ++q) {
-				const int row               = r + RSTEP * q;
-				s[row * VPR + slot(row, kv)] = ok ? v[q] : vec_t {};
-			}
-		}
 	}
 };
 
